@@ -1,0 +1,114 @@
+// Device-side data layout of one clustering context and the launchers of its
+// kernels.  A context holds M >= 1 meshes packed back to back: items (mesh
+// vertices) of mesh m occupy the global range [item_off, item_off + n_items)
+// and its clusters [clus_off, clus_off + K).  Every kernel runs over the
+// packed arrays, so a single mesh is simply the case M == 1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "acvd_math.h"
+
+namespace acvd {
+
+constexpr int kBlock = 256;
+
+struct DevMesh {
+  int64_t item_off;
+  int64_t n_items;
+  int64_t clus_off;
+  int32_t K;
+  int32_t metric;  // 0 isotropic, 1 QEM
+  int32_t level;   // QEM quadrics optimisation level
+  uint32_t seed;
+  int32_t has_indicator;
+  int32_t pad;
+  double gradation;
+  double clamp_ratio;
+  Scales scales;
+};
+
+// Small control block (device), read back by the host between pass chunks.
+struct Ctrl {
+  int32_t cand_count;
+  int32_t any_sweep;
+  int32_t any_need_clean;
+  int32_t fill_changed;
+  unsigned long long evaluated;  // boundary items evaluated (all passes)
+  unsigned long long proposed;   // candidates proposed (all passes)
+  unsigned long long accepted;   // moves applied (all passes)
+  unsigned long long passes;     // passes run (any mesh sweeping)
+};
+
+// Pointers handed to the sweep kernels (kernel argument block).
+struct SweepArgs {
+  int64_t n_items;
+  int64_t n_clusters;
+  int32_t n_meshes;
+  int32_t pad;
+  const DevMesh* meshes;
+  MeshState* state;
+  const int32_t* item_mesh;  // null when n_meshes == 1
+  const int32_t* clus_mesh;  // null when n_meshes == 1
+  const int32_t* row_ptr;    // n_items + 1
+  const int32_t* col;        // global item ids
+  const float4* xw;          // position + weight
+  const float* q;            // 10 floats per item (QEM meshes), may be null
+  int32_t* labels;
+  unsigned long long* acc_ws;  // 4 per cluster: W, Sx, Sy, Sz (int64 fixed)
+  unsigned long long* acc_q;   // 10 per cluster (QEM), may be null
+  ClusterGeom* geom;           // centroid + W per cluster (FP64)
+  double* T;                   // W |q - c|^2 per cluster (QEM), may be null
+  int32_t* sizes;
+  int32_t* stamp;              // mesh-local loop of the last modification
+  unsigned long long* prio;    // accept keys per cluster
+  int32_t* cand_pass;          // per item: loop of its last proposal
+  int2* cand;                  // (item, target cluster) of this pass
+  Ctrl* ctrl;
+};
+
+// ---- launchers (defined in the .hip files) --------------------------------
+// build
+hipError_t launch_csr_count(hipStream_t s, int64_t nf, const int32_t* tri_global, int32_t* deg2, int32_t* vfdeg);
+hipError_t launch_csr_fill(hipStream_t s, int64_t nf, const int32_t* tri_global, const int32_t* tptr,
+                           int32_t* tfill, int32_t* tcol, const int32_t* vfptr, int32_t* vffill, int32_t* vfcol);
+hipError_t launch_csr_sort_unique(hipStream_t s, int64_t n, const int32_t* tptr, int32_t* tcol, int32_t* deg,
+                                  const int32_t* vfptr, int32_t* vfcol);
+hipError_t launch_csr_compact(hipStream_t s, int64_t n, const int32_t* tptr, const int32_t* tcol,
+                              const int32_t* row_ptr, int32_t* col);
+hipError_t exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, void** tmp,
+                              size_t* tmp_bytes);
+hipError_t launch_item_build(hipStream_t s, const SweepArgs& a, const double* pos, const int32_t* tri_global,
+                             const int32_t* vfptr, const int32_t* vfcol, const double* indicator,
+                             double* raw_w, float4* xw, float* q);
+hipError_t launch_seg_reduce(hipStream_t s, int n_blocks, const int2* block_tab, const SweepArgs& a,
+                             const double* raw_w, int what, double* partial);
+hipError_t launch_clamp_and_scales(hipStream_t s, const SweepArgs& a, int n_blocks, const int2* block_tab,
+                                   const double* partial, int stage, const double* raw_w, DevMesh* meshes_rw);
+// sweep
+hipError_t launch_zero_clusters(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask);
+hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask,
+                             unsigned long long* mesh_count);
+hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int set_stamp);
+hipError_t launch_propose(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem);
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid);
+hipError_t launch_controller(hipStream_t s, const SweepArgs& a);
+hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* pending);
+hipError_t launch_cc(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* parent,
+                     int32_t* comp_size, unsigned long long* best, int32_t* ncomp, int32_t* ndisc);
+hipError_t launch_seed_keys(hipStream_t s, const SweepArgs& a, int64_t off, int64_t n, uint32_t seed,
+                            uint32_t* keys, int32_t* vals);
+hipError_t launch_seed_assign(hipStream_t s, int32_t* labels, int64_t item_off, int64_t clus_off, int32_t K,
+                              const int32_t* sorted_vals);
+hipError_t sort_pairs_u32(hipStream_t s, const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout,
+                          int64_t n, void** tmp, size_t* tmp_bytes);
+hipError_t launch_cluster_points(hipStream_t s, const SweepArgs& a, int32_t mesh, double* pts);
+hipError_t launch_item_energy(hipStream_t s, const SweepArgs& a, const double* pts_all, double* e_item);
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
+
+}  // namespace acvd

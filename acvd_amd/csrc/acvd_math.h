@@ -1,0 +1,526 @@
+// Per-item / per-cluster arithmetic of the boundary sweep, shared verbatim by
+// the HIP kernels and the host-side replay so that both produce bit-identical
+// labels.  Rules that keep host and device in lock-step:
+//   - every translation unit that includes this header is compiled with
+//     -ffp-contract=off and without fast-math (no silent FMA formation);
+//   - only +, -, *, / and sqrt are used (IEEE correctly rounded on both sides),
+//     no transcendental functions;
+//   - conversions double -> int64 round to nearest even (llrint on the host,
+//     __double2ll_rn on the device), int64 -> double is exact-rounding on both;
+//   - integer priorities and integer tie-breaks only.
+//
+// Energy model (both metrics, per cluster c with weighted sum S, weight W,
+// centroid c = S/W and quadric-optimal point q):
+//   isotropic (vtkIsotropicMetricForClustering.h:114-121):  E_c = -|S|^2/W
+//   QEM       (vtkQEMetricForClustering.h:194-208):        E_c = W|q|^2 - 2 q.S
+//                                                           = -|S|^2/W + W|q-c|^2
+// Moving item i (position x, weight w) from cluster a to cluster b changes the
+// isotropic part by the Hartigan form
+//   dE_iso = w [ W_b/(W_b+w) |x-c_b|^2 - W_a/(W_a-w) |x-c_a|^2 ]
+// which equals the reference's Try2 - Try1 (vtkUniformClustering.h:925-940)
+// without its cancellation, and the QEM part by
+//   dT = T_a' + T_b' - T_a - T_b,  T = W |q - c|^2 = W |Delta|^2,
+// with Delta the rank-truncated pseudo-inverse step of
+// vtkQuadricTools::ComputeDisplacement (Common/vtkQuadricTools.cxx:83-163).
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ACVD_HD __host__ __device__ inline
+#else
+#include <cmath>
+#define ACVD_HD inline
+#endif
+
+namespace acvd {
+
+// Null label used inside the library (the reference uses K, the cluster count).
+constexpr int32_t kNull = -1;
+
+// Decision threshold: a move is taken only if dE < -kAcceptEps * M where M is
+// the sum of magnitudes of the terms of dE.  kAcceptEps is well above the FP32
+// rounding of the terms, so an accepted move lowers the exact energy and no
+// sequence of moves can cycle.
+constexpr float kAcceptEps = 1.0f / 262144.0f;  // 2^-18
+
+// Singular-value truncation threshold (vtkQuadricTools.h:36).
+constexpr double kSvThreshold = 1e-3;
+
+ACVD_HD double dsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_sqrt(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+
+ACVD_HD int64_t to_fixed(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __double2ll_rn(v);
+#else
+  return std::llrint(v);
+#endif
+}
+
+// Power-of-two fixed-point scales of one mesh: value * s -> int64.
+struct Scales {
+  double sw, ss, sq;     // weight, weighted position, quadric
+  double isw, iss, isq;  // inverses (exact, powers of two)
+};
+
+// Fixed-point images of one item (weights / weighted positions / quadric).
+ACVD_HD int64_t fix_w(float w, const Scales& s) { return to_fixed(double(w) * s.sw); }
+ACVD_HD int64_t fix_s(float w, float x, const Scales& s) { return to_fixed(double(w) * double(x) * s.ss); }
+ACVD_HD int64_t fix_q(float q, const Scales& s) { return to_fixed(double(q) * s.sq); }
+
+// ---------------------------------------------------------------------------
+// Rank-truncated representative point (FP64).
+// Q: 9 (or more) quadric coefficients in the compact layout
+//   [0]=A00 [1]=A01 [2]=A02 [3]=A0d [4]=A11 [5]=A12 [6]=A1d [7]=A22 [8]=A2d
+// c: start point (the centroid).  Writes Delta = A+_trunc (b - A c) with
+// b = -(Q3,Q6,Q8); returns the rank deficiency (0..3).
+// Eigen-decomposition: cyclic Jacobi, at most 8 sweeps, stopping as soon as
+// the off-diagonal part is negligible (same decisions on host and device since
+// the inputs are bit-identical).
+ACVD_HD int rep_point_delta(const double* Q, const double* c, int level, double* D) {
+  double a00 = Q[0], a01 = Q[1], a02 = Q[2], a11 = Q[4], a12 = Q[5], a22 = Q[7];
+  double r0 = -Q[3] - (Q[0] * c[0] + Q[1] * c[1] + Q[2] * c[2]);
+  double r1 = -Q[6] - (Q[1] * c[0] + Q[4] * c[1] + Q[5] * c[2]);
+  double r2 = -Q[8] - (Q[2] * c[0] + Q[5] * c[1] + Q[7] * c[2]);
+  // V = identity; columns are eigenvectors
+  double v00 = 1, v01 = 0, v02 = 0, v10 = 0, v11 = 1, v12 = 0, v20 = 0, v21 = 0, v22 = 1;
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    double off = (a01 < 0 ? -a01 : a01) + (a02 < 0 ? -a02 : a02) + (a12 < 0 ? -a12 : a12);
+    double dia = (a00 < 0 ? -a00 : a00) + (a11 < 0 ? -a11 : a11) + (a22 < 0 ? -a22 : a22);
+    if (!(off > dia * 1e-17)) break;
+    // three rotations (p,q) = (0,1), (0,2), (1,2)
+#define ACVD_JACOBI_ROT(app, aqq, apq, akp, akq, vp0, vq0, vp1, vq1, vp2, vq2)              \
+    if (apq != 0.0) {                                                                     \
+      double theta = (aqq - app) / (2.0 * apq);                                           \
+      double at = theta < 0 ? -theta : theta;                                             \
+      double t = (at > 1e150) ? 0.5 / theta                                               \
+                              : (theta >= 0 ? 1.0 : -1.0) / (at + dsqrt(theta * theta + 1.0)); \
+      double cs = 1.0 / dsqrt(t * t + 1.0);                                               \
+      double sn = t * cs;                                                                 \
+      double tau = t * apq;                                                               \
+      app = app - tau;                                                                    \
+      aqq = aqq + tau;                                                                    \
+      apq = 0.0;                                                                          \
+      double kp = akp, kq = akq;                                                          \
+      akp = cs * kp - sn * kq;                                                            \
+      akq = sn * kp + cs * kq;                                                            \
+      double x0 = vp0, y0 = vq0;                                                          \
+      vp0 = cs * x0 - sn * y0;                                                            \
+      vq0 = sn * x0 + cs * y0;                                                            \
+      double x1 = vp1, y1 = vq1;                                                          \
+      vp1 = cs * x1 - sn * y1;                                                            \
+      vq1 = sn * x1 + cs * y1;                                                            \
+      double x2 = vp2, y2 = vq2;                                                          \
+      vp2 = cs * x2 - sn * y2;                                                            \
+      vq2 = sn * x2 + cs * y2;                                                            \
+    }
+    // (0,1): third index 2 -> couples a02 (k=2,p=0) and a12 (k=2,q=1)
+    ACVD_JACOBI_ROT(a00, a11, a01, a02, a12, v00, v01, v10, v11, v20, v21)
+    // (0,2): third index 1 -> a01 (p=0) and a12 (q=2)
+    ACVD_JACOBI_ROT(a00, a22, a02, a01, a12, v00, v02, v10, v12, v20, v22)
+    // (1,2): third index 0 -> a01 (p=1) and a02 (q=2)
+    ACVD_JACOBI_ROT(a11, a22, a12, a01, a02, v01, v02, v11, v12, v21, v22)
+#undef ACVD_JACOBI_ROT
+  }
+  double lam[3] = {a00, a11, a22};
+  double aw[3] = {lam[0] < 0 ? -lam[0] : lam[0], lam[1] < 0 ? -lam[1] : lam[1], lam[2] < 0 ? -lam[2] : lam[2]};
+  double maxw = aw[0];
+  if (aw[1] > maxw) maxw = aw[1];
+  if (aw[2] > maxw) maxw = aw[2];
+  // visit eigenvalues by decreasing |lambda|, first index on ties
+  int keep0 = 0, keep1 = 0, keep2 = 0;
+  int rank_def = 0;
+  int left = level;
+  double rem0 = aw[0], rem1 = aw[1], rem2 = aw[2];
+  for (int it = 0; it < 3; ++it) {
+    int idx = 0;
+    double best = rem0;
+    if (best < rem1) { best = rem1; idx = 1; }
+    if (best < rem2) { best = rem2; idx = 2; }
+    bool ok = (maxw > 0.0) && (best / maxw > kSvThreshold) && (left > 0);
+    if (ok) {
+      if (idx == 0) keep0 = 1;
+      else if (idx == 1) keep1 = 1;
+      else keep2 = 1;
+    } else {
+      rank_def++;
+    }
+    if (idx == 0) rem0 = -2.0;
+    else if (idx == 1) rem1 = -2.0;
+    else rem2 = -2.0;
+    left--;
+  }
+  double d0 = 0, d1 = 0, d2 = 0;
+  if (keep0) {
+    double p = (v00 * r0 + v10 * r1 + v20 * r2) / lam[0];
+    d0 += v00 * p; d1 += v10 * p; d2 += v20 * p;
+  }
+  if (keep1) {
+    double p = (v01 * r0 + v11 * r1 + v21 * r2) / lam[1];
+    d0 += v01 * p; d1 += v11 * p; d2 += v21 * p;
+  }
+  if (keep2) {
+    double p = (v02 * r0 + v12 * r1 + v22 * r2) / lam[2];
+    d0 += v02 * p; d1 += v12 * p; d2 += v22 * p;
+  }
+  D[0] = d0;
+  D[1] = d1;
+  D[2] = d2;
+  return rank_def;
+}
+
+// ---------------------------------------------------------------------------
+// Cluster summaries from fixed-point accumulators.
+//   ws[0] = W, ws[1..3] = S  (int64 fixed)
+//   q[0..9]                  (int64 fixed, QEM only)
+struct ClusterGeom {
+  double cx, cy, cz, W;
+};
+
+ACVD_HD ClusterGeom geom_from_fixed(int64_t W, int64_t Sx, int64_t Sy, int64_t Sz, const Scales& s) {
+  ClusterGeom g;
+  g.W = double(W) * s.isw;
+  double sx = double(Sx) * s.iss, sy = double(Sy) * s.iss, sz = double(Sz) * s.iss;
+  if (g.W > 0.0) {
+    g.cx = sx / g.W;
+    g.cy = sy / g.W;
+    g.cz = sz / g.W;
+  } else {
+    g.cx = g.cy = g.cz = 0.0;
+  }
+  return g;
+}
+
+// T = W |q - c|^2 of a cluster given its fixed accumulators (QEM metric with
+// active constraints; 0 otherwise).  Optionally returns q.
+ACVD_HD double qem_T_from_fixed(const int64_t* q10, const ClusterGeom& g, const Scales& s, int level,
+                                double* q_out) {
+  if (!(g.W > 0.0) || level <= 0) {
+    if (q_out) { q_out[0] = g.cx; q_out[1] = g.cy; q_out[2] = g.cz; }
+    return 0.0;
+  }
+  double Q[9];
+  for (int k = 0; k < 9; ++k) Q[k] = double(q10[k]) * s.isq;
+  double c[3] = {g.cx, g.cy, g.cz};
+  double D[3];
+  rep_point_delta(Q, c, level, D);
+  if (q_out) { q_out[0] = c[0] + D[0]; q_out[1] = c[1] + D[1]; q_out[2] = c[2] + D[2]; }
+  return g.W * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Hartigan terms (FP32 with the position difference taken in FP64 first, so
+// that |x - c| keeps FP32 relative precision for clusters much smaller than
+// the coordinates).
+//   leave term:  W_a/(W_a-w) |x-c_a|^2   (valid only for W_a > w)
+//   join term:   W_b/(W_b+w) |x-c_b|^2
+ACVD_HD float dist2_f(float x, float y, float z, const ClusterGeom& g) {
+  float dx = float(double(x) - g.cx);
+  float dy = float(double(y) - g.cy);
+  float dz = float(double(z) - g.cz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+ACVD_HD float leave_term(float x, float y, float z, float w, const ClusterGeom& a) {
+  float f = float(a.W / (a.W - double(w)));
+  return f * dist2_f(x, y, z, a);
+}
+
+ACVD_HD float join_term(float x, float y, float z, float w, const ClusterGeom& b) {
+  float f = float(b.W / (b.W + double(w)));
+  return f * dist2_f(x, y, z, b);
+}
+
+// True when the leave term is usable (the item does not carry the whole
+// cluster weight up to FP64 resolution).
+ACVD_HD bool can_leave(float w, const ClusterGeom& a) { return a.W - double(w) > a.W * 9.5367431640625e-7; }
+
+// ---------------------------------------------------------------------------
+// Seeded priorities for the conflict-free accept step.
+ACVD_HD uint32_t prio_hash(uint32_t seed, uint32_t pass, uint32_t item) {
+  uint32_t h = item * 0x9E3779B1u ^ (pass * 0x85EBCA77u + seed * 0xC2B2AE3Du);
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+ACVD_HD uint64_t prio_key(uint32_t seed, uint32_t pass, uint32_t item) {
+  return (uint64_t(prio_hash(seed, pass, item)) << 32) | uint64_t(item);
+}
+
+// Accept key of a candidate move in the conflict-free accept step:
+//   [63:40] 0xFFFFFF - (loop & 0xFFFFFF)  newer passes give smaller keys, so a
+//                                         slot never needs resetting
+//   [39:27] seeded hash of (loop, item)   random priority
+//   [26:0]  mesh-local item index         unique tie-break
+// A move is applied iff its key is the minimum at both touched clusters.
+ACVD_HD uint64_t accept_key(uint32_t seed, int32_t loop, uint32_t local_item) {
+  uint64_t tag = uint64_t(0xFFFFFFu - (uint32_t(loop) & 0xFFFFFFu));
+  uint64_t h = uint64_t(prio_hash(seed, uint32_t(loop), local_item) & 0x1FFFu);
+  return (tag << 40) | (h << 27) | uint64_t(local_item & 0x7FFFFFFu);
+}
+constexpr int64_t kMaxItemsPerMesh = int64_t(1) << 27;
+constexpr int32_t kMaxLoops = 0xFFFFFF;
+
+// Isotropic acceptance: dE = w (tb - ta) < -eps * w (ta + tb).
+ACVD_HD bool iso_accept(float ta, float tb) { return (tb - ta) < -kAcceptEps * (ta + tb); }
+
+// QEM move energy and its magnitude (for the acceptance threshold).
+ACVD_HD double qem_delta(float w, float ta, float tb, double Ta_new, double Ta, double Tb_new, double Tb) {
+  return double(w * (tb - ta)) + ((Ta_new - Ta) + (Tb_new - Tb));
+}
+ACVD_HD double qem_mag(float w, float ta, float tb, double Ta_new, double Ta, double Tb_new, double Tb) {
+  return double(w * (ta + tb)) + (Ta_new + Ta + Tb_new + Tb);
+}
+ACVD_HD bool qem_accept(double delta, double mag) { return delta < -double(kAcceptEps) * mag; }
+
+// T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item.
+ACVD_HD double qem_T_trial(const int64_t* ws, const int64_t* q10, int64_t iw, const int64_t* is,
+                           const int64_t* iq, int sign, const Scales& s, int level) {
+  int64_t W = ws[0] + sign * iw;
+  int64_t Sx = ws[1] + sign * is[0], Sy = ws[2] + sign * is[1], Sz = ws[3] + sign * is[2];
+  int64_t Q[10];
+  for (int k = 0; k < 10; ++k) Q[k] = q10[k] + sign * iq[k];
+  ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, s);
+  return qem_T_from_fixed(Q, g, s, level, nullptr);
+}
+
+// Ring check of the vertex path (ConnexityConstraintProblemLocal,
+// DiscreteRemeshing/vtkVerticesProcessing.h:168-237): would removing item i
+// from cluster a split the ring neighbours of i that belong to a, when those
+// neighbours are connected only through mesh edges between themselves?
+// rp/col is the CSR adjacency with each row sorted by neighbour id.  Rings
+// with more than 64 neighbours are refused (treated as a problem).
+ACVD_HD bool ring_problem(const int32_t* rp, const int32_t* col, const int32_t* lab, int32_t i, int32_t a) {
+  const int32_t r0 = rp[i], r1 = rp[i + 1];
+  const int32_t deg = r1 - r0;
+  if (deg > 64) return true;
+  uint64_t M = 0;
+  for (int32_t k = 0; k < deg; ++k)
+    if (lab[col[r0 + k]] == a) M |= uint64_t(1) << k;
+  if (M == 0) return false;
+  uint64_t reach = M & (~M + 1);
+  uint64_t frontier = reach;
+  while (frontier) {
+    int t = __builtin_ctzll(frontier);
+    frontier &= frontier - 1;
+    const int32_t j = col[r0 + t];
+    const int32_t s0 = rp[j], s1 = rp[j + 1];
+    // merge-walk the two sorted rows
+    int32_t p = r0, q = s0;
+    while (p < r1 && q < s1) {
+      const int32_t u = col[p], v = col[q];
+      if (u < v) {
+        ++p;
+      } else if (v < u) {
+        ++q;
+      } else {
+        const uint64_t bit = uint64_t(1) << (p - r0);
+        if ((M & bit) && !(reach & bit)) {
+          reach |= bit;
+          frontier |= bit;
+        }
+        ++p;
+        ++q;
+      }
+    }
+  }
+  return reach != M;
+}
+
+// ---------------------------------------------------------------------------
+// Read-only view of the sweep state used by the propose rule.  On the device
+// the pointers are global memory; in the replay they are host arrays.
+struct SweepView {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* labels;
+  const float* xw;                   // 4 floats per item: x, y, z, w
+  const float* q;                    // 10 floats per item (QEM)
+  const unsigned long long* acc_ws;  // 4 per cluster (int64 fixed bits)
+  const unsigned long long* acc_q;   // 10 per cluster
+  const ClusterGeom* geom;
+  const double* T;
+  const int32_t* sizes;
+  const int32_t* stamp;
+  const int32_t* cand_pass;
+};
+
+// Item i must be evaluated in the pass of loop L: it is on a cluster
+// boundary, and its own cluster or a neighbouring one changed in this or the
+// previous loop (the reference's skip test, vtkUniformClustering.h:909-920),
+// or its proposal of the previous loop lost the accept step.
+ACVD_HD bool needs_eval(const SweepView& v, int64_t i, int32_t lab, int32_t L) {
+  bool dirty = (v.cand_pass[i] == L - 1);
+  if (lab != kNull) dirty |= (v.stamp[lab] + 1 >= L);
+  bool boundary = false;
+  const int32_t r1 = v.row_ptr[i + 1];
+  for (int32_t k = v.row_ptr[i]; k < r1; ++k) {
+    const int32_t b = v.labels[v.col[k]];
+    if (b != lab) {
+      boundary = true;
+      if (b != kNull && v.stamp[b] + 1 >= L) dirty = true;
+    }
+  }
+  return boundary && dirty;
+}
+
+ACVD_HD void load_acc(const SweepView& v, int32_t c, int64_t* ws, int64_t* q10) {
+  for (int k = 0; k < 4; ++k) ws[k] = int64_t(v.acc_ws[4 * int64_t(c) + k]);
+  for (int k = 0; k < 10; ++k) q10[k] = int64_t(v.acc_q[10 * int64_t(c) + k]);
+}
+
+// Best move of an evaluated item (label lab, loop L).  Returns true and the
+// target cluster if the move is proposed.
+//  - null item: joins the neighbouring cluster with the smallest join term
+//    (the reference absorbs it into the first neighbour met,
+//    vtkUniformClustering.h:881-907);
+//  - otherwise: no move out of a size-1 cluster or against the ring check
+//    (vtkUniformClustering.h:929, 945), then the neighbouring cluster of
+//    lowest dE (ties: lowest cluster id) if dE passes the acceptance test.
+template <int METRIC>
+ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained, int level,
+                          const Scales& sc, int32_t* target_out) {
+  const float px = v.xw[4 * i], py = v.xw[4 * i + 1], pz = v.xw[4 * i + 2], pw = v.xw[4 * i + 3];
+  const int32_t r0 = v.row_ptr[i], r1 = v.row_ptr[i + 1];
+  int32_t target = kNull;
+  if (lab == kNull) {
+    float best = 3.0e38f;
+    for (int32_t k = r0; k < r1; ++k) {
+      const int32_t b = v.labels[v.col[k]];
+      if (b == kNull) continue;
+      const float tb = join_term(px, py, pz, pw, v.geom[b]);
+      if (tb < best || (tb == best && b < target)) {
+        best = tb;
+        target = b;
+      }
+    }
+    *target_out = target;
+    return target != kNull;
+  }
+  if (v.sizes[lab] <= 1) return false;
+  if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
+  const ClusterGeom ga = v.geom[lab];
+  if (!can_leave(pw, ga)) return false;
+  const float ta = leave_term(px, py, pz, pw, ga);
+  if (METRIC == 0) {
+    float best = 3.0e38f;
+    for (int32_t k = r0; k < r1; ++k) {
+      const int32_t b = v.labels[v.col[k]];
+      if (b == kNull || b == lab) continue;
+      const float tb = join_term(px, py, pz, pw, v.geom[b]);
+      if (tb < best || (tb == best && b < target)) {
+        best = tb;
+        target = b;
+      }
+    }
+    *target_out = target;
+    return target != kNull && iso_accept(ta, best);
+  }
+  const int64_t iw = fix_w(pw, sc);
+  const int64_t is[3] = {fix_s(pw, px, sc), fix_s(pw, py, sc), fix_s(pw, pz, sc)};
+  int64_t iq[10];
+  for (int k = 0; k < 10; ++k) iq[k] = fix_q(v.q[10 * i + k], sc);
+  double Ta = 0.0, Ta_new = 0.0;
+  if (constrained) {
+    int64_t ws[4], q10[10];
+    load_acc(v, lab, ws, q10);
+    Ta = v.T[lab];
+    Ta_new = qem_T_trial(ws, q10, iw, is, iq, -1, sc, level);
+  }
+  double best_d = 0.0, best_m = 0.0;
+  for (int32_t k = r0; k < r1; ++k) {
+    const int32_t b = v.labels[v.col[k]];
+    if (b == kNull || b == lab) continue;
+    bool seen = false;  // each neighbouring cluster is evaluated once
+    for (int32_t t = r0; t < k; ++t)
+      if (v.labels[v.col[t]] == b) {
+        seen = true;
+        break;
+      }
+    if (seen) continue;
+    const float tb = join_term(px, py, pz, pw, v.geom[b]);
+    double Tb = 0.0, Tb_new = 0.0;
+    if (constrained) {
+      int64_t ws[4], q10[10];
+      load_acc(v, b, ws, q10);
+      Tb = v.T[b];
+      Tb_new = qem_T_trial(ws, q10, iw, is, iq, +1, sc, level);
+    }
+    const double d = qem_delta(pw, ta, tb, Ta_new, Ta, Tb_new, Tb);
+    if (target == kNull || d < best_d || (d == best_d && b < target)) {
+      best_d = d;
+      best_m = qem_mag(pw, ta, tb, Ta_new, Ta, Tb_new, Tb);
+      target = b;
+    }
+  }
+  *target_out = target;
+  return target != kNull && qem_accept(best_d, best_m);
+}
+
+// Per-mesh convergence state, advanced once per pass by the controller with
+// the reference's phase logic (vtkUniformClustering.h:724-830).
+enum MeshPhase : int32_t {
+  kPhaseSweep = 0,     // boundary passes running
+  kPhaseNeedClean = 1, // a convergence was reached; clean + fill pending
+  kPhaseDone = 2,
+  kPhaseIdle = 3       // not part of the current run
+};
+
+struct MeshState {
+  int32_t phase;
+  int32_t loops;            // NumberOfLoops
+  int32_t convergences;     // NumberOfConvergences
+  int32_t connexity;        // ConnexityConstraint
+  int32_t constrained;      // QEM ActiveConstraints
+  int32_t unconstrained_init;
+  int32_t max_loops;
+  int32_t max_convergences;
+  int64_t early_stop_items; // items whose cluster is not frozen, at MinimizeEnergy start
+  int32_t last_mods;        // modifications of the last pass
+  int32_t pass_mods;        // accumulated by the apply step of the current pass
+  int64_t total_mods;
+  int64_t evaluated;        // boundary items evaluated (statistics)
+  int64_t proposals;        // candidates proposed (statistics)
+  int32_t last_disconnected;
+  int32_t pad;
+};
+
+// Controller transition after one pass (the decision part of MinimizeEnergy).
+// Returns true if the mesh reached a convergence (clean + fill must follow).
+ACVD_HD bool controller_step(MeshState& s) {
+  int mods = s.pass_mods;
+  s.last_mods = mods;
+  s.total_mods += mods;
+  s.pass_mods = 0;
+  s.loops++;
+  bool conv = (mods == 0) || (s.loops > s.max_loops) ||
+              (int64_t(mods) <= s.early_stop_items / 1000 && s.convergences <= 1);
+  if (!conv) return false;
+  if (s.unconstrained_init && s.convergences == 0) s.constrained = 1;
+  if (s.convergences >= 1) s.connexity = 1;
+  s.convergences++;
+  s.phase = kPhaseNeedClean;
+  return true;
+}
+
+// Decision after clean + fill: true if the mesh is finished.
+ACVD_HD bool finish_after_clean(MeshState& s, int disconnected) {
+  s.last_disconnected = disconnected;
+  if (disconnected == 0 && s.last_mods == 0) return true;
+  if (s.loops >= s.max_loops) return true;
+  if (s.convergences >= s.max_convergences) return true;
+  return false;
+}
+
+}  // namespace acvd

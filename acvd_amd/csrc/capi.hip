@@ -1,0 +1,840 @@
+// C-ABI implementation: contexts, device build, and the MinimizeEnergy
+// driver (vtkUniformClustering.h:654-830) over the sweep kernels.
+//
+// Host work per ProcessClustering is limited to launching kernels and, between
+// chunks of passes, reading back a small control block (and the per-mesh
+// states when a mesh reached a convergence).  The chunk of passes
+// (propose -> apply -> controller) x kChunk is captured once into a hipGraph
+// and replayed; all pass-dependent values live in device memory.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/acvd_c.h"
+#include "acvd_ctx.h"
+
+using namespace acvd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(ACVD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+};
+
+struct HostMesh {
+  int64_t nv = 0, nf = 0;
+  std::vector<double> xyz;
+  std::vector<int32_t> tri;
+  std::vector<double> indicator;
+  std::vector<int32_t> init_labels;
+  acvd_params params;
+  // layout in the packed context
+  int64_t item_off = 0, clus_off = 0, n_adj = 0, adj_off = 0;
+  acvd_stats stats;
+};
+
+constexpr int kChunk = 16;        // passes per graph launch
+constexpr int kReduceItems = 16384;  // items per block of the deterministic reductions
+
+}  // namespace
+
+struct acvd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<HostMesh> meshes;
+  bool built = false;
+  bool any_iso = false, any_qem = false;
+  int64_t N = 0, K = 0, F = 0, E2 = 0;
+  // device arrays
+  DBuf<DevMesh> d_meshes;
+  DBuf<MeshState> d_state;
+  DBuf<Ctrl> d_ctrl;
+  DBuf<int32_t> d_item_mesh, d_clus_mesh;
+  DBuf<int32_t> d_row_ptr, d_col, d_tri;
+  DBuf<float4> d_xw;
+  DBuf<float> d_q;
+  DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
+  DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
+  DBuf<ClusterGeom> d_geom;
+  DBuf<double> d_T;
+  DBuf<int2> d_cand;
+  DBuf<int32_t> d_mask_all, d_mask, d_ndisc;
+  DBuf<unsigned long long> d_mesh_count;
+  // clean scratch
+  DBuf<int32_t> d_parent, d_comp_size, d_ncomp;
+  DBuf<unsigned long long> d_best;
+  // pinned host mirrors
+  Ctrl* h_ctrl = nullptr;
+  MeshState* h_state = nullptr;
+  std::vector<DevMesh> h_meshes;
+  void* cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+  hipGraphExec_t chunk_exec = nullptr;
+  hipGraph_t chunk_graph = nullptr;
+  double build_ms = 0.0, converge_ms = 0.0;
+  Ctrl last_ctrl{};
+
+  ~acvd_ctx() {
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    if (cub_tmp) (void)hipFree(cub_tmp);
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
+    if (h_state) (void)hipHostFree(h_state);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  SweepArgs args() const {
+    SweepArgs a{};
+    a.n_items = N;
+    a.n_clusters = K;
+    a.n_meshes = int32_t(meshes.size());
+    a.meshes = d_meshes.p;
+    a.state = d_state.p;
+    a.item_mesh = meshes.size() > 1 ? d_item_mesh.p : nullptr;
+    a.clus_mesh = meshes.size() > 1 ? d_clus_mesh.p : nullptr;
+    a.row_ptr = d_row_ptr.p;
+    a.col = d_col.p;
+    a.xw = d_xw.p;
+    a.q = any_qem ? d_q.p : nullptr;
+    a.labels = d_labels.p;
+    a.acc_ws = d_acc_ws.p;
+    a.acc_q = any_qem ? d_acc_q.p : nullptr;
+    a.geom = d_geom.p;
+    a.T = any_qem ? d_T.p : nullptr;
+    a.sizes = d_sizes.p;
+    a.stamp = d_stamp.p;
+    a.prio = d_prio.p;
+    a.cand_pass = d_cand_pass.p;
+    a.cand = d_cand.p;
+    a.ctrl = d_ctrl.p;
+    return a;
+  }
+
+  void invalidate() {
+    built = false;
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    chunk_exec = nullptr;
+    chunk_graph = nullptr;
+  }
+};
+
+namespace {
+
+int check_ctx(const acvd_ctx* c) {
+  if (!c) return fail(ACVD_ERR_ARG, "null context");
+  return ACVD_OK;
+}
+
+int check_mesh(const acvd_ctx* c, int32_t m) {
+  if (!c) return fail(ACVD_ERR_ARG, "null context");
+  if (m < 0 || m >= int32_t(c->meshes.size())) return fail(ACVD_ERR_ARG, "mesh id out of range");
+  return ACVD_OK;
+}
+
+template <class P>
+int add_mesh(acvd_ctx* c, int64_t nv, const P* xyz, int64_t nf, const int32_t* tri, int32_t* mesh_id) {
+  if (!c) return fail(ACVD_ERR_ARG, "null context");
+  if (nv <= 0 || nf <= 0 || !xyz || !tri) return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  if (nv >= kMaxItemsPerMesh) return fail(ACVD_ERR_LIMIT, "mesh has more than 2^27 vertices");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv)
+      return fail(ACVD_ERR_ARG, "triangle " + std::to_string(k / 3) + " references vertex " + std::to_string(tri[k]) +
+                                    " but only " + std::to_string(nv) + " vertices exist");
+  HostMesh h;
+  h.nv = nv;
+  h.nf = nf;
+  h.xyz.resize(3 * nv);
+  for (int64_t k = 0; k < 3 * nv; ++k) h.xyz[k] = double(xyz[k]);
+  h.tri.assign(tri, tri + 3 * nf);
+  acvd_params_init(&h.params);
+  std::memset(&h.stats, 0, sizeof(h.stats));
+  c->meshes.push_back(std::move(h));
+  c->invalidate();
+  if (mesh_id) *mesh_id = int32_t(c->meshes.size() - 1);
+  return ACVD_OK;
+}
+
+// ------------------------------------------------------------------- build
+int build(acvd_ctx* c) {
+  if (c->built) return ACVD_OK;
+  if (c->meshes.empty()) return fail(ACVD_ERR_STATE, "no input mesh");
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int M = int(c->meshes.size());
+  int64_t N = 0, K = 0, F = 0;
+  c->any_iso = c->any_qem = false;
+  for (auto& h : c->meshes) {
+    if (h.params.number_of_clusters <= 0)
+      return fail(ACVD_ERR_ARG, "must set NumberOfClusters to more than zero");
+    if (h.params.number_of_clusters > h.nv) return fail(ACVD_ERR_ARG, "more clusters than vertices");
+    h.item_off = N;
+    h.clus_off = K;
+    N += h.nv;
+    K += h.params.number_of_clusters;
+    F += h.nf;
+    if (h.params.metric == ACVD_METRIC_QEM) c->any_qem = true;
+    else c->any_iso = true;
+  }
+  if (N >= (int64_t(1) << 31) - 1 || K >= (int64_t(1) << 31) - 1 || 3 * F >= (int64_t(1) << 31))
+    return fail(ACVD_ERR_LIMIT, "context exceeds 2^31 items, clusters or face corners");
+  c->N = N;
+  c->K = K;
+  c->F = F;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, s));
+
+  // per-mesh descriptors
+  c->h_meshes.assign(M, DevMesh{});
+  std::vector<int32_t> item_mesh, clus_mesh;
+  if (M > 1) {
+    item_mesh.resize(N);
+    clus_mesh.resize(K);
+  }
+  std::vector<double> pos(3 * N), ind(N, 0.0);
+  std::vector<int32_t> tri(3 * F);
+  bool any_ind = false;
+  int64_t foff = 0;
+  for (int m = 0; m < M; ++m) {
+    HostMesh& h = c->meshes[m];
+    DevMesh& d = c->h_meshes[m];
+    d.item_off = h.item_off;
+    d.n_items = h.nv;
+    d.clus_off = h.clus_off;
+    d.K = h.params.number_of_clusters;
+    d.metric = h.params.metric == ACVD_METRIC_QEM ? 1 : 0;
+    d.level = h.params.quadrics_optimization_level;
+    d.seed = h.params.seed;
+    d.has_indicator = h.indicator.empty() ? 0 : 1;
+    d.gradation = h.params.gradation;
+    d.clamp_ratio = d.metric == 1 ? 10000.0 : 100000.0;
+    std::copy(h.xyz.begin(), h.xyz.end(), pos.begin() + 3 * h.item_off);
+    if (!h.indicator.empty()) {
+      any_ind = true;
+      std::copy(h.indicator.begin(), h.indicator.end(), ind.begin() + h.item_off);
+    }
+    for (int64_t k = 0; k < 3 * h.nf; ++k) tri[3 * foff + k] = int32_t(h.tri[k] + h.item_off);
+    foff += h.nf;
+    if (M > 1) {
+      std::fill(item_mesh.begin() + h.item_off, item_mesh.begin() + h.item_off + h.nv, m);
+      std::fill(clus_mesh.begin() + h.clus_off, clus_mesh.begin() + h.clus_off + d.K, m);
+    }
+  }
+  // allocations
+  HIP_TRY(c->d_meshes.alloc(M));
+  HIP_TRY(c->d_state.alloc(M));
+  HIP_TRY(c->d_ctrl.alloc(1));
+  HIP_TRY(c->d_mask_all.alloc(M));
+  HIP_TRY(c->d_mask.alloc(M));
+  HIP_TRY(c->d_ndisc.alloc(M));
+  HIP_TRY(c->d_mesh_count.alloc(M));
+  if (M > 1) {
+    HIP_TRY(c->d_item_mesh.alloc(N));
+    HIP_TRY(c->d_clus_mesh.alloc(K));
+    HIP_TRY(hipMemcpyAsync(c->d_item_mesh.p, item_mesh.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_clus_mesh.p, clus_mesh.data(), K * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
+  HIP_TRY(hipMemcpyAsync(c->d_meshes.p, c->h_meshes.data(), M * sizeof(DevMesh), hipMemcpyHostToDevice, s));
+  std::vector<int32_t> ones(M, 1);
+  HIP_TRY(hipMemcpyAsync(c->d_mask_all.p, ones.data(), M * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(c->d_tri.alloc(3 * F));
+  HIP_TRY(hipMemcpyAsync(c->d_tri.p, tri.data(), 3 * F * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  DBuf<double> d_pos, d_ind, d_raw;
+  HIP_TRY(d_pos.alloc(3 * N));
+  HIP_TRY(hipMemcpyAsync(d_pos.p, pos.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  if (any_ind) {
+    HIP_TRY(d_ind.alloc(N));
+    HIP_TRY(hipMemcpyAsync(d_ind.p, ind.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  // ---- CSR
+  {
+    DBuf<int32_t> deg2, vfdeg, tptr, vfptr, tfill, vffill, tcol, vfcol, deg;
+    HIP_TRY(deg2.alloc(N + 1));
+    HIP_TRY(vfdeg.alloc(N + 1));
+    HIP_TRY(tptr.alloc(N + 1));
+    HIP_TRY(vfptr.alloc(N + 1));
+    HIP_TRY(tfill.alloc(N));
+    HIP_TRY(vffill.alloc(N));
+    HIP_TRY(hipMemsetAsync(deg2.p, 0, (N + 1) * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(vfdeg.p, 0, (N + 1) * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(tfill.p, 0, N * sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(vffill.p, 0, N * sizeof(int32_t), s));
+    HIP_TRY(launch_csr_count(s, F, c->d_tri.p, deg2.p, vfdeg.p));
+    HIP_TRY(exclusive_scan_i32(s, deg2.p, tptr.p, N + 1, &c->cub_tmp, &c->cub_bytes));
+    HIP_TRY(exclusive_scan_i32(s, vfdeg.p, vfptr.p, N + 1, &c->cub_tmp, &c->cub_bytes));
+    int32_t tot[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&tot[0], tptr.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&tot[1], vfptr.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(tcol.alloc(tot[0]));
+    HIP_TRY(vfcol.alloc(tot[1]));
+    HIP_TRY(launch_csr_fill(s, F, c->d_tri.p, tptr.p, tfill.p, tcol.p, vfptr.p, vffill.p, vfcol.p));
+    HIP_TRY(deg.alloc(N + 1));
+    HIP_TRY(hipMemsetAsync(deg.p, 0, (N + 1) * sizeof(int32_t), s));
+    HIP_TRY(launch_csr_sort_unique(s, N, tptr.p, tcol.p, deg.p, vfptr.p, vfcol.p));
+    HIP_TRY(c->d_row_ptr.alloc(N + 1));
+    HIP_TRY(exclusive_scan_i32(s, deg.p, c->d_row_ptr.p, N + 1, &c->cub_tmp, &c->cub_bytes));
+    int32_t e2 = 0;
+    HIP_TRY(hipMemcpyAsync(&e2, c->d_row_ptr.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c->E2 = e2;
+    HIP_TRY(c->d_col.alloc(e2));
+    HIP_TRY(launch_csr_compact(s, N, tptr.p, tcol.p, c->d_row_ptr.p, c->d_col.p));
+    // ---- items
+    HIP_TRY(c->d_xw.alloc(N));
+    if (c->any_qem) HIP_TRY(c->d_q.alloc(10 * N));
+    HIP_TRY(d_raw.alloc(N));
+    SweepArgs a = c->args();
+    HIP_TRY(launch_item_build(s, a, d_pos.p, c->d_tri.p, vfptr.p, vfcol.p, any_ind ? d_ind.p : nullptr, d_raw.p,
+                              c->d_xw.p, c->any_qem ? c->d_q.p : nullptr));
+  }
+  // ---- deterministic reductions: clamp then scales
+  {
+    std::vector<int2> tab;
+    for (auto& h : c->meshes)
+      for (int64_t b = h.item_off; b < h.item_off + h.nv; b += kReduceItems)
+        tab.push_back(make_int2(int32_t(b), int32_t(std::min<int64_t>(b + kReduceItems, h.item_off + h.nv))));
+    DBuf<int2> d_tab;
+    DBuf<double> d_part;
+    HIP_TRY(d_tab.alloc(tab.size()));
+    HIP_TRY(d_part.alloc(4 * tab.size() + M));
+    HIP_TRY(hipMemcpyAsync(d_tab.p, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    SweepArgs a = c->args();
+    HIP_TRY(launch_seg_reduce(s, int(tab.size()), d_tab.p, a, d_raw.p, 0, d_part.p));
+    HIP_TRY(launch_clamp_and_scales(s, a, int(tab.size()), d_tab.p, d_part.p, 0, d_raw.p, c->d_meshes.p));
+    HIP_TRY(launch_seg_reduce(s, int(tab.size()), d_tab.p, a, d_raw.p, 1, d_part.p));
+    HIP_TRY(launch_clamp_and_scales(s, a, int(tab.size()), d_tab.p, d_part.p, 1, d_raw.p, c->d_meshes.p));
+    HIP_TRY(hipMemcpyAsync(c->h_meshes.data(), c->d_meshes.p, M * sizeof(DevMesh), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  // ---- sweep state
+  HIP_TRY(c->d_labels.alloc(N));
+  HIP_TRY(c->d_sizes.alloc(K));
+  HIP_TRY(c->d_stamp.alloc(K));
+  HIP_TRY(c->d_cand_pass.alloc(N));
+  HIP_TRY(c->d_pending.alloc(N));
+  HIP_TRY(c->d_acc_ws.alloc(4 * K));
+  if (c->any_qem) {
+    HIP_TRY(c->d_acc_q.alloc(10 * K));
+    HIP_TRY(c->d_T.alloc(K));
+  }
+  HIP_TRY(c->d_prio.alloc(K));
+  HIP_TRY(c->d_geom.alloc(K));
+  HIP_TRY(c->d_cand.alloc(N));
+  HIP_TRY(c->d_parent.alloc(N));
+  HIP_TRY(c->d_comp_size.alloc(N));
+  HIP_TRY(c->d_ncomp.alloc(K));
+  HIP_TRY(c->d_best.alloc(K));
+  HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  c->build_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto& h : c->meshes) {
+    h.n_adj = 0;
+  }
+  // per-mesh adjacency sizes (row_ptr at mesh boundaries)
+  {
+    std::vector<int32_t> rp(N + 1);
+    HIP_TRY(hipMemcpy(rp.data(), c->d_row_ptr.p, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (auto& h : c->meshes) {
+      h.adj_off = rp[h.item_off];
+      h.n_adj = int64_t(rp[h.item_off + h.nv]) - h.adj_off;
+    }
+  }
+  c->built = true;
+  return ACVD_OK;
+}
+
+// ------------------------------------------------------------ minimisation
+int fill_until_stable(acvd_ctx* c, const int32_t* mask, int max_rounds) {
+  hipStream_t s = c->stream;
+  SweepArgs a = c->args();
+  for (int r = 0; r < max_rounds; ++r) {
+    HIP_TRY(hipMemsetAsync(&c->d_ctrl.p->fill_changed, 0, sizeof(int32_t), s));
+    HIP_TRY(launch_fill_round(s, a, mask, c->d_pending.p));
+    int32_t changed = 0;
+    HIP_TRY(hipMemcpyAsync(&changed, &c->d_ctrl.p->fill_changed, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!changed) break;
+  }
+  return ACVD_OK;
+}
+
+int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsigned long long* mesh_count) {
+  hipStream_t s = c->stream;
+  SweepArgs a = c->args();
+  HIP_TRY(launch_zero_clusters(s, a, mask));
+  HIP_TRY(launch_accumulate(s, a, mask, mesh_count));
+  HIP_TRY(launch_derive(s, a, mask, set_stamp));
+  return ACVD_OK;
+}
+
+int enqueue_chunk(acvd_ctx* c) {
+  hipStream_t s = c->stream;
+  SweepArgs a = c->args();
+  const int apply_grid = int(std::max<int64_t>(1, std::min<int64_t>((c->N + kBlock - 1) / kBlock, 2048)));
+  for (int p = 0; p < kChunk; ++p) {
+    HIP_TRY(launch_propose(s, a, c->any_iso, c->any_qem));
+    HIP_TRY(launch_apply(s, a, apply_grid));
+    HIP_TRY(launch_controller(s, a));
+  }
+  return ACVD_OK;
+}
+
+int run_chunk(acvd_ctx* c, bool use_graph) {
+  hipStream_t s = c->stream;
+  if (use_graph) {
+    if (!c->chunk_exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_chunk(c);
+      hipGraph_t g = nullptr;
+      hipError_t ec = hipStreamEndCapture(s, &g);
+      if (rc != ACVD_OK) return rc;
+      HIP_TRY(ec);
+      c->chunk_graph = g;
+      HIP_TRY(hipGraphInstantiate(&c->chunk_exec, g, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(c->chunk_exec, s));
+    return ACVD_OK;
+  }
+  return enqueue_chunk(c);
+}
+
+int minimize_energy(acvd_ctx* c) {
+  hipStream_t s = c->stream;
+  const int M = int(c->meshes.size());
+  SweepArgs a = c->args();
+  const bool use_graph = std::getenv("ACVD_NO_GRAPH") == nullptr;
+  // FillHolesInClustering, then ReComputeStatistics + SetAllClustersToModified
+  int rc = fill_until_stable(c, c->d_mask_all.p, 1 << 20);
+  if (rc) return rc;
+  for (int m = 0; m < M; ++m) {
+    const acvd_params& p = c->meshes[m].params;
+    MeshState& st = c->h_state[m];
+    std::memset(&st, 0, sizeof(st));
+    st.phase = kPhaseSweep;
+    st.max_loops = std::min<int32_t>(p.max_number_of_loops, kMaxLoops - 2);
+    st.max_convergences = p.max_number_of_convergences;
+    st.unconstrained_init = p.unconstrained_initialization;
+    // ProcessClustering: SetConstrainedClustering(0) when unconstrained
+    // initialisation is requested (vtkUniformClustering.h:683-688)
+    st.constrained = p.unconstrained_initialization ? 0 : 1;
+  }
+  HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(c->d_mesh_count.p, 0, M * sizeof(unsigned long long), s));
+  rc = recompute_statistics(c, c->d_mask_all.p, 1, c->d_mesh_count.p);
+  if (rc) return rc;
+  std::vector<unsigned long long> cnt(M);
+  HIP_TRY(hipMemcpyAsync(cnt.data(), c->d_mesh_count.p, M * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int m = 0; m < M; ++m) c->h_state[m].early_stop_items = int64_t(cnt[m]);
+  HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_fill_i32(s, c->d_cand_pass.p, c->N, -2));
+  HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
+  HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
+  std::vector<int32_t> mask(M), ndisc(M);
+  int64_t guard = 0;
+  while (true) {
+    int rc2 = run_chunk(c, use_graph);
+    if (rc2) return rc2;
+    HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (c->h_ctrl->any_need_clean) {
+      HIP_TRY(hipMemcpyAsync(c->h_state, c->d_state.p, M * sizeof(MeshState), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      for (int m = 0; m < M; ++m) mask[m] = c->h_state[m].phase == kPhaseNeedClean ? 1 : 0;
+      HIP_TRY(hipMemcpyAsync(c->d_mask.p, mask.data(), M * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemsetAsync(c->d_ndisc.p, 0, M * sizeof(int32_t), s));
+      // CleanClustering, FillHolesInClustering
+      HIP_TRY(launch_cc(s, a, c->d_mask.p, c->d_parent.p, c->d_comp_size.p, c->d_best.p, c->d_ncomp.p,
+                        c->d_ndisc.p));
+      rc = fill_until_stable(c, c->d_mask.p, 1 << 20);
+      if (rc) return rc;
+      HIP_TRY(hipMemcpyAsync(ndisc.data(), c->d_ndisc.p, M * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      for (int m = 0; m < M; ++m) {
+        if (!mask[m]) continue;
+        MeshState& st = c->h_state[m];
+        if (c->meshes[m].params.console_output)
+          std::printf("\nConvergence: %d disconnected classes (mesh %d, loop %d)\n", ndisc[m], m, st.loops);
+        st.phase = finish_after_clean(st, ndisc[m]) ? kPhaseDone : kPhaseSweep;
+      }
+      // statistics of the cleaned clusterings; continuing meshes restart
+      // with every cluster marked modified
+      rc = recompute_statistics(c, c->d_mask.p, 1, nullptr);
+      if (rc) return rc;
+      HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      bool any = false;
+      for (int m = 0; m < M; ++m) any |= c->h_state[m].phase == kPhaseSweep;
+      if (!any) break;
+      continue;
+    }
+    if (!c->h_ctrl->any_sweep) break;
+    if (++guard > (int64_t(1) << 40)) return fail(ACVD_ERR_STATE, "sweep did not terminate");
+  }
+  HIP_TRY(hipMemcpyAsync(c->h_state, c->d_state.p, M * sizeof(MeshState), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c->last_ctrl = *c->h_ctrl;
+  return ACVD_OK;
+}
+
+int initial_labels(acvd_ctx* c) {
+  hipStream_t s = c->stream;
+  SweepArgs a = c->args();
+  for (auto& h : c->meshes) {
+    const int32_t K = h.params.number_of_clusters;
+    if (!h.init_labels.empty()) {
+      std::vector<int32_t> lab(h.nv);
+      for (int64_t i = 0; i < h.nv; ++i) {
+        int32_t l = h.init_labels[i];
+        lab[i] = (l < 0 || l >= K) ? kNull : int32_t(l + h.clus_off);
+      }
+      HIP_TRY(hipMemcpyAsync(c->d_labels.p + h.item_off, lab.data(), h.nv * sizeof(int32_t), hipMemcpyHostToDevice,
+                             s));
+      HIP_TRY(hipStreamSynchronize(s));
+    } else {
+      DBuf<uint32_t> k0, k1;
+      DBuf<int32_t> v0, v1;
+      HIP_TRY(k0.alloc(h.nv));
+      HIP_TRY(k1.alloc(h.nv));
+      HIP_TRY(v0.alloc(h.nv));
+      HIP_TRY(v1.alloc(h.nv));
+      HIP_TRY(launch_fill_i32(s, c->d_labels.p + h.item_off, h.nv, kNull));
+      HIP_TRY(launch_seed_keys(s, a, h.item_off, h.nv, h.params.seed, k0.p, v0.p));
+      HIP_TRY(sort_pairs_u32(s, k0.p, k1.p, v0.p, v1.p, h.nv, &c->cub_tmp, &c->cub_bytes));
+      HIP_TRY(launch_seed_assign(s, c->d_labels.p, h.item_off, h.clus_off, K, v1.p));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+  }
+  return ACVD_OK;
+}
+
+}  // namespace
+
+// ============================================================== C interface
+extern "C" {
+
+const char* acvd_version(void) { return "acvd_amd 0.1 (gfx950)"; }
+
+const char* acvd_last_error(void) { return g_last_error.c_str(); }
+
+int acvd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void acvd_params_init(acvd_params* p) {
+  if (!p) return;
+  p->number_of_clusters = 0;
+  p->metric = ACVD_METRIC_ISOTROPIC;
+  p->quadrics_optimization_level = 3;
+  p->unconstrained_initialization = 0;
+  p->max_number_of_loops = 5000000;
+  p->max_number_of_convergences = 1000000000;
+  p->console_output = 0;
+  p->seed = 0;
+  p->gradation = 0.0;
+}
+
+int acvd_create(acvd_ctx** out, int device) {
+  if (!out) return fail(ACVD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail(ACVD_ERR_HIP, std::string("no HIP device available: ") + hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(ACVD_ERR_ARG, "device index out of range");
+  std::unique_ptr<acvd_ctx> c(new acvd_ctx());
+  c->device = device;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c->h_ctrl), sizeof(Ctrl), hipHostMallocDefault));
+  *out = c.release();
+  return ACVD_OK;
+}
+
+int acvd_destroy(acvd_ctx* ctx) {
+  if (!ctx) return ACVD_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return ACVD_OK;
+}
+
+int acvd_set_input_f64(acvd_ctx* ctx, int64_t nv, const double* xyz, int64_t nf, const int32_t* tri) {
+  if (!ctx) return fail(ACVD_ERR_ARG, "null context");
+  ctx->meshes.clear();
+  return add_mesh(ctx, nv, xyz, nf, tri, nullptr);
+}
+int acvd_set_input_f32(acvd_ctx* ctx, int64_t nv, const float* xyz, int64_t nf, const int32_t* tri) {
+  if (!ctx) return fail(ACVD_ERR_ARG, "null context");
+  ctx->meshes.clear();
+  return add_mesh(ctx, nv, xyz, nf, tri, nullptr);
+}
+int acvd_add_input_f64(acvd_ctx* ctx, int64_t nv, const double* xyz, int64_t nf, const int32_t* tri,
+                       int32_t* mesh_id) {
+  return add_mesh(ctx, nv, xyz, nf, tri, mesh_id);
+}
+int acvd_add_input_f32(acvd_ctx* ctx, int64_t nv, const float* xyz, int64_t nf, const int32_t* tri,
+                       int32_t* mesh_id) {
+  return add_mesh(ctx, nv, xyz, nf, tri, mesh_id);
+}
+int acvd_num_meshes(const acvd_ctx* ctx) { return ctx ? int(ctx->meshes.size()) : 0; }
+
+int acvd_set_params(acvd_ctx* ctx, int32_t mesh_id, const acvd_params* p) {
+  if (!ctx || !p) return fail(ACVD_ERR_ARG, "null argument");
+  if (p->metric != ACVD_METRIC_ISOTROPIC && p->metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "unknown metric");
+  if (p->quadrics_optimization_level < 0 || p->quadrics_optimization_level > 3)
+    return fail(ACVD_ERR_ARG, "quadrics optimization level must be 0..3");
+  if (p->max_number_of_loops < 0) return fail(ACVD_ERR_ARG, "negative loop limit");
+  if (mesh_id == -1) {
+    for (auto& h : ctx->meshes) h.params = *p;
+  } else {
+    if (int rc = check_mesh(ctx, mesh_id)) return rc;
+    ctx->meshes[mesh_id].params = *p;
+  }
+  ctx->invalidate();
+  return ACVD_OK;
+}
+
+int acvd_get_params(const acvd_ctx* ctx, int32_t mesh_id, acvd_params* p) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!p) return fail(ACVD_ERR_ARG, "null argument");
+  *p = ctx->meshes[mesh_id].params;
+  return ACVD_OK;
+}
+
+int acvd_set_indicator(acvd_ctx* ctx, int32_t mesh_id, const double* indicator) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  HostMesh& h = ctx->meshes[mesh_id];
+  if (!indicator) h.indicator.clear();
+  else h.indicator.assign(indicator, indicator + h.nv);
+  ctx->invalidate();
+  return ACVD_OK;
+}
+
+int acvd_set_initial_labels(acvd_ctx* ctx, int32_t mesh_id, const int32_t* labels) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  HostMesh& h = ctx->meshes[mesh_id];
+  if (!labels) h.init_labels.clear();
+  else h.init_labels.assign(labels, labels + h.nv);
+  return ACVD_OK;
+}
+
+int acvd_build(acvd_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  return build(ctx);
+}
+
+int acvd_process_clustering(acvd_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (ctx->meshes.empty()) return fail(ACVD_ERR_STATE, "no input mesh");
+  for (auto& h : ctx->meshes)
+    if (h.params.number_of_clusters <= 0) return fail(ACVD_ERR_ARG, "must set NumberOfClusters to more than zero");
+  int rc = build(ctx);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int M = int(ctx->meshes.size());
+  // (re)allocate the pinned state mirror for M meshes
+  if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+  ctx->h_state = nullptr;
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_state), M * sizeof(MeshState), hipHostMallocDefault));
+  for (auto& h : ctx->meshes)
+    if (h.params.console_output) std::printf("Clustering......\n");
+  rc = initial_labels(ctx);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, ctx->stream));
+  rc = minimize_energy(ctx);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(e1, ctx->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->converge_ms = ms;
+  for (int m = 0; m < M; ++m) {
+    HostMesh& h = ctx->meshes[m];
+    const MeshState& st = ctx->h_state[m];
+    acvd_stats& S = h.stats;
+    S.loops = st.loops;
+    S.convergences = st.convergences;
+    S.total_moves = st.total_mods;
+    S.last_disconnected = st.last_disconnected;
+    S.finished = st.phase == kPhaseDone ? 1 : 0;
+    S.ctx_passes = int64_t(ctx->last_ctrl.passes);
+    S.ctx_evaluated = int64_t(ctx->last_ctrl.evaluated);
+    S.ctx_proposed = int64_t(ctx->last_ctrl.proposed);
+    S.ctx_accepted = int64_t(ctx->last_ctrl.accepted);
+    S.converge_ms = ms;
+    S.build_ms = ctx->build_ms;
+    S.n_items = h.nv;
+    S.n_adjacency = h.n_adj;
+    if (h.params.console_output) {
+      std::printf("The clustering took :%g seconds.\n", ms / 1000.0);
+      std::printf("Number of loops:\t%lld\n", (long long)st.loops);
+    }
+  }
+  return ACVD_OK;
+}
+
+int acvd_get_labels(acvd_ctx* ctx, int32_t mesh_id, int32_t* labels) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!labels) return fail(ACVD_ERR_ARG, "null buffer");
+  if (!ctx->built || !ctx->d_labels.p) return fail(ACVD_ERR_STATE, "no clustering computed");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  HIP_TRY(hipMemcpy(labels, ctx->d_labels.p + h.item_off, h.nv * sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int32_t K = h.params.number_of_clusters;
+  for (int64_t i = 0; i < h.nv; ++i) labels[i] = labels[i] == kNull ? K : int32_t(labels[i] - h.clus_off);
+  return ACVD_OK;
+}
+
+int acvd_get_cluster_sizes(acvd_ctx* ctx, int32_t mesh_id, int32_t* sizes) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!sizes) return fail(ACVD_ERR_ARG, "null buffer");
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "no clustering computed");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  HIP_TRY(hipMemcpy(sizes, ctx->d_sizes.p + h.clus_off, h.params.number_of_clusters * sizeof(int32_t),
+                    hipMemcpyDeviceToHost));
+  return ACVD_OK;
+}
+
+int acvd_get_cluster_points(acvd_ctx* ctx, int32_t mesh_id, double* xyz) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!xyz) return fail(ACVD_ERR_ARG, "null buffer");
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "no clustering computed");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  DBuf<double> d;
+  HIP_TRY(d.alloc(3 * size_t(h.params.number_of_clusters)));
+  HIP_TRY(launch_cluster_points(ctx->stream, ctx->args(), mesh_id, d.p));
+  HIP_TRY(hipMemcpyAsync(xyz, d.p, 3 * size_t(h.params.number_of_clusters) * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ACVD_OK;
+}
+
+int acvd_get_energy(acvd_ctx* ctx, int32_t mesh_id, double* energy) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!energy) return fail(ACVD_ERR_ARG, "null buffer");
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "no clustering computed");
+  hipStream_t s = ctx->stream;
+  DBuf<double> pts, e;
+  HIP_TRY(pts.alloc(3 * size_t(ctx->K)));
+  HIP_TRY(e.alloc(ctx->N));
+  for (int m = 0; m < int(ctx->meshes.size()); ++m)
+    HIP_TRY(launch_cluster_points(s, ctx->args(), m, pts.p + 3 * ctx->meshes[m].clus_off));
+  HIP_TRY(launch_item_energy(s, ctx->args(), pts.p, e.p));
+  const HostMesh& h = ctx->meshes[mesh_id];
+  std::vector<double> he(h.nv);
+  HIP_TRY(hipMemcpyAsync(he.data(), e.p + h.item_off, h.nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  // fixed-order pairwise sum on the host (diagnostic only)
+  while (he.size() > 1) {
+    size_t half = (he.size() + 1) / 2;
+    for (size_t k = 0; k + half < he.size(); ++k) he[k] += he[k + half];
+    he.resize(half);
+  }
+  *energy = he.empty() ? 0.0 : he[0];
+  return ACVD_OK;
+}
+
+int acvd_get_stats(acvd_ctx* ctx, int32_t mesh_id, acvd_stats* st) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!st) return fail(ACVD_ERR_ARG, "null buffer");
+  *st = ctx->meshes[mesh_id].stats;
+  return ACVD_OK;
+}
+
+int acvd_get_adjacency_size(acvd_ctx* ctx, int32_t mesh_id, int64_t* n_adj) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "device inputs not built");
+  *n_adj = ctx->meshes[mesh_id].n_adj;
+  return ACVD_OK;
+}
+
+int acvd_get_adjacency(acvd_ctx* ctx, int32_t mesh_id, int32_t* row_ptr, int32_t* col) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "device inputs not built");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  HIP_TRY(hipMemcpy(row_ptr, ctx->d_row_ptr.p + h.item_off, (h.nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i <= h.nv; ++i) row_ptr[i] -= int32_t(h.adj_off);
+  HIP_TRY(hipMemcpy(col, ctx->d_col.p + h.adj_off, h.n_adj * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < h.n_adj; ++k) col[k] -= int32_t(h.item_off);
+  return ACVD_OK;
+}
+
+int acvd_get_items(acvd_ctx* ctx, int32_t mesh_id, float* xw, float* q) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "device inputs not built");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  if (xw) HIP_TRY(hipMemcpy(xw, ctx->d_xw.p + h.item_off, h.nv * sizeof(float4), hipMemcpyDeviceToHost));
+  if (q) {
+    if (h.params.metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "mesh has no quadrics (isotropic metric)");
+    HIP_TRY(hipMemcpy(q, ctx->d_q.p + 10 * h.item_off, 10 * h.nv * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  return ACVD_OK;
+}
+
+int acvd_get_scales(acvd_ctx* ctx, int32_t mesh_id, double* scales3) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!ctx->built) return fail(ACVD_ERR_STATE, "device inputs not built");
+  const Scales& s = ctx->h_meshes[mesh_id].scales;
+  scales3[0] = s.sw;
+  scales3[1] = s.ss;
+  scales3[2] = s.sq;
+  return ACVD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,318 @@
+// Device build of the clustering inputs:
+//   (a) the item/edge adjacency as a CSR with rows sorted by neighbour id,
+//       plus a vertex -> face CSR (rows sorted by face id), both from the
+//       triangle list (the edge graph that vtkSurfaceBase builds in
+//       Common/vtkSurfaceBase.cxx:1166-1221, 1407-1468);
+//   (b) per-vertex items: FP32 position and clamped weight
+//       w = clamp(area * indicator^gradation) with area = sum of adjacent face
+//       areas / 3 (Common/vtkSurface.cxx:1280-1283, 1342-1359;
+//       vtkIsotropicMetricForClustering.h:249-291 clamp ratio 1e5,
+//       vtkQEMetricForClustering.h:324-362 clamp ratio 1e4) and, for QEM
+//       meshes, the vertex quadric = sum of the adjacent triangles' plane
+//       quadrics (vtkQEMetricForClustering.h:151-167,
+//       Common/vtkQuadricTools.cxx:56-79), 10 coefficients;
+//   (c) deterministic per-mesh reductions (fixed block partition, fixed tree)
+//       and the power-of-two fixed-point scales derived from them.
+// All FP64 accumulation happens in a fixed order so that a build is
+// reproducible run to run.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "acvd_ctx.h"
+
+namespace acvd {
+
+namespace {
+
+inline unsigned grid_for(int64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+__global__ void k_csr_count(int64_t nf, const int32_t* __restrict__ tri, int32_t* deg2, int32_t* vfdeg) {
+  int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  for (int j = 0; j < 3; ++j) {
+    int32_t v = tri[3 * f + j];
+    atomicAdd(&deg2[v], 2);
+    atomicAdd(&vfdeg[v], 1);
+  }
+}
+
+__global__ void k_csr_fill(int64_t nf, const int32_t* __restrict__ tri, const int32_t* __restrict__ tptr,
+                           int32_t* tfill, int32_t* tcol, const int32_t* __restrict__ vfptr, int32_t* vffill,
+                           int32_t* vfcol) {
+  int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int32_t v[3] = {tri[3 * f], tri[3 * f + 1], tri[3 * f + 2]};
+  for (int j = 0; j < 3; ++j) {
+    int32_t a = v[j], b = v[(j + 1) % 3], c = v[(j + 2) % 3];
+    int32_t slot = tptr[a] + atomicAdd(&tfill[a], 2);
+    tcol[slot] = b;
+    tcol[slot + 1] = c;
+    int32_t s2 = vfptr[a] + atomicAdd(&vffill[a], 1);
+    vfcol[s2] = int32_t(f);
+  }
+}
+
+// insertion sort of a short row held in global memory
+__device__ void sort_row(int32_t* r, int32_t n) {
+  for (int32_t i = 1; i < n; ++i) {
+    int32_t x = r[i];
+    int32_t j = i - 1;
+    while (j >= 0 && r[j] > x) {
+      r[j + 1] = r[j];
+      --j;
+    }
+    r[j + 1] = x;
+  }
+}
+
+__global__ void k_csr_sort_unique(int64_t n, const int32_t* __restrict__ tptr, int32_t* tcol, int32_t* deg,
+                                  const int32_t* __restrict__ vfptr, int32_t* vfcol) {
+  int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int32_t* r = tcol + tptr[v];
+  int32_t len = tptr[v + 1] - tptr[v];
+  sort_row(r, len);
+  int32_t u = 0;
+  for (int32_t i = 0; i < len; ++i) {
+    if (i > 0 && r[i] == r[i - 1]) continue;
+    r[u++] = r[i];
+  }
+  deg[v] = u;
+  sort_row(vfcol + vfptr[v], vfptr[v + 1] - vfptr[v]);
+}
+
+__global__ void k_csr_compact(int64_t n, const int32_t* __restrict__ tptr, const int32_t* __restrict__ tcol,
+                              const int32_t* __restrict__ row_ptr, int32_t* col) {
+  int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int32_t len = row_ptr[v + 1] - row_ptr[v];
+  const int32_t* src = tcol + tptr[v];
+  int32_t* dst = col + row_ptr[v];
+  for (int32_t i = 0; i < len; ++i) dst[i] = src[i];
+}
+
+// Per-vertex item build (one thread per vertex, faces visited in increasing
+// face id).  raw_w receives the unclamped weight; xw.w is filled later.
+__global__ void k_item_build(SweepArgs a, const double* __restrict__ pos, const int32_t* __restrict__ tri,
+                             const int32_t* __restrict__ vfptr, const int32_t* __restrict__ vfcol,
+                             const double* __restrict__ indicator, double* raw_w, float4* xw, float* q) {
+  int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= a.n_items) return;
+  const int m = a.item_mesh ? a.item_mesh[v] : 0;
+  const DevMesh dm = a.meshes[m];
+  double area = 0.0;
+  double Q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const bool qem = dm.metric == 1;
+  for (int32_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
+    const int64_t f = vfcol[k];
+    const int32_t i0 = tri[3 * f], i1 = tri[3 * f + 1], i2 = tri[3 * f + 2];
+    const double p1[3] = {pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]};
+    const double p2[3] = {pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]};
+    const double p3[3] = {pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]};
+    double ea = (p1[0] - p2[0]) * (p1[0] - p2[0]) + (p1[1] - p2[1]) * (p1[1] - p2[1]) + (p1[2] - p2[2]) * (p1[2] - p2[2]);
+    double eb = (p2[0] - p3[0]) * (p2[0] - p3[0]) + (p2[1] - p3[1]) * (p2[1] - p3[1]) + (p2[2] - p3[2]) * (p2[2] - p3[2]);
+    double ec = (p3[0] - p1[0]) * (p3[0] - p1[0]) + (p3[1] - p1[1]) * (p3[1] - p1[1]) + (p3[2] - p1[2]) * (p3[2] - p1[2]);
+    double t = 4.0 * ea * ec - (ea - eb + ec) * (ea - eb + ec);
+    area += 0.25 * sqrt(t < 0 ? -t : t) / 3.0;
+    if (qem) {
+      // plane of the triangle: n = (p2-p1) x (p3-p1), d = -n.p1
+      double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+      double w[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+      double n0 = u[1] * w[2] - u[2] * w[1];
+      double n1 = u[2] * w[0] - u[0] * w[2];
+      double n2 = u[0] * w[1] - u[1] * w[0];
+      double d = -(n0 * p1[0] + n1 * p1[1] + n2 * p1[2]);
+      Q[0] += n0 * n0; Q[1] += n0 * n1; Q[2] += n0 * n2; Q[3] += n0 * d;
+      Q[4] += n1 * n1; Q[5] += n1 * n2; Q[6] += n1 * d;
+      Q[7] += n2 * n2; Q[8] += n2 * d;  Q[9] += d * d;
+    }
+  }
+  double wgt = area;
+  if (dm.has_indicator && (!qem || dm.gradation > 0.0)) wgt *= pow(indicator[v], dm.gradation);
+  raw_w[v] = wgt;
+  xw[v] = make_float4(float(pos[3 * v]), float(pos[3 * v + 1]), float(pos[3 * v + 2]), 0.0f);
+  if (qem)
+    for (int k = 0; k < 10; ++k) q[10 * v + k] = float(Q[k]);
+}
+
+// Deterministic segmented block reduction.  Block b covers the item range
+// block_tab[b] = (begin, end) of a single mesh.  what:
+//   0: sum raw_w                      (average weight before the clamp)
+//   1: sum w, sum w * max|x|, sum max|q|, max w   (scales)
+__global__ void k_seg_reduce(const int2* __restrict__ block_tab, SweepArgs a, const double* __restrict__ raw_w,
+                             int what, double* partial) {
+  __shared__ double sh[4][kBlock];
+  const int2 r = block_tab[blockIdx.x];
+  double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+  // each thread folds a fixed strided subset in increasing order
+  for (int64_t i = int64_t(r.x) + threadIdx.x; i < r.y; i += kBlock) {
+    if (what == 0) {
+      v0 += raw_w[i];
+    } else {
+      float4 p = a.xw[i];
+      double w = p.w;
+      double mx = fmax(fabs(double(p.x)), fmax(fabs(double(p.y)), fabs(double(p.z))));
+      v0 += w;
+      v1 += w * mx;
+      if (a.q) {
+        const int m = a.item_mesh ? a.item_mesh[i] : 0;
+        if (a.meshes[m].metric == 1) {
+          double qm = 0;
+          for (int k = 0; k < 10; ++k) qm = fmax(qm, fabs(double(a.q[10 * i + k])));
+          v2 += qm;
+        }
+      }
+      v3 = fmax(v3, w);
+    }
+  }
+  sh[0][threadIdx.x] = v0;
+  sh[1][threadIdx.x] = v1;
+  sh[2][threadIdx.x] = v2;
+  sh[3][threadIdx.x] = v3;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+      sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
+      sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) partial[4 * blockIdx.x + k] = sh[k][0];
+}
+
+// power of two p with v * p < 2^60 (v > 0)
+__device__ double pow2_scale_below(double v) {
+  if (!(v > 0.0)) return 1.0;
+  int e;
+  frexp(v, &e);  // v = f * 2^e, f in [0.5, 1)
+  return ldexp(1.0, 60 - e);
+}
+
+// stage 0: per-mesh average of raw weights -> clamp -> xw.w
+// stage 1: per-mesh scales
+__global__ void k_mesh_fold(SweepArgs a, int n_blocks, const int2* __restrict__ block_tab,
+                            const double* __restrict__ partial, int stage, DevMesh* meshes_rw, double* mesh_out) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.n_meshes) return;
+  DevMesh dm = meshes_rw[m];
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    int2 r = block_tab[b];
+    if (r.x < dm.item_off || r.x >= dm.item_off + dm.n_items) continue;
+    s0 += partial[4 * b];
+    s1 += partial[4 * b + 1];
+    s2 += partial[4 * b + 2];
+    s3 = fmax(s3, partial[4 * b + 3]);
+  }
+  if (stage == 0) {
+    mesh_out[m] = s0 / double(dm.n_items);  // average
+  } else {
+    Scales sc;
+    sc.sw = pow2_scale_below(s0);
+    sc.ss = pow2_scale_below(s1);
+    sc.sq = pow2_scale_below(s2);
+    sc.isw = 1.0 / sc.sw;
+    sc.iss = 1.0 / sc.ss;
+    sc.isq = 1.0 / sc.sq;
+    dm.scales = sc;
+    meshes_rw[m] = dm;
+  }
+}
+
+__global__ void k_clamp(SweepArgs a, const double* __restrict__ raw_w, const double* __restrict__ avg,
+                        float4* xw) {
+  int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= a.n_items) return;
+  const int m = a.item_mesh ? a.item_mesh[v] : 0;
+  const double ratio = a.meshes[m].clamp_ratio;
+  const double av = avg[m];
+  const double mn = av / ratio, mx = av * ratio;
+  double w = raw_w[v];
+  if (w > mx) w = mx;
+  if (w < mn) w = mn;
+  xw[v].w = float(w);
+}
+
+}  // namespace
+
+hipError_t launch_csr_count(hipStream_t s, int64_t nf, const int32_t* tri, int32_t* deg2, int32_t* vfdeg) {
+  if (nf == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_count, dim3(grid_for(nf)), dim3(kBlock), 0, s, nf, tri, deg2, vfdeg);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_fill(hipStream_t s, int64_t nf, const int32_t* tri, const int32_t* tptr, int32_t* tfill,
+                           int32_t* tcol, const int32_t* vfptr, int32_t* vffill, int32_t* vfcol) {
+  if (nf == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(nf)), dim3(kBlock), 0, s, nf, tri, tptr, tfill, tcol, vfptr,
+                     vffill, vfcol);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_sort_unique(hipStream_t s, int64_t n, const int32_t* tptr, int32_t* tcol, int32_t* deg,
+                                  const int32_t* vfptr, int32_t* vfcol) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_sort_unique, dim3(grid_for(n)), dim3(kBlock), 0, s, n, tptr, tcol, deg, vfptr, vfcol);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_compact(hipStream_t s, int64_t n, const int32_t* tptr, const int32_t* tcol,
+                              const int32_t* row_ptr, int32_t* col) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_compact, dim3(grid_for(n)), dim3(kBlock), 0, s, n, tptr, tcol, row_ptr, col);
+  return hipGetLastError();
+}
+
+// out[0..n] = exclusive prefix sum of in[0..n-1] (out has n + 1 entries: the
+// input is read with one trailing zero appended by the caller).
+hipError_t exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, void** tmp,
+                              size_t* tmp_bytes) {
+  size_t need = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, int(n), s);
+  if (e != hipSuccess) return e;
+  if (need > *tmp_bytes) {
+    if (*tmp) (void)hipFree(*tmp);
+    *tmp = nullptr;
+    e = hipMalloc(tmp, need);
+    if (e != hipSuccess) return e;
+    *tmp_bytes = need;
+  }
+  return hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, int(n), s);
+}
+
+hipError_t launch_item_build(hipStream_t s, const SweepArgs& a, const double* pos, const int32_t* tri,
+                             const int32_t* vfptr, const int32_t* vfcol, const double* indicator, double* raw_w,
+                             float4* xw, float* q) {
+  if (a.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_item_build, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, pos, tri, vfptr, vfcol,
+                     indicator, raw_w, xw, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_reduce(hipStream_t s, int n_blocks, const int2* block_tab, const SweepArgs& a,
+                             const double* raw_w, int what, double* partial) {
+  if (n_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_reduce, dim3(n_blocks), dim3(kBlock), 0, s, block_tab, a, raw_w, what, partial);
+  return hipGetLastError();
+}
+
+// stage 0: fold partial sums into per-mesh averages (into mesh_tmp) and clamp
+// the weights; stage 1: fold into scales.
+hipError_t launch_clamp_and_scales(hipStream_t s, const SweepArgs& a, int n_blocks, const int2* block_tab,
+                                   const double* partial, int stage, const double* raw_w, DevMesh* meshes_rw) {
+  // mesh_tmp lives after the partial array (caller sizes it)
+  double* mesh_tmp = const_cast<double*>(partial) + 4 * size_t(n_blocks);
+  hipLaunchKernelGGL(k_mesh_fold, dim3((a.n_meshes + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, n_blocks,
+                     block_tab, partial, stage, meshes_rw, mesh_tmp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || stage != 0) return e;
+  hipLaunchKernelGGL(k_clamp, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, raw_w, mesh_tmp,
+                     const_cast<float4*>(a.xw));
+  return hipGetLastError();
+}
+
+}  // namespace acvd

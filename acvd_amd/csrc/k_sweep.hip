@@ -1,0 +1,555 @@
+// Boundary sweep of the vertex clustering, one pass =
+//   propose  : every boundary item whose clusters changed recently evaluates
+//              its best move to a neighbouring cluster (isotropic or QEM
+//              energy, vtkUniformClustering.h:922-991) and, if it lowers the
+//              energy, registers it with an atomicMin of its accept key at the
+//              source and target clusters;
+//   apply    : a candidate whose key is the minimum at both clusters is
+//              applied: label write plus int64 fixed-point atomics on the two
+//              cluster accumulators, then the two cluster summaries (centroid,
+//              W, and the FP64 rank-truncated QEM point) are refreshed;
+//   controller: per-mesh phase logic of MinimizeEnergy
+//              (vtkUniformClustering.h:724-830) after the pass.
+// Since a cluster takes part in at most one applied move per pass, every
+// applied dE is exact with respect to the state at the start of the pass and
+// the energy decreases monotonically.  Between convergences the host runs the
+// clean (largest connected component per cluster, vtkUniformClustering.h:405-549)
+// and fill (regrowth of null items, :551-633) kernels below.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "acvd_ctx.h"
+
+namespace acvd {
+
+namespace {
+
+inline unsigned grid_for(int64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+__device__ inline int mesh_of_item(const SweepArgs& a, int64_t i) { return a.item_mesh ? a.item_mesh[i] : 0; }
+__device__ inline int mesh_of_cluster(const SweepArgs& a, int64_t c) { return a.clus_mesh ? a.clus_mesh[c] : 0; }
+
+__device__ inline int64_t ld_i64(const unsigned long long* p) { return int64_t(*p); }
+
+// wave-aggregated atomic increment of a 32-bit counter; returns this lane's slot
+__device__ inline int32_t wave_append(int32_t* counter, bool want) {
+  const unsigned long long mask = __ballot(want);
+  if (!want) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+  base = __shfl(base, leader);
+  return base + rank;
+}
+
+__device__ inline void block_count_add(unsigned long long* dst, unsigned v) {
+  // wave reduction then one atomic per wave
+  unsigned x = v;
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  if (__lane_id() == 0 && x) atomicAdd(dst, (unsigned long long)x);
+}
+
+// ----------------------------------------------------------------- statistics
+__global__ void k_zero_clusters(SweepArgs a, const int32_t* __restrict__ mask) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= a.n_clusters) return;
+  if (!mask[mesh_of_cluster(a, c)]) return;
+  for (int k = 0; k < 4; ++k) a.acc_ws[4 * c + k] = 0ull;
+  if (a.acc_q)
+    for (int k = 0; k < 10; ++k) a.acc_q[10 * c + k] = 0ull;
+  a.sizes[c] = 0;
+}
+
+// Item contributions in int64 fixed point (order-independent sums).
+__global__ void k_accumulate(SweepArgs a, const int32_t* __restrict__ mask, unsigned long long* mesh_count) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items) return;
+  const int m = mesh_of_item(a, i);
+  if (!mask[m]) return;
+  const int32_t c = a.labels[i];
+  if (c == kNull) return;
+  const DevMesh& dm = a.meshes[m];
+  const float4 p = a.xw[i];
+  const Scales sc = dm.scales;
+  atomicAdd(&a.acc_ws[4 * int64_t(c)], (unsigned long long)fix_w(p.w, sc));
+  atomicAdd(&a.acc_ws[4 * int64_t(c) + 1], (unsigned long long)fix_s(p.w, p.x, sc));
+  atomicAdd(&a.acc_ws[4 * int64_t(c) + 2], (unsigned long long)fix_s(p.w, p.y, sc));
+  atomicAdd(&a.acc_ws[4 * int64_t(c) + 3], (unsigned long long)fix_s(p.w, p.z, sc));
+  if (dm.metric == 1) {
+    for (int k = 0; k < 10; ++k)
+      atomicAdd(&a.acc_q[10 * int64_t(c) + k], (unsigned long long)fix_q(a.q[10 * i + k], sc));
+  }
+  atomicAdd(&a.sizes[c], 1);
+  if (mesh_count) atomicAdd(&mesh_count[m], 1ull);
+}
+
+__device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm, int constrained) {
+  const int64_t W = ld_i64(&a.acc_ws[4 * c]);
+  const ClusterGeom g = geom_from_fixed(W, ld_i64(&a.acc_ws[4 * c + 1]), ld_i64(&a.acc_ws[4 * c + 2]),
+                                        ld_i64(&a.acc_ws[4 * c + 3]), dm.scales);
+  a.geom[c] = g;
+  if (dm.metric == 1) {
+    double T = 0.0;
+    if (constrained) {
+      int64_t q10[10];
+      for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
+      T = qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr);
+    }
+    a.T[c] = T;
+  }
+}
+
+__global__ void k_derive(SweepArgs a, const int32_t* __restrict__ mask, int set_stamp) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= a.n_clusters) return;
+  const int m = mesh_of_cluster(a, c);
+  if (!mask[m]) return;
+  const DevMesh dm = a.meshes[m];
+  derive_cluster(a, c, dm, a.state[m].constrained);
+  if (set_stamp) a.stamp[c] = a.state[m].loops;  // SetAllClustersToModified
+}
+
+// ------------------------------------------------------------------- propose
+__device__ inline SweepView make_view(const SweepArgs& a) {
+  SweepView v;
+  v.row_ptr = a.row_ptr;
+  v.col = a.col;
+  v.labels = a.labels;
+  v.xw = reinterpret_cast<const float*>(a.xw);
+  v.q = a.q;
+  v.acc_ws = a.acc_ws;
+  v.acc_q = a.acc_q;
+  v.geom = a.geom;
+  v.T = a.T;
+  v.sizes = a.sizes;
+  v.stamp = a.stamp;
+  v.cand_pass = a.cand_pass;
+  return v;
+}
+
+template <int METRIC>
+__global__ __launch_bounds__(kBlock) void k_propose(SweepArgs a) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool evaluated = false;
+  bool proposed = false;
+  int32_t target = kNull;
+  if (i < a.n_items) {
+    const int m = mesh_of_item(a, i);
+    const DevMesh& dm = a.meshes[m];
+    const MeshState& st = a.state[m];
+    if (dm.metric == METRIC && st.phase == kPhaseSweep) {
+      const SweepView v = make_view(a);
+      const int32_t L = st.loops;
+      const int32_t lab = a.labels[i];
+      if (needs_eval(v, i, lab, L)) {
+        evaluated = true;
+        proposed = propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target);
+        if (proposed) {
+          const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
+          if (lab != kNull) atomicMin(&a.prio[lab], key);
+          atomicMin(&a.prio[target], key);
+          a.cand_pass[i] = L;
+        }
+      }
+    }
+  }
+  const int32_t slot = wave_append(&a.ctrl->cand_count, proposed);
+  if (proposed) a.cand[slot] = make_int2(int32_t(i), target);
+  block_count_add(&a.ctrl->evaluated, evaluated ? 1u : 0u);
+  block_count_add(&a.ctrl->proposed, proposed ? 1u : 0u);
+}
+
+// --------------------------------------------------------------------- apply
+__device__ void update_cluster(const SweepArgs& a, int64_t c, int sign, int64_t iw, const int64_t* is,
+                               const int64_t* iq, const DevMesh& dm, int constrained) {
+  const unsigned long long s = (unsigned long long)int64_t(sign);
+  int64_t W = int64_t(atomicAdd(&a.acc_ws[4 * c], (unsigned long long)iw * s)) + sign * iw;
+  int64_t Sx = int64_t(atomicAdd(&a.acc_ws[4 * c + 1], (unsigned long long)is[0] * s)) + sign * is[0];
+  int64_t Sy = int64_t(atomicAdd(&a.acc_ws[4 * c + 2], (unsigned long long)is[1] * s)) + sign * is[1];
+  int64_t Sz = int64_t(atomicAdd(&a.acc_ws[4 * c + 3], (unsigned long long)is[2] * s)) + sign * is[2];
+  const ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, dm.scales);
+  a.geom[c] = g;
+  if (dm.metric == 1) {
+    int64_t q10[10];
+    for (int k = 0; k < 10; ++k)
+      q10[k] = int64_t(atomicAdd(&a.acc_q[10 * c + k], (unsigned long long)iq[k] * s)) + sign * iq[k];
+    a.T[c] = constrained ? qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr) : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a) {
+  const int32_t n = a.ctrl->cand_count;
+  unsigned accepted = 0;
+  for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int2 cd = a.cand[t];
+    const int64_t i = cd.x;
+    const int32_t b = cd.y;
+    const int m = mesh_of_item(a, i);
+    const DevMesh dm = a.meshes[m];
+    MeshState* st = &a.state[m];
+    const int32_t L = st->loops;
+    const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
+    const int32_t lab = a.labels[i];
+    if (lab != kNull && a.prio[lab] != key) continue;
+    if (a.prio[b] != key) continue;
+    // exclusive owner of clusters lab and b for this pass
+    a.labels[i] = b;
+    const float4 p = a.xw[i];
+    const Scales sc = dm.scales;
+    const int64_t iw = fix_w(p.w, sc);
+    const int64_t is[3] = {fix_s(p.w, p.x, sc), fix_s(p.w, p.y, sc), fix_s(p.w, p.z, sc)};
+    int64_t iq[10];
+    if (dm.metric == 1)
+      for (int k = 0; k < 10; ++k) iq[k] = fix_q(a.q[10 * i + k], sc);
+    const int constrained = st->constrained;
+    if (lab != kNull) {
+      update_cluster(a, lab, -1, iw, is, iq, dm, constrained);
+      atomicSub(&a.sizes[lab], 1);
+      a.stamp[lab] = L;
+    }
+    update_cluster(a, b, +1, iw, is, iq, dm, constrained);
+    atomicAdd(&a.sizes[b], 1);
+    a.stamp[b] = L;
+    atomicAdd(&st->pass_mods, 1);
+    ++accepted;
+  }
+  block_count_add(&a.ctrl->accepted, accepted);
+}
+
+// ---------------------------------------------------------------- controller
+__global__ void k_controller(SweepArgs a) {
+  __shared__ int any_sweep, any_clean;
+  if (threadIdx.x == 0) {
+    any_sweep = 0;
+    any_clean = 0;
+  }
+  __syncthreads();
+  bool ran = false;
+  for (int m = threadIdx.x; m < a.n_meshes; m += blockDim.x) {
+    MeshState s = a.state[m];
+    if (s.phase == kPhaseSweep) {
+      ran = true;
+      controller_step(s);
+      a.state[m] = s;
+    }
+    if (s.phase == kPhaseSweep) atomicOr(&any_sweep, 1);
+    if (s.phase == kPhaseNeedClean) atomicOr(&any_clean, 1);
+  }
+  if (ran) atomicOr(&any_sweep, 2);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.ctrl->cand_count = 0;
+    a.ctrl->any_sweep = any_sweep & 1;
+    a.ctrl->any_need_clean = any_clean;
+    if (any_sweep & 2) a.ctrl->passes += 1;
+  }
+}
+
+// ---------------------------------------------------------------------- fill
+// One synchronous round: every null item with a labelled neighbour takes the
+// label of its lowest-id labelled neighbour (as of the start of the round).
+__global__ void k_fill_pick(SweepArgs a, const int32_t* __restrict__ mask, int32_t* pending) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool changed = false;
+  if (i < a.n_items && mask[mesh_of_item(a, i)] && a.labels[i] == kNull) {
+    int32_t pick = kNull;
+    for (int32_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+      const int32_t b = a.labels[a.col[k]];
+      if (b != kNull) {
+        pick = b;
+        break;
+      }
+    }
+    pending[i] = pick;
+    changed = pick != kNull;
+  }
+  if (__ballot(changed) != 0 && __lane_id() == 0) a.ctrl->fill_changed = 1;
+}
+
+__global__ void k_fill_commit(SweepArgs a, const int32_t* __restrict__ mask, const int32_t* __restrict__ pending) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items || !mask[mesh_of_item(a, i)] || a.labels[i] != kNull) return;
+  a.labels[i] = pending[i];
+}
+
+// --------------------------------------------------- connected components
+// Union-find over same-label edges; a root is always the smallest item id of
+// its tree (larger roots are hooked under smaller ones), so the final
+// representative of a component is its minimum item id.
+__device__ int32_t cc_find(int32_t* parent, int32_t x) {
+  while (true) {
+    int32_t p = parent[x];
+    if (p == x) return x;
+    int32_t gp = parent[p];
+    if (gp != p) parent[x] = gp;  // path halving (benign race: ancestors only)
+    x = gp;
+  }
+}
+
+__global__ void k_cc_init(SweepArgs a, const int32_t* __restrict__ mask, int32_t* parent, int32_t* comp_size) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items || !mask[mesh_of_item(a, i)]) return;
+  parent[i] = int32_t(i);
+  comp_size[i] = 0;
+}
+
+__global__ void k_cc_hook(SweepArgs a, const int32_t* __restrict__ mask, int32_t* parent) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items || !mask[mesh_of_item(a, i)]) return;
+  const int32_t lab = a.labels[i];
+  if (lab == kNull) return;
+  for (int32_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+    const int32_t j = a.col[k];
+    if (j <= i || a.labels[j] != lab) continue;
+    int32_t u = int32_t(i), v = j;
+    while (true) {
+      u = cc_find(parent, u);
+      v = cc_find(parent, v);
+      if (u == v) break;
+      if (u > v) {
+        int32_t t = u;
+        u = v;
+        v = t;
+      }
+      // hook root v under the smaller root u
+      if (atomicCAS(&parent[v], v, u) == v) break;
+    }
+  }
+}
+
+__global__ void k_cc_count(SweepArgs a, const int32_t* __restrict__ mask, int32_t* parent, int32_t* comp_size) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items || !mask[mesh_of_item(a, i)]) return;
+  if (a.labels[i] == kNull) return;
+  const int32_t r = cc_find(parent, int32_t(i));
+  atomicAdd(&comp_size[r], 1);
+}
+
+__global__ void k_cc_best(SweepArgs a, const int32_t* __restrict__ mask, const int32_t* __restrict__ parent,
+                          const int32_t* __restrict__ comp_size, unsigned long long* best, int32_t* ncomp) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items || !mask[mesh_of_item(a, i)]) return;
+  const int32_t lab = a.labels[i];
+  if (lab == kNull || parent[i] != i) return;
+  // larger component first, then the smaller first item
+  const unsigned long long key = (uint64_t(uint32_t(comp_size[i])) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(i));
+  atomicMax(&best[lab], key);
+  atomicAdd(&ncomp[lab], 1);
+}
+
+__global__ void k_cc_reset(SweepArgs a, const int32_t* __restrict__ mask, int32_t* parent,
+                           const unsigned long long* __restrict__ best) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items || !mask[mesh_of_item(a, i)]) return;
+  const int32_t lab = a.labels[i];
+  if (lab == kNull) return;
+  const int32_t r = cc_find(parent, int32_t(i));
+  const uint32_t keep = 0xFFFFFFFFu - uint32_t(best[lab] & 0xFFFFFFFFull);
+  if (uint32_t(r) != keep) a.labels[i] = kNull;
+}
+
+__global__ void k_cc_zero(SweepArgs a, const int32_t* __restrict__ mask, unsigned long long* best, int32_t* ncomp) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= a.n_clusters || !mask[mesh_of_cluster(a, c)]) return;
+  best[c] = 0ull;
+  ncomp[c] = 0;
+}
+
+__global__ void k_cc_disc(SweepArgs a, const int32_t* __restrict__ mask, const int32_t* __restrict__ ncomp,
+                          int32_t* ndisc) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= a.n_clusters) return;
+  const int m = mesh_of_cluster(a, c);
+  if (!mask[m]) return;
+  if (ncomp[c] > 1) atomicAdd(&ndisc[m], 1);
+}
+
+// ----------------------------------------------------------------- seeding
+// Weighted random seeds: key_i = E_i / w_i with E_i ~ Exp(1); the K smallest
+// keys of a mesh (one sort) seed its clusters, so that denser regions get
+// more seeds.
+__global__ void k_seed_keys(SweepArgs a, int64_t off, int64_t n, uint32_t seed, uint32_t* keys, int32_t* vals) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = off + t;
+  uint32_t h = prio_hash(seed ^ 0x5EEDu, 0x9E37u, uint32_t(t));
+  double u = (double(h) + 0.5) * (1.0 / 4294967296.0);
+  double w = a.xw[i].w;
+  float k = float(-log(u) / (w > 0 ? w : 1e-300));
+  keys[t] = __float_as_uint(k);  // positive floats order like their bits
+  vals[t] = int32_t(t);
+}
+
+__global__ void k_seed_assign(int32_t* labels, int64_t item_off, int64_t clus_off, int32_t K,
+                              const int32_t* __restrict__ sorted_vals) {
+  int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= K) return;
+  labels[item_off + sorted_vals[r]] = int32_t(clus_off + r);
+}
+
+// ----------------------------------------------------------------- outputs
+// Cluster point of mesh `mesh`: centroid (isotropic) or the QEM point when the
+// metric is constrained (GetClusterCentroid semantics).
+__global__ void k_cluster_points(SweepArgs a, int32_t mesh, double* pts) {
+  const DevMesh dm = a.meshes[mesh];
+  int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= dm.K) return;
+  const int64_t c = dm.clus_off + r;
+  const int64_t W = ld_i64(&a.acc_ws[4 * c]);
+  const ClusterGeom g = geom_from_fixed(W, ld_i64(&a.acc_ws[4 * c + 1]), ld_i64(&a.acc_ws[4 * c + 2]),
+                                        ld_i64(&a.acc_ws[4 * c + 3]), dm.scales);
+  double q[3] = {g.cx, g.cy, g.cz};
+  if (!(g.W > 0.0)) {
+    q[0] = q[1] = q[2] = __builtin_nan("");
+  } else if (dm.metric == 1 && a.state[mesh].constrained) {
+    int64_t q10[10];
+    for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
+    qem_T_from_fixed(q10, g, dm.scales, dm.level, q);
+  }
+  pts[3 * r] = q[0];
+  pts[3 * r + 1] = q[1];
+  pts[3 * r + 2] = q[2];
+}
+
+// Per-item unshifted energy w |x - p_c|^2 with p_c given per global cluster.
+__global__ void k_item_energy(SweepArgs a, const double* __restrict__ pts_all, double* e_item) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items) return;
+  const int32_t c = a.labels[i];
+  if (c == kNull) {
+    e_item[i] = 0.0;
+    return;
+  }
+  const float4 p = a.xw[i];
+  double dx = double(p.x) - pts_all[3 * int64_t(c)];
+  double dy = double(p.y) - pts_all[3 * int64_t(c) + 1];
+  double dz = double(p.z) - pts_all[3 * int64_t(c) + 2];
+  e_item[i] = double(p.w) * (dx * dx + dy * dy + dz * dz);
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+hipError_t launch_zero_clusters(hipStream_t s, const SweepArgs& a, const int32_t* mask) {
+  if (a.n_clusters == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_clusters, dim3(grid_for(a.n_clusters)), dim3(kBlock), 0, s, a, mask);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* mask, unsigned long long* mesh_count) {
+  if (a.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, mask, mesh_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mask, int set_stamp) {
+  if (a.n_clusters == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_derive, dim3(grid_for(a.n_clusters)), dim3(kBlock), 0, s, a, mask, set_stamp);
+  return hipGetLastError();
+}
+
+hipError_t launch_propose(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem) {
+  if (a.n_items == 0) return hipSuccess;
+  if (any_iso) hipLaunchKernelGGL(k_propose<0>, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a);
+  if (any_qem) hipLaunchKernelGGL(k_propose<1>, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid) {
+  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_controller(hipStream_t s, const SweepArgs& a) {
+  hipLaunchKernelGGL(k_controller, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mask, int32_t* pending) {
+  if (a.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_pick, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, mask, pending);
+  hipLaunchKernelGGL(k_fill_commit, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, mask, pending);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc(hipStream_t s, const SweepArgs& a, const int32_t* mask, int32_t* parent, int32_t* comp_size,
+                     unsigned long long* best, int32_t* ncomp, int32_t* ndisc) {
+  if (a.n_items == 0) return hipSuccess;
+  const dim3 gi(grid_for(a.n_items)), gc(grid_for(a.n_clusters)), b(kBlock);
+  hipLaunchKernelGGL(k_cc_init, gi, b, 0, s, a, mask, parent, comp_size);
+  hipLaunchKernelGGL(k_cc_zero, gc, b, 0, s, a, mask, best, ncomp);
+  hipLaunchKernelGGL(k_cc_hook, gi, b, 0, s, a, mask, parent);
+  hipLaunchKernelGGL(k_cc_count, gi, b, 0, s, a, mask, parent, comp_size);
+  hipLaunchKernelGGL(k_cc_best, gi, b, 0, s, a, mask, parent, comp_size, best, ncomp);
+  hipLaunchKernelGGL(k_cc_reset, gi, b, 0, s, a, mask, parent, best);
+  hipLaunchKernelGGL(k_cc_disc, gc, b, 0, s, a, mask, ncomp, ndisc);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_keys(hipStream_t s, const SweepArgs& a, int64_t off, int64_t n, uint32_t seed,
+                            uint32_t* keys, int32_t* vals) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seed_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, a, off, n, seed, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_assign(hipStream_t s, int32_t* labels, int64_t item_off, int64_t clus_off, int32_t K,
+                              const int32_t* sorted_vals) {
+  if (K == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seed_assign, dim3(grid_for(K)), dim3(kBlock), 0, s, labels, item_off, clus_off, K,
+                     sorted_vals);
+  return hipGetLastError();
+}
+
+hipError_t sort_pairs_u32(hipStream_t s, const uint32_t* kin, uint32_t* kout, const int32_t* vin, int32_t* vout,
+                          int64_t n, void** tmp, size_t* tmp_bytes) {
+  size_t need = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, vin, vout, int(n), 0, 32, s);
+  if (e != hipSuccess) return e;
+  if (need > *tmp_bytes) {
+    if (*tmp) (void)hipFree(*tmp);
+    *tmp = nullptr;
+    e = hipMalloc(tmp, need);
+    if (e != hipSuccess) return e;
+    *tmp_bytes = need;
+  }
+  return hipcub::DeviceRadixSort::SortPairs(*tmp, need, kin, kout, vin, vout, int(n), 0, 32, s);
+}
+
+hipError_t launch_cluster_points(hipStream_t s, const SweepArgs& a, int32_t mesh, double* pts) {
+  hipLaunchKernelGGL(k_cluster_points, dim3(unsigned((a.n_clusters + kBlock - 1) / kBlock) + 1), dim3(kBlock), 0, s,
+                     a, mesh, pts);
+  return hipGetLastError();
+}
+
+hipError_t launch_item_energy(hipStream_t s, const SweepArgs& a, const double* pts_all, double* e_item) {
+  if (a.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_item_energy, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, pts_all, e_item);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace acvd

@@ -1,0 +1,310 @@
+"""Drop-in mirror of the reference's remeshing classes on the MI355X path.
+
+Reference classes (DiscreteRemeshing/vtkIsotropicDiscreteRemeshing.h:42-102):
+  vtkIsotropicDiscreteRemeshing  = vtkVerticesProcessing< vtkDiscreteRemeshing< vtkIsotropicMetricForClustering > >
+  vtkQIsotropicDiscreteRemeshing = vtkVerticesProcessing< vtkDiscreteRemeshing< vtkQEMetricForClustering > >
+The method names, argument meaning and error behaviour follow the reference
+(vtkUniformClustering.h:62-134, vtkDiscreteRemeshing.h:61-83,
+vtkSurfaceClustering.h:58); the work runs in libacvd_amd.so through the
+C-ABI of include/acvd_c.h.  `Context` exposes the batched form (many meshes
+clustered by one call).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from ._native import AcvdError, check
+
+_p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+
+
+class Mesh:
+    """Triangle mesh (replaces vtkSurface for the clustering path)."""
+
+    def __init__(self, points, triangles):
+        pts = np.asarray(points)
+        if pts.dtype not in (np.float32, np.float64):
+            pts = pts.astype(np.float64)
+        self.points = np.ascontiguousarray(pts).reshape(-1, 3)
+        self.triangles = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+
+    def GetNumberOfPoints(self):
+        return len(self.points)
+
+    def GetNumberOfCells(self):
+        return len(self.triangles)
+
+    def GetPoint(self, i):
+        return tuple(float(v) for v in self.points[i])
+
+
+class Context:
+    """One library context (one HIP stream) holding one or more meshes."""
+
+    def __init__(self, device=0):
+        L = N.lib()
+        h = ctypes.c_void_p()
+        check(L.acvd_create(ctypes.byref(h), device))
+        self._h = h
+        self._nv = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().acvd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def _mesh_args(points, triangles):
+        pts = np.ascontiguousarray(points)
+        if pts.dtype not in (np.float32, np.float64):
+            pts = pts.astype(np.float64)
+        pts = pts.reshape(-1, 3)
+        tri = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+        return pts, tri
+
+    def set_input(self, points, triangles):
+        pts, tri = self._mesh_args(points, triangles)
+        f = N.lib().acvd_set_input_f32 if pts.dtype == np.float32 else N.lib().acvd_set_input_f64
+        check(f(self._h, len(pts), _p(pts), len(tri), _p(tri)))
+        self._nv = [len(pts)]
+        return 0
+
+    def add_input(self, points, triangles):
+        pts, tri = self._mesh_args(points, triangles)
+        f = N.lib().acvd_add_input_f32 if pts.dtype == np.float32 else N.lib().acvd_add_input_f64
+        mid = ctypes.c_int32()
+        check(f(self._h, len(pts), _p(pts), len(tri), _p(tri), ctypes.byref(mid)))
+        self._nv.append(len(pts))
+        return mid.value
+
+    @property
+    def num_meshes(self):
+        return len(self._nv)
+
+    def set_params(self, mesh_id=-1, **kw):
+        p = N.Params()
+        if mesh_id >= 0:
+            check(N.lib().acvd_get_params(self._h, mesh_id, ctypes.byref(p)))
+        else:
+            N.lib().acvd_params_init(ctypes.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(f"unknown parameter {k}")
+            setattr(p, k, v)
+        check(N.lib().acvd_set_params(self._h, mesh_id, ctypes.byref(p)))
+
+    def get_params(self, mesh_id=0):
+        p = N.Params()
+        check(N.lib().acvd_get_params(self._h, mesh_id, ctypes.byref(p)))
+        return {k: getattr(p, k) for k, _ in p._fields_}
+
+    def set_indicator(self, mesh_id, indicator):
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        if ind is not None and len(ind) != self._nv[mesh_id]:
+            raise ValueError("indicator length must equal the number of vertices")
+        check(N.lib().acvd_set_indicator(self._h, mesh_id, _p(ind)))
+
+    def set_initial_labels(self, mesh_id, labels):
+        lab = None if labels is None else np.ascontiguousarray(labels, np.int32)
+        if lab is not None and len(lab) != self._nv[mesh_id]:
+            raise ValueError("initial clustering length must equal the number of vertices")
+        check(N.lib().acvd_set_initial_labels(self._h, mesh_id, _p(lab)))
+
+    def build(self):
+        check(N.lib().acvd_build(self._h))
+
+    def process(self):
+        check(N.lib().acvd_process_clustering(self._h))
+
+    def labels(self, mesh_id=0):
+        out = np.empty(self._nv[mesh_id], np.int32)
+        check(N.lib().acvd_get_labels(self._h, mesh_id, _p(out)))
+        return out
+
+    def cluster_sizes(self, mesh_id=0):
+        K = self.get_params(mesh_id)["number_of_clusters"]
+        out = np.empty(K, np.int32)
+        check(N.lib().acvd_get_cluster_sizes(self._h, mesh_id, _p(out)))
+        return out
+
+    def cluster_points(self, mesh_id=0):
+        K = self.get_params(mesh_id)["number_of_clusters"]
+        out = np.empty((K, 3), np.float64)
+        check(N.lib().acvd_get_cluster_points(self._h, mesh_id, _p(out)))
+        return out
+
+    def energy(self, mesh_id=0):
+        e = np.zeros(1, np.float64)
+        check(N.lib().acvd_get_energy(self._h, mesh_id, _p(e)))
+        return float(e[0])
+
+    def stats(self, mesh_id=0):
+        st = N.Stats()
+        check(N.lib().acvd_get_stats(self._h, mesh_id, ctypes.byref(st)))
+        return st.as_dict()
+
+    def adjacency(self, mesh_id=0):
+        n = ctypes.c_int64()
+        check(N.lib().acvd_get_adjacency_size(self._h, mesh_id, ctypes.byref(n)))
+        rp = np.empty(self._nv[mesh_id] + 1, np.int32)
+        col = np.empty(n.value, np.int32)
+        check(N.lib().acvd_get_adjacency(self._h, mesh_id, _p(rp), _p(col)))
+        return rp, col
+
+    def items(self, mesh_id=0):
+        nv = self._nv[mesh_id]
+        xw = np.empty((nv, 4), np.float32)
+        q = None
+        if self.get_params(mesh_id)["metric"] == N.METRIC_QEM:
+            q = np.empty((nv, 10), np.float32)
+        check(N.lib().acvd_get_items(self._h, mesh_id, _p(xw), _p(q)))
+        return xw, q
+
+    def scales(self, mesh_id=0):
+        s = np.empty(3, np.float64)
+        check(N.lib().acvd_get_scales(self._h, mesh_id, _p(s)))
+        return s
+
+
+class _MetricProxy:
+    """What GetMetric() returns: the metric-dependent options."""
+
+    def __init__(self, owner):
+        self._o = owner
+
+    def SetGradation(self, g):
+        self._o._params["gradation"] = float(g)
+
+    def GetGradation(self):
+        return self._o._params["gradation"]
+
+    def SetQuadricsOptimizationLevel(self, level):
+        if self._o.METRIC != N.METRIC_QEM:
+            raise AttributeError("SetQuadricsOptimizationLevel exists on the QEM metric only")
+        self._o._params["quadrics_optimization_level"] = int(level)
+
+    def SetCurvatureInfo(self, indicator):
+        """Per-vertex indicator, used as weight = area * indicator^gradation."""
+        self._o._indicator = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+
+
+class _DiscreteRemeshing:
+    METRIC = N.METRIC_ISOTROPIC
+
+    def __init__(self):
+        self._input = None
+        self._params = {"number_of_clusters": 0, "metric": self.METRIC, "quadrics_optimization_level": 3,
+                        "unconstrained_initialization": 0, "max_number_of_loops": 5000000,
+                        "max_number_of_convergences": 1000000000, "console_output": 0, "seed": 0,
+                        "gradation": 0.0}
+        self._indicator = None
+        self._initial = None
+        self._clustering = None
+        self._ctx = None
+        self._subsampling_threshold = 10
+        self._metric = _MetricProxy(self)
+
+    @classmethod
+    def New(cls):
+        return cls()
+
+    def Delete(self):
+        if self._ctx is not None:
+            self._ctx.close()
+            self._ctx = None
+
+    # ---- configuration (vtkUniformClustering.h:62-134) ----
+    def SetInput(self, mesh):
+        if not isinstance(mesh, Mesh):
+            mesh = Mesh(*mesh)
+        self._input = mesh
+
+    def GetInput(self):
+        return self._input
+
+    def SetNumberOfClusters(self, n):
+        self._params["number_of_clusters"] = int(n)
+
+    def GetNumberOfClusters(self):
+        return self._params["number_of_clusters"]
+
+    def SetConsoleOutput(self, v):
+        self._params["console_output"] = int(v)
+
+    def SetUnconstrainedInitialization(self, v):
+        self._params["unconstrained_initialization"] = int(v)
+
+    def SetMaxNumberOfLoops(self, v):
+        self._params["max_number_of_loops"] = int(v)
+
+    def SetMaxNumberOfConvergences(self, v):
+        self._params["max_number_of_convergences"] = int(v)
+
+    def SetInitialClustering(self, labels):
+        self._initial = None if labels is None else np.ascontiguousarray(labels, np.int32)
+
+    def SetSubsamplingThreshold(self, t):
+        self._subsampling_threshold = int(t)
+
+    def SetDisplay(self, d):
+        pass  # no rendering on this path
+
+    def SetSeed(self, s):
+        self._params["seed"] = int(s) & 0xFFFFFFFF
+
+    def GetMetric(self):
+        return self._metric
+
+    def GetNumberOfItems(self):
+        return self._input.GetNumberOfPoints()
+
+    def GetClusteringType(self):
+        return 1  # items are vertices
+
+    # ---- run ----
+    def ProcessClustering(self):
+        """vtkUniformClustering::ProcessClustering; returns the labels (0..K, K = null).
+        Returns None when NumberOfClusters is zero, like the reference (:659-664)."""
+        if self._params["number_of_clusters"] <= 0:
+            return None
+        if self._input is None:
+            raise AcvdError("SetInput must be called first")
+        if self._ctx is None:
+            self._ctx = Context()
+        ctx = self._ctx
+        ctx.set_input(self._input.points, self._input.triangles)
+        ctx.set_params(0, **self._params)
+        ctx.set_indicator(0, self._indicator)
+        ctx.set_initial_labels(0, self._initial)
+        ctx.process()
+        self._clustering = ctx.labels(0)
+        return self._clustering
+
+    def GetClustering(self):
+        return self._clustering
+
+    def GetClusterSize(self, c):
+        return int(self._ctx.cluster_sizes(0)[c])
+
+    def GetClusterPoints(self):
+        return self._ctx.cluster_points(0)
+
+    def GetStats(self):
+        return self._ctx.stats(0)
+
+    def GetContext(self):
+        return self._ctx
+
+
+class IsotropicDiscreteRemeshing(_DiscreteRemeshing):
+    """vtkIsotropicDiscreteRemeshing (ACVD)."""
+    METRIC = N.METRIC_ISOTROPIC
+
+
+class QIsotropicDiscreteRemeshing(_DiscreteRemeshing):
+    """vtkQIsotropicDiscreteRemeshing (ACVDQ)."""
+    METRIC = N.METRIC_QEM

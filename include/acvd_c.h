@@ -1,0 +1,166 @@
+/* C-ABI of the MI355X vertex-clustering library (libacvd_amd.so).
+ *
+ * This is the drop-in boundary for the reference's clustering path:
+ * vtkVerticesProcessing< vtkDiscreteRemeshing< Metric > > with Metric =
+ * vtkIsotropicMetricForClustering (ACVD) or vtkQEMetricForClustering (ACVDQ),
+ * i.e. vtkIsotropicDiscreteRemeshing / vtkQIsotropicDiscreteRemeshing
+ * (DiscreteRemeshing/vtkIsotropicDiscreteRemeshing.h:42-102).  Plain
+ * pointers and sizes only; every call returns ACVD_OK (0) or a negative
+ * status, with a message available from acvd_last_error() (the reference
+ * prints to cout and returns 0 / exits instead, vtkUniformClustering.h:659-664,
+ * Common/vtkSurfaceBase.cxx:1248-1256).
+ *
+ * Threading: one context = one HIP stream; a context is not thread-safe.
+ * Distinct contexts may be used concurrently (as distinct reference objects).
+ *
+ * Labels follow the reference convention: cluster ids 0..K-1 and the null
+ * cluster K (vtkUniformClustering.h:1131-1132).
+ *
+ * A context holds one mesh (acvd_set_input_*) or a batch of meshes
+ * (acvd_add_input_*), all clustered together by acvd_process_clustering().
+ */
+#ifndef ACVD_C_H
+#define ACVD_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACVD_OK 0
+#define ACVD_ERR_ARG -1      /* invalid argument */
+#define ACVD_ERR_STATE -2    /* call not valid in the current state */
+#define ACVD_ERR_HIP -3      /* HIP runtime error (no device, launch failure...) */
+#define ACVD_ERR_ALLOC -4    /* device or host allocation failed */
+#define ACVD_ERR_LIMIT -5    /* input beyond a documented limit */
+
+#define ACVD_METRIC_ISOTROPIC 0 /* vtkIsotropicMetricForClustering */
+#define ACVD_METRIC_QEM 1       /* vtkQEMetricForClustering */
+
+typedef struct acvd_ctx acvd_ctx;
+
+/* Per-mesh clustering parameters.  Field <- reference setter:
+ *   number_of_clusters          <- SetNumberOfClusters (vtkUniformClustering.h:62-69)
+ *   metric                      <- template parameter Metric (vtkIsotropicDiscreteRemeshing.h:42-102)
+ *   quadrics_optimization_level <- GetMetric()->SetQuadricsOptimizationLevel (vtkQEMetricForClustering.h:50-54), default 3 (:370)
+ *   gradation                   <- GetMetric()->SetGradation (vtkIsotropicMetricForClustering.h:73-76, vtkQEMetricForClustering.h:86-90)
+ *   unconstrained_initialization<- SetUnconstrainedInitialization (vtkUniformClustering.h:81)
+ *   max_number_of_loops         <- SetMaxNumberOfLoops (:103), default 5000000 (:1407)
+ *   max_number_of_convergences  <- SetMaxNumberOfConvergences (:99), default 1e9 (:1406)
+ *   console_output              <- SetConsoleOutput (:95)
+ *   seed                        <- (no counterpart) seed of the accept priorities and of the initial sampling
+ */
+typedef struct acvd_params {
+  int32_t number_of_clusters;
+  int32_t metric;
+  int32_t quadrics_optimization_level;
+  int32_t unconstrained_initialization;
+  int32_t max_number_of_loops;
+  int32_t max_number_of_convergences;
+  int32_t console_output;
+  uint32_t seed;
+  double gradation;
+} acvd_params;
+
+/* Statistics of the last acvd_process_clustering() for one mesh. */
+typedef struct acvd_stats {
+  int64_t loops;            /* passes run (NumberOfLoops) */
+  int64_t convergences;     /* NumberOfConvergences */
+  int64_t total_moves;      /* applied moves, all passes */
+  int32_t last_disconnected;/* disconnected clusters found by the last clean */
+  int32_t finished;         /* 1 when the mesh reached its stopping rule */
+  /* context-wide counters (same for every mesh of a batch) */
+  int64_t ctx_passes;       /* passes launched while any mesh was sweeping */
+  int64_t ctx_evaluated;    /* boundary items evaluated */
+  int64_t ctx_proposed;     /* candidate moves proposed */
+  int64_t ctx_accepted;     /* candidate moves applied */
+  double converge_ms;       /* device time from MinimizeEnergy start to the last mesh finishing */
+  double build_ms;          /* device time of the CSR + item build */
+  int64_t n_items;          /* items of this mesh */
+  int64_t n_adjacency;      /* CSR entries of this mesh (2E) */
+} acvd_stats;
+
+/* ---- library -------------------------------------------------------------- */
+const char* acvd_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* acvd_last_error(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int acvd_device_count(void);
+/* Fills p with the reference defaults: K = 0, isotropic, level 3, gradation 0,
+ * unconstrained initialisation off, max loops 5000000, max convergences 1e9,
+ * console output 0, seed 0. */
+void acvd_params_init(acvd_params* p);
+
+/* ---- context -------------------------------------------------------------- */
+/* New() (vtkIsotropicDiscreteRemeshing.h:59-69, 84-94) on HIP device `device`. */
+int acvd_create(acvd_ctx** out, int device);
+/* Delete() (vtkUniformClustering.h:1421-1443). */
+int acvd_destroy(acvd_ctx* ctx);
+
+/* ---- input ---------------------------------------------------------------- */
+/* SetInput(vtkSurface*) (vtkSurfaceClustering.h:58, 178-211): replaces the
+ * context content by one triangle mesh.  xyz: 3*nv coordinates, tri: 3*nf
+ * vertex ids.  The data is copied. */
+int acvd_set_input_f64(acvd_ctx* ctx, int64_t nv, const double* xyz, int64_t nf, const int32_t* tri);
+int acvd_set_input_f32(acvd_ctx* ctx, int64_t nv, const float* xyz, int64_t nf, const int32_t* tri);
+/* Batching (no reference counterpart): appends one more mesh; *mesh_id gets
+ * its index.  All meshes of a context are clustered by one call. */
+int acvd_add_input_f64(acvd_ctx* ctx, int64_t nv, const double* xyz, int64_t nf, const int32_t* tri,
+                       int32_t* mesh_id);
+int acvd_add_input_f32(acvd_ctx* ctx, int64_t nv, const float* xyz, int64_t nf, const int32_t* tri,
+                       int32_t* mesh_id);
+int acvd_num_meshes(const acvd_ctx* ctx);
+
+/* Parameters of mesh `mesh_id` (-1: every mesh). */
+int acvd_set_params(acvd_ctx* ctx, int32_t mesh_id, const acvd_params* p);
+int acvd_get_params(const acvd_ctx* ctx, int32_t mesh_id, acvd_params* p);
+/* Metric::SetCurvatureInfo (vtkQEMetricForClustering.h:75-83,
+ * vtkIsotropicMetricForClustering.h:66-72): per-vertex indicator used as
+ * weight = area * indicator^gradation (:329-336 / :256-259).  NULL clears. */
+int acvd_set_indicator(acvd_ctx* ctx, int32_t mesh_id, const double* indicator);
+/* SetInitialClustering(vtkIntArray*) (vtkUniformClustering.h:336-342,
+ * 1166-1169): labels in 0..K (K = null), one per vertex.  NULL clears (the
+ * library then seeds its own initial clustering on the device). */
+int acvd_set_initial_labels(acvd_ctx* ctx, int32_t mesh_id, const int32_t* labels);
+
+/* ---- run ------------------------------------------------------------------ */
+/* ProcessClustering() (vtkUniformClustering.h:654-714) for every mesh of the
+ * context: device build of CSR + items (first call after an input change),
+ * initial sampling, then MinimizeEnergy (:724-830) with the boundary sweep on
+ * the GPU.  Returns ACVD_ERR_ARG if a mesh has number_of_clusters <= 0. */
+int acvd_process_clustering(acvd_ctx* ctx);
+
+/* ---- results (copies into caller buffers) --------------------------------- */
+/* GetClustering() (vtkUniformClustering.h:78): nv labels in 0..K. */
+int acvd_get_labels(acvd_ctx* ctx, int32_t mesh_id, int32_t* labels);
+/* GetClusterSize (vtkUniformClustering.h:132) for all K clusters. */
+int acvd_get_cluster_sizes(acvd_ctx* ctx, int32_t mesh_id, int32_t* sizes);
+/* GetMetric()->GetClusterCentroid for all K clusters (3*K doubles):
+ * isotropic S/W (vtkIsotropicMetricForClustering.h:178-183); QEM the
+ * rank-truncated quadric point (vtkQEMetricForClustering.h:263-285,
+ * Common/vtkQuadricTools.cxx:168-177).  Empty clusters give NaN. */
+int acvd_get_cluster_points(acvd_ctx* ctx, int32_t mesh_id, double* xyz);
+/* Unshifted clustering energy sum_i w_i |x_i - p(c_i)|^2 of the current
+ * labels, accumulated in FP64 on the device (diagnostic). */
+int acvd_get_energy(acvd_ctx* ctx, int32_t mesh_id, double* energy);
+int acvd_get_stats(acvd_ctx* ctx, int32_t mesh_id, acvd_stats* st);
+
+/* ---- device inputs of the sweep (for replay and inspection) --------------- */
+/* Number of CSR adjacency entries of the mesh (valid after a build). */
+int acvd_get_adjacency_size(acvd_ctx* ctx, int32_t mesh_id, int64_t* n_adj);
+/* CSR with mesh-local vertex ids: row_ptr[nv+1], col[n_adj], rows sorted. */
+int acvd_get_adjacency(acvd_ctx* ctx, int32_t mesh_id, int32_t* row_ptr, int32_t* col);
+/* Items: xw[4*nv] = (x, y, z, w) in FP32; q[10*nv] quadrics (QEM only, may
+ * be NULL).  Build the device inputs without clustering with acvd_build(). */
+int acvd_get_items(acvd_ctx* ctx, int32_t mesh_id, float* xw, float* q);
+/* Fixed-point scales of the mesh: {weight, weighted position, quadric}. */
+int acvd_get_scales(acvd_ctx* ctx, int32_t mesh_id, double* scales3);
+/* Runs the device build (CSR, items, scales) without clustering. */
+int acvd_build(acvd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACVD_C_H */

@@ -1,0 +1,336 @@
+// TEST INFRASTRUCTURE — CPU oracle, not part of the product.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load liboracle.so, and only as the checker / the CPU baseline.
+//
+// extern "C" entry points of the oracles:
+//   A  serial restatement of the reference clustering (serial.h)
+//   B  threaded restatement of vtkThreadedClustering (threaded.h)
+//   C  replay of the GPU schedule (replay.h)
+//   E  FP64 unshifted energy of a labelling (energy below)
+// plus the reference's quadric tools and ring check for unit tests.
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "metrics.h"
+#include "quadric.h"
+#include "ref_mesh.h"
+#include "replay.h"
+#include "serial.h"
+#include "threaded.h"
+
+using namespace oracle;
+
+namespace {
+thread_local std::string g_err;
+
+struct OrcStats {
+  int64_t loops, convergences, total_moves, evaluated;
+  double seconds;
+  double shifted_energy;
+};
+
+template <class M>
+int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int level, int unconstrained,
+               const int32_t* init, int max_loops, int32_t* out, OrcStats* st) {
+  Clustering<M> cl(m);
+  if constexpr (M::kQEM) cl.metric.level = level;
+  cl.set_number_of_clusters(K);
+  cl.unconstrained_init = unconstrained;
+  cl.max_loops = max_loops;
+  if (init) {
+    cl.initial_sampling_type = 2;
+    cl.initial_labels = init;
+  }
+  cl.process(ind, gradation);
+  for (int64_t i = 0; i < m.nv; ++i) out[i] = cl.labels[i];
+  if (st) {
+    st->loops = cl.stats.loops;
+    st->convergences = cl.stats.convergences;
+    st->total_moves = cl.stats.total_modifications;
+    st->evaluated = cl.stats.evaluated_edges;
+    st->seconds = cl.stats.seconds;
+    st->shifted_energy = cl.stats.shifted_energy;
+  }
+  return 0;
+}
+
+template <class M>
+int run_threaded(const RefMesh& m, int K, double gradation, const double* ind, int level, int unconstrained,
+                 const int32_t* init, int max_loops, int nthreads, int pooling, int32_t* out, OrcStats* st) {
+  ThreadedClustering<M> cl(m);
+  if constexpr (M::kQEM) cl.metric.level = level;
+  cl.set_number_of_clusters(K);
+  cl.unconstrained_init = unconstrained;
+  cl.max_loops = max_loops;
+  cl.num_threads = nthreads;
+  cl.pooling_ratio = pooling;
+  if (init) {
+    cl.initial_sampling_type = 2;
+    cl.initial_labels = init;
+  }
+  cl.process(ind, gradation);
+  for (int64_t i = 0; i < m.nv; ++i) out[i] = cl.labels[i];
+  if (st) {
+    st->loops = cl.stats.loops;
+    st->convergences = cl.stats.convergences;
+    st->total_moves = cl.stats.total_modifications;
+    st->evaluated = cl.stats.evaluated_edges;
+    st->seconds = cl.stats.seconds;
+    st->shifted_energy = cl.stats.shifted_energy;
+  }
+  return 0;
+}
+
+// Unshifted FP64 energy sum_i w_i |x_i - p_c|^2 with p_c the cluster point of
+// the metric (centroid, or the rank-truncated quadric point at `level`),
+// cluster sums taken in item order.  out[0] = unshifted energy, out[1] =
+// shifted energy sum_c E_c (reference form), out[2] = sum_i w_i |x_i|^2.
+template <class M>
+void energy(const RefMesh& m, const double* ind, double gradation, int level, int K, const int32_t* labels,
+            double* out) {
+  M met;
+  if constexpr (M::kQEM) met.level = level;
+  met.build(m, ind, gradation);
+  std::vector<typename M::Cluster> cl(K);
+  for (auto& c : cl) M::reset(&c);
+  for (int64_t i = 0; i < m.nv; ++i)
+    if (labels[i] >= 0 && labels[i] < K) met.add_item(i, &cl[labels[i]]);
+  double shifted = 0.0;
+  std::vector<double> pts(3 * size_t(K));
+  for (int c = 0; c < K; ++c) {
+    met.compute_centroid(&cl[c]);
+    M::compute_energy(&cl[c]);
+    if (cl[c].W > 0) shifted += M::energy(&cl[c]);
+    M::centroid(&cl[c], &pts[3 * c]);
+  }
+  double e = 0.0, sx = 0.0;
+  for (int64_t i = 0; i < m.nv; ++i) {
+    const double w = met.item_weight(i);
+    const double* x = &m.P[3 * i];
+    sx += w * (x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const int32_t c = labels[i];
+    if (c < 0 || c >= K) continue;
+    const double* p = &pts[3 * c];
+    e += w * ((x[0] - p[0]) * (x[0] - p[0]) + (x[1] - p[1]) * (x[1] - p[1]) + (x[2] - p[2]) * (x[2] - p[2]));
+  }
+  out[0] = e;
+  out[1] = shifted;
+  out[2] = sx;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+void* orc_mesh_create(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri) {
+  try {
+    RefMesh* m = new RefMesh();
+    m->build(nv, xyz, nf, tri);
+    return m;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void orc_mesh_destroy(void* m) { delete static_cast<RefMesh*>(m); }
+
+int64_t orc_mesh_num_edges(void* mh) { return static_cast<RefMesh*>(mh)->num_edges(); }
+
+void orc_mesh_edges(void* mh, int32_t* ev1, int32_t* ev2) {
+  RefMesh* m = static_cast<RefMesh*>(mh);
+  std::memcpy(ev1, m->ev1.data(), m->ev1.size() * sizeof(int32_t));
+  std::memcpy(ev2, m->ev2.data(), m->ev2.size() * sizeof(int32_t));
+}
+
+void orc_mesh_vertex_areas(void* mh, double* out) {
+  std::vector<double> a;
+  static_cast<RefMesh*>(mh)->vertex_areas(a);
+  std::memcpy(out, a.data(), a.size() * sizeof(double));
+}
+
+// Items of the metric (BuildMetric): weights w[nv]; for QEM also Q[9*nv].
+int orc_build_items(void* mh, int metric, const double* ind, double gradation, double* w, double* q9) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  if (metric == 0) {
+    IsoMetric met;
+    met.build(m, ind, gradation);
+    for (int64_t i = 0; i < m.nv; ++i) w[i] = met.items[i].W;
+  } else {
+    QEMMetric met;
+    met.build(m, ind, gradation);
+    for (int64_t i = 0; i < m.nv; ++i) {
+      w[i] = met.items[i].W;
+      if (q9)
+        for (int k = 0; k < 9; ++k) q9[9 * i + k] = met.items[i].Q[k];
+    }
+  }
+  return 0;
+}
+
+int orc_serial_run(void* mh, int metric, int K, double gradation, const double* ind, int level, int unconstrained,
+                   const int32_t* init, int max_loops, int32_t* out, OrcStats* st) {
+  try {
+    const RefMesh& m = *static_cast<RefMesh*>(mh);
+    if (metric == 0) return run_serial<IsoMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, out, st);
+    return run_serial<QEMMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, out, st);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int orc_threaded_run(void* mh, int metric, int K, double gradation, const double* ind, int level, int unconstrained,
+                     const int32_t* init, int max_loops, int nthreads, int pooling, int32_t* out, OrcStats* st) {
+  try {
+    const RefMesh& m = *static_cast<RefMesh*>(mh);
+    if (metric == 0)
+      return run_threaded<IsoMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, nthreads, pooling,
+                                     out, st);
+    return run_threaded<QEMMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, nthreads, pooling,
+                                   out, st);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Reference initial sampling (type 1, ComputeInitialRandomSampling).
+int orc_initial_sampling(void* mh, int metric, int K, double gradation, const double* ind, int32_t* out) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  std::vector<int32_t> lab;
+  if (metric == 0) {
+    Clustering<IsoMetric> cl(m);
+    cl.set_number_of_clusters(K);
+    cl.init(ind, gradation);
+    cl.initial_random_sampling(lab, K);
+  } else {
+    Clustering<QEMMetric> cl(m);
+    cl.set_number_of_clusters(K);
+    cl.init(ind, gradation);
+    cl.initial_random_sampling(lab, K);
+  }
+  std::memcpy(out, lab.data(), lab.size() * sizeof(int32_t));
+  return 0;
+}
+
+// Reference FillHolesInClustering (connexity off) on a labelling (in place).
+int orc_fill_holes(void* mh, int K, int32_t* labels) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  Clustering<IsoMetric> cl(m);
+  cl.set_number_of_clusters(K);
+  cl.allocate();
+  std::vector<int32_t> lab(labels, labels + m.nv);
+  cl.labels = lab;
+  cl.fill_holes();
+  std::memcpy(labels, cl.labels.data(), m.nv * sizeof(int32_t));
+  return 0;
+}
+
+// Reference CleanClustering on a labelling (in place); returns the count.
+int orc_clean(void* mh, int K, int32_t* labels) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  Clustering<IsoMetric> cl(m);
+  cl.set_number_of_clusters(K);
+  cl.allocate();
+  cl.labels.assign(labels, labels + m.nv);
+  int n = cl.clean_clustering();
+  std::memcpy(labels, cl.labels.data(), m.nv * sizeof(int32_t));
+  return n;
+}
+
+int orc_energy(void* mh, int metric, const double* ind, double gradation, int level, int K, const int32_t* labels,
+               double* out3) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  if (metric == 0) energy<IsoMetric>(m, ind, gradation, level, K, labels, out3);
+  else energy<QEMMetric>(m, ind, gradation, level, K, labels, out3);
+  return 0;
+}
+
+// vtkQuadricTools::ComputeRepresentativePoint on (Q[9], P[3]) -> P, rank def.
+int orc_rep_point(const double* Q9, double* P, int level, double thr) {
+  return compute_representative_point(Q9, P, level, thr);
+}
+
+void orc_sym_eigen(const double* A9, double* lam3, double* V9) {
+  double A[3][3], V[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[i][j] = A9[3 * i + j];
+  sym_eigen3(A, lam3, V);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V9[3 * i + j] = V[i][j];
+}
+
+void orc_triangle_quadric(const double* x1, const double* x2, const double* x3, double* Q10) {
+  for (int k = 0; k < 10; ++k) Q10[k] = 0.0;
+  add_triangle_quadric(Q10, x1, x2, x3, true);
+}
+
+// Reference ring check (ConnexityConstraintProblemLocal) on the oracle mesh.
+int orc_ring_problem(void* mh, const int32_t* labels, int64_t item, int32_t cluster) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  Clustering<IsoMetric> cl(m);
+  cl.set_number_of_clusters(1);
+  cl.labels.assign(labels, labels + m.nv);
+  cl.connexity = 1;
+  return cl.connexity_problem(item, cluster);
+}
+
+// Shared-header functions evaluated on the host (compared with the oracle).
+int orc_hdr_rep_point_delta(const double* Q9, const double* c, int level, double* D) {
+  return acvd::rep_point_delta(Q9, c, level, D);
+}
+
+int orc_hdr_ring_problem(const int32_t* rp, const int32_t* col, const int32_t* labels, int32_t item,
+                         int32_t cluster) {
+  return acvd::ring_problem(rp, col, labels, item, cluster) ? 1 : 0;
+}
+
+uint64_t orc_hdr_accept_key(uint32_t seed, int32_t loop, uint32_t item) { return acvd::accept_key(seed, loop, item); }
+
+// Replay C.  params: K, metric, level, seed, unconstrained, max_loops,
+// max_convergences.  scales3 = {sw, ss, sq}.  init/out in the reference
+// convention (K = null).
+int orc_replay_run(int64_t n, const int32_t* rp, const int32_t* col, const float* xw, const float* q,
+                   const double* scales3, int K, int metric, int level, uint32_t seed, int unconstrained,
+                   int max_loops, int max_conv, const int32_t* init, int32_t* out, int64_t* stats6) {
+  Replay r;
+  r.N = n;
+  r.K = K;
+  r.metric = metric;
+  r.level = level;
+  r.seed = seed;
+  r.unconstrained_init = unconstrained;
+  r.max_loops = max_loops;
+  r.max_conv = max_conv;
+  r.rp = rp;
+  r.col = col;
+  r.xw = xw;
+  r.q = q;
+  r.sc.sw = scales3[0];
+  r.sc.ss = scales3[1];
+  r.sc.sq = scales3[2];
+  r.sc.isw = 1.0 / r.sc.sw;
+  r.sc.iss = 1.0 / r.sc.ss;
+  r.sc.isq = 1.0 / r.sc.sq;
+  r.run(init);
+  for (int64_t i = 0; i < n; ++i) out[i] = r.labels[i] == acvd::kNull ? K : r.labels[i];
+  if (stats6) {
+    stats6[0] = r.stats.loops;
+    stats6[1] = r.stats.convergences;
+    stats6[2] = r.stats.total_moves;
+    stats6[3] = r.stats.evaluated;
+    stats6[4] = r.stats.proposed;
+    stats6[5] = r.stats.last_disconnected;
+  }
+  return 0;
+}
+
+}  // extern "C"

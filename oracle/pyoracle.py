@@ -1,0 +1,219 @@
+"""TEST INFRASTRUCTURE — ctypes wrapper of the CPU oracles (oracle/_build/liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module, and only as the checker / CPU baseline.
+
+Oracles (all restate the reference in C++, see the headers for file:line):
+  serial_run    A: vtkUniformClustering + vtkVerticesProcessing + metric (FP64)
+  threaded_run  B: vtkThreadedClustering (ACVDP / ACVDQP), the timed CPU baseline
+  replay_run    C: the GPU pass schedule replayed on the host (bit-exact target)
+  energy        FP64 unshifted energy of a labelling
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+ISO, QEM = 0, 1
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [("loops", ctypes.c_int64), ("convergences", ctypes.c_int64),
+                ("total_moves", ctypes.c_int64), ("evaluated", ctypes.c_int64),
+                ("seconds", ctypes.c_double), ("shifted_energy", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        L.orc_mesh_create.restype = P
+        L.orc_mesh_create.argtypes = [ctypes.c_int64, P, ctypes.c_int64, P]
+        L.orc_mesh_destroy.argtypes = [P]
+        L.orc_mesh_num_edges.restype = ctypes.c_int64
+        L.orc_mesh_num_edges.argtypes = [P]
+        L.orc_mesh_edges.argtypes = [P, P, P]
+        L.orc_mesh_vertex_areas.argtypes = [P, P]
+        L.orc_build_items.argtypes = [P, ctypes.c_int, P, ctypes.c_double, P, P]
+        L.orc_serial_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int, ctypes.c_int,
+                                     P, ctypes.c_int, P, P]
+        L.orc_threaded_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int,
+                                       ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
+        L.orc_initial_sampling.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P]
+        L.orc_fill_holes.argtypes = [P, ctypes.c_int, P]
+        L.orc_clean.argtypes = [P, ctypes.c_int, P]
+        L.orc_energy.argtypes = [P, ctypes.c_int, P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P]
+        L.orc_rep_point.argtypes = [P, P, ctypes.c_int, ctypes.c_double]
+        L.orc_sym_eigen.argtypes = [P, P, P]
+        L.orc_triangle_quadric.argtypes = [P, P, P, P]
+        L.orc_ring_problem.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
+        L.orc_hdr_rep_point_delta.argtypes = [P, P, ctypes.c_int, P]
+        L.orc_hdr_ring_problem.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32]
+        L.orc_hdr_accept_key.restype = ctypes.c_uint64
+        L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32]
+        L.orc_replay_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
+        L.orc_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class RefMesh:
+    """Reference-ordered mesh (edge table in face order)."""
+
+    def __init__(self, xyz, tri):
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self.tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+        self.nv = len(self.xyz)
+        self.h = lib().orc_mesh_create(self.nv, _p(self.xyz), len(self.tri), _p(self.tri))
+        if not self.h:
+            raise ValueError(lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_destroy(self.h)
+            self.h = None
+
+    def edges(self):
+        ne = lib().orc_mesh_num_edges(self.h)
+        a = np.empty(ne, np.int32)
+        b = np.empty(ne, np.int32)
+        lib().orc_mesh_edges(self.h, _p(a), _p(b))
+        return a, b
+
+    def vertex_areas(self):
+        out = np.empty(self.nv, np.float64)
+        lib().orc_mesh_vertex_areas(self.h, _p(out))
+        return out
+
+    def items(self, metric, indicator=None, gradation=0.0):
+        w = np.empty(self.nv, np.float64)
+        q = np.empty((self.nv, 9), np.float64) if metric == QEM else None
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        lib().orc_build_items(self.h, metric, _p(ind), gradation, _p(w), _p(q))
+        return w, q
+
+    def serial_run(self, metric, K, gradation=0.0, indicator=None, level=3, unconstrained=0, init=None,
+                   max_loops=5000000):
+        out = np.empty(self.nv, np.int32)
+        st = OrcStats()
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        ini = None if init is None else np.ascontiguousarray(init, np.int32)
+        rc = lib().orc_serial_run(self.h, metric, K, gradation, _p(ind), level, unconstrained, _p(ini), max_loops,
+                                  _p(out), ctypes.byref(st))
+        if rc:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out, st.as_dict()
+
+    def threaded_run(self, metric, K, gradation=0.0, indicator=None, level=3, unconstrained=0, init=None,
+                     max_loops=5000000, threads=1, pooling=5):
+        out = np.empty(self.nv, np.int32)
+        st = OrcStats()
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        ini = None if init is None else np.ascontiguousarray(init, np.int32)
+        rc = lib().orc_threaded_run(self.h, metric, K, gradation, _p(ind), level, unconstrained, _p(ini), max_loops,
+                                    threads, pooling, _p(out), ctypes.byref(st))
+        if rc:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return out, st.as_dict()
+
+    def initial_sampling(self, metric, K, gradation=0.0, indicator=None):
+        out = np.empty(self.nv, np.int32)
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        lib().orc_initial_sampling(self.h, metric, K, gradation, _p(ind), _p(out))
+        return out
+
+    def fill_holes(self, K, labels):
+        lab = np.array(labels, np.int32, copy=True)
+        lib().orc_fill_holes(self.h, K, _p(lab))
+        return lab
+
+    def clean(self, K, labels):
+        lab = np.array(labels, np.int32, copy=True)
+        n = lib().orc_clean(self.h, K, _p(lab))
+        return lab, n
+
+    def energy(self, metric, K, labels, gradation=0.0, indicator=None, level=3):
+        """(unshifted, shifted, sum w|x|^2) in FP64."""
+        out = np.empty(3, np.float64)
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        lab = np.ascontiguousarray(labels, np.int32)
+        lib().orc_energy(self.h, metric, _p(ind), gradation, level, K, _p(lab), _p(out))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def ring_problem(self, labels, item, cluster):
+        lab = np.ascontiguousarray(labels, np.int32)
+        return lib().orc_ring_problem(self.h, _p(lab), int(item), int(cluster))
+
+
+def rep_point(Q9, P, level=3, thr=1e-3):
+    Q = np.ascontiguousarray(Q9, np.float64)
+    p = np.array(P, np.float64, copy=True)
+    rd = lib().orc_rep_point(_p(Q), _p(p), level, thr)
+    return p, rd
+
+
+def sym_eigen(A):
+    A = np.ascontiguousarray(A, np.float64).reshape(9)
+    lam = np.empty(3)
+    V = np.empty(9)
+    lib().orc_sym_eigen(_p(A), _p(lam), _p(V))
+    return lam, V.reshape(3, 3)
+
+
+def triangle_quadric(x1, x2, x3):
+    out = np.empty(10)
+    a, b, c = (np.ascontiguousarray(x, np.float64) for x in (x1, x2, x3))
+    lib().orc_triangle_quadric(_p(a), _p(b), _p(c), _p(out))
+    return out
+
+
+def hdr_rep_point_delta(Q9, c, level=3):
+    Q = np.ascontiguousarray(Q9, np.float64)
+    cc = np.ascontiguousarray(c, np.float64)
+    D = np.empty(3)
+    rd = lib().orc_hdr_rep_point_delta(_p(Q), _p(cc), level, _p(D))
+    return D, rd
+
+
+def hdr_ring_problem(row_ptr, col, labels, item, cluster):
+    return lib().orc_hdr_ring_problem(_p(np.ascontiguousarray(row_ptr, np.int32)),
+                                      _p(np.ascontiguousarray(col, np.int32)),
+                                      _p(np.ascontiguousarray(labels, np.int32)), int(item), int(cluster))
+
+
+def hdr_accept_key(seed, loop, item):
+    return lib().orc_hdr_accept_key(seed, loop, item)
+
+
+def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, unconstrained=0, max_loops=5000000,
+               max_conv=1000000000):
+    n = len(row_ptr) - 1
+    out = np.empty(n, np.int32)
+    st = np.zeros(6, np.int64)
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    cl = np.ascontiguousarray(col, np.int32)
+    x = np.ascontiguousarray(xw, np.float32)
+    qq = None if q is None else np.ascontiguousarray(q, np.float32)
+    sc = np.ascontiguousarray(scales, np.float64)
+    ini = np.ascontiguousarray(init, np.int32)
+    lib().orc_replay_run(n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed, unconstrained, max_loops,
+                         max_conv, _p(ini), _p(out), _p(st))
+    keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
+    return out, dict(zip(keys, (int(v) for v in st)))

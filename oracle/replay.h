@@ -1,0 +1,262 @@
+// TEST INFRASTRUCTURE — CPU oracle, not part of the product.
+//
+// Replay C: the GPU's pass schedule executed sequentially on the host for one
+// mesh.  It reads the same inputs the device sweeps (CSR adjacency, FP32
+// items, fixed-point scales) and calls the same per-item rule and cluster
+// arithmetic (acvd_amd/csrc/acvd_math.h, compiled here with
+// -ffp-contract=off), so its final labels must equal the device's bit for bit.
+// What it restates independently of the device code: the pass structure
+// (propose over a frozen state, min-key accept, exclusive apply), the
+// synchronous fill rounds, the largest-component clean and the host-side
+// phase decisions between convergences.
+#pragma once
+
+#include <cstdint>
+#include <deque>
+#include <vector>
+
+#include "../acvd_amd/csrc/acvd_math.h"
+
+namespace oracle {
+
+struct ReplayStats {
+  int64_t loops = 0, convergences = 0, total_moves = 0, evaluated = 0, proposed = 0;
+  int32_t last_disconnected = 0;
+};
+
+struct Replay {
+  int64_t N = 0;
+  int32_t K = 0;
+  int metric = 0, level = 3;
+  uint32_t seed = 0;
+  acvd::Scales sc{};
+  int max_loops = 5000000, max_conv = 1000000000, unconstrained_init = 0;
+  const int32_t* rp = nullptr;
+  const int32_t* col = nullptr;
+  const float* xw = nullptr;
+  const float* q = nullptr;
+
+  std::vector<int32_t> labels, sizes, stamp, cand_pass, pending;
+  std::vector<unsigned long long> acc_ws, acc_q;
+  std::vector<acvd::ClusterGeom> geom;
+  std::vector<double> T;
+  std::vector<unsigned long long> prio;
+  acvd::MeshState st{};
+  ReplayStats stats;
+
+  acvd::SweepView view() const {
+    acvd::SweepView v;
+    v.row_ptr = rp;
+    v.col = col;
+    v.labels = labels.data();
+    v.xw = xw;
+    v.q = q;
+    v.acc_ws = acc_ws.data();
+    v.acc_q = acc_q.data();
+    v.geom = geom.data();
+    v.T = T.data();
+    v.sizes = sizes.data();
+    v.stamp = stamp.data();
+    v.cand_pass = cand_pass.data();
+    return v;
+  }
+
+  void fill_until_stable() {
+    for (;;) {
+      bool changed = false;
+      for (int64_t i = 0; i < N; ++i) {
+        if (labels[i] != acvd::kNull) continue;
+        int32_t pick = acvd::kNull;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+          if (labels[col[k]] != acvd::kNull) {
+            pick = labels[col[k]];
+            break;
+          }
+        pending[i] = pick;
+        changed |= pick != acvd::kNull;
+      }
+      for (int64_t i = 0; i < N; ++i)
+        if (labels[i] == acvd::kNull) labels[i] = pending[i];
+      if (!changed) break;
+    }
+  }
+
+  void derive(int32_t c) {
+    const int64_t* ws = reinterpret_cast<const int64_t*>(&acc_ws[4 * int64_t(c)]);
+    geom[c] = acvd::geom_from_fixed(ws[0], ws[1], ws[2], ws[3], sc);
+    if (metric == 1) {
+      const int64_t* q10 = reinterpret_cast<const int64_t*>(&acc_q[10 * int64_t(c)]);
+      T[c] = st.constrained ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr) : 0.0;
+    }
+  }
+
+  int64_t recompute(bool set_stamp) {
+    std::fill(acc_ws.begin(), acc_ws.end(), 0ull);
+    std::fill(acc_q.begin(), acc_q.end(), 0ull);
+    std::fill(sizes.begin(), sizes.end(), 0);
+    int64_t count = 0;
+    for (int64_t i = 0; i < N; ++i) {
+      const int32_t c = labels[i];
+      if (c == acvd::kNull) continue;
+      const float w = xw[4 * i + 3];
+      acc_ws[4 * int64_t(c)] += (unsigned long long)acvd::fix_w(w, sc);
+      for (int k = 0; k < 3; ++k)
+        acc_ws[4 * int64_t(c) + 1 + k] += (unsigned long long)acvd::fix_s(w, xw[4 * i + k], sc);
+      if (metric == 1)
+        for (int k = 0; k < 10; ++k) acc_q[10 * int64_t(c) + k] += (unsigned long long)acvd::fix_q(q[10 * i + k], sc);
+      sizes[c]++;
+      count++;
+    }
+    for (int32_t c = 0; c < K; ++c) {
+      derive(c);
+      if (set_stamp) stamp[c] = st.loops;
+    }
+    return count;
+  }
+
+  void update(int32_t c, int sign, int64_t iw, const int64_t* is, const int64_t* iq) {
+    acc_ws[4 * int64_t(c)] += (unsigned long long)(sign * iw);
+    for (int k = 0; k < 3; ++k) acc_ws[4 * int64_t(c) + 1 + k] += (unsigned long long)(sign * is[k]);
+    if (metric == 1)
+      for (int k = 0; k < 10; ++k) acc_q[10 * int64_t(c) + k] += (unsigned long long)(sign * iq[k]);
+    derive(c);
+  }
+
+  void pass() {
+    const int32_t L = st.loops;
+    const acvd::SweepView v = view();
+    std::vector<std::pair<int64_t, int32_t>> cands;
+    for (int64_t i = 0; i < N; ++i) {
+      const int32_t lab = labels[i];
+      if (!acvd::needs_eval(v, i, lab, L)) continue;
+      stats.evaluated++;
+      int32_t target = acvd::kNull;
+      bool ok = metric == 0 ? acvd::propose_item<0>(v, i, lab, st.connexity, st.constrained, level, sc, &target)
+                            : acvd::propose_item<1>(v, i, lab, st.connexity, st.constrained, level, sc, &target);
+      if (!ok) continue;
+      cands.emplace_back(i, target);
+    }
+    // accept keys (min at both clusters) and proposal stamps
+    for (auto& c : cands) {
+      const int64_t i = c.first;
+      const uint64_t key = acvd::accept_key(seed, L, uint32_t(i));
+      const int32_t lab = labels[i];
+      if (lab != acvd::kNull && key < prio[lab]) prio[lab] = key;
+      if (key < prio[c.second]) prio[c.second] = key;
+      cand_pass[i] = L;
+    }
+    stats.proposed += int64_t(cands.size());
+    int mods = 0;
+    for (auto& c : cands) {
+      const int64_t i = c.first;
+      const int32_t b = c.second;
+      const uint64_t key = acvd::accept_key(seed, L, uint32_t(i));
+      const int32_t lab = labels[i];
+      if (lab != acvd::kNull && prio[lab] != key) continue;
+      if (prio[b] != key) continue;
+      labels[i] = b;
+      const float w = xw[4 * i + 3];
+      const int64_t iw = acvd::fix_w(w, sc);
+      const int64_t is[3] = {acvd::fix_s(w, xw[4 * i], sc), acvd::fix_s(w, xw[4 * i + 1], sc),
+                             acvd::fix_s(w, xw[4 * i + 2], sc)};
+      int64_t iq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      if (metric == 1)
+        for (int k = 0; k < 10; ++k) iq[k] = acvd::fix_q(q[10 * i + k], sc);
+      if (lab != acvd::kNull) {
+        update(lab, -1, iw, is, iq);
+        sizes[lab]--;
+        stamp[lab] = L;
+      }
+      update(b, +1, iw, is, iq);
+      sizes[b]++;
+      stamp[b] = L;
+      mods++;
+    }
+    st.pass_mods = mods;
+  }
+
+  // Largest connected component per cluster (ties: smallest first item);
+  // other components become null.  Returns the number of clusters that had
+  // more than one component.
+  int clean() {
+    std::vector<int32_t> comp(N, -1);
+    std::vector<int64_t> comp_size;
+    std::vector<int64_t> comp_root;
+    std::deque<int64_t> q2;
+    for (int64_t i = 0; i < N; ++i) {
+      if (comp[i] >= 0 || labels[i] == acvd::kNull) continue;
+      const int32_t id = int32_t(comp_size.size());
+      const int32_t lab = labels[i];
+      int64_t size = 0;
+      comp[i] = id;
+      q2.clear();
+      q2.push_back(i);
+      while (!q2.empty()) {
+        int64_t u = q2.front();
+        q2.pop_front();
+        size++;
+        for (int32_t k = rp[u]; k < rp[u + 1]; ++k) {
+          int32_t v = col[k];
+          if (comp[v] < 0 && labels[v] == lab) {
+            comp[v] = id;
+            q2.push_back(v);
+          }
+        }
+      }
+      comp_size.push_back(size);
+      comp_root.push_back(i);
+    }
+    std::vector<int32_t> best(K, -1), ncomp(K, 0);
+    for (size_t c = 0; c < comp_size.size(); ++c) {
+      const int32_t lab = labels[comp_root[c]];
+      ncomp[lab]++;
+      const int32_t b = best[lab];
+      if (b < 0 || comp_size[c] > comp_size[b]) best[lab] = int32_t(c);  // roots increase: first max kept
+    }
+    for (int64_t i = 0; i < N; ++i) {
+      const int32_t lab = labels[i];
+      if (lab == acvd::kNull) continue;
+      if (comp[i] != best[lab]) labels[i] = acvd::kNull;
+    }
+    int nd = 0;
+    for (int32_t c = 0; c < K; ++c) nd += ncomp[c] > 1;
+    return nd;
+  }
+
+  // init: labels in the reference convention (0..K-1, K = null)
+  void run(const int32_t* init) {
+    labels.assign(N, acvd::kNull);
+    for (int64_t i = 0; i < N; ++i) labels[i] = (init[i] < 0 || init[i] >= K) ? acvd::kNull : init[i];
+    sizes.assign(K, 0);
+    stamp.assign(K, 0);
+    cand_pass.assign(N, -2);
+    pending.assign(N, acvd::kNull);
+    acc_ws.assign(4 * size_t(K), 0ull);
+    acc_q.assign(metric == 1 ? 10 * size_t(K) : 0, 0ull);
+    geom.assign(K, acvd::ClusterGeom{});
+    T.assign(metric == 1 ? K : 0, 0.0);
+    prio.assign(K, ~0ull);
+    fill_until_stable();
+    st = acvd::MeshState{};
+    st.phase = acvd::kPhaseSweep;
+    st.max_loops = std::min<int32_t>(max_loops, acvd::kMaxLoops - 2);
+    st.max_convergences = max_conv;
+    st.unconstrained_init = unconstrained_init;
+    st.constrained = unconstrained_init ? 0 : 1;
+    st.early_stop_items = recompute(true);
+    while (st.phase == acvd::kPhaseSweep) {
+      pass();
+      if (!acvd::controller_step(st)) continue;
+      int nd = clean();
+      fill_until_stable();
+      st.phase = acvd::finish_after_clean(st, nd) ? acvd::kPhaseDone : acvd::kPhaseSweep;
+      recompute(true);
+    }
+    stats.loops = st.loops;
+    stats.convergences = st.convergences;
+    stats.total_moves = st.total_mods;
+    stats.last_disconnected = st.last_disconnected;
+  }
+};
+
+}  // namespace oracle
