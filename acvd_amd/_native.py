@@ -49,10 +49,15 @@ class Stats(ctypes.Structure):
         ("build_ms", ctypes.c_double),
         ("n_items", ctypes.c_int64),
         ("n_adjacency", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double * 4),
+        ("kernel_launches", ctypes.c_int64 * 4),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["kernel_ms"] = list(self.kernel_ms)
+        d["kernel_launches"] = list(self.kernel_launches)
+        return d
 
 
 # every symbol declared in include/acvd_c.h
@@ -62,7 +67,7 @@ EXPORTS = [
     "acvd_set_params", "acvd_get_params", "acvd_set_indicator", "acvd_set_initial_labels",
     "acvd_process_clustering", "acvd_get_labels", "acvd_get_cluster_sizes", "acvd_get_cluster_points",
     "acvd_get_energy", "acvd_get_stats", "acvd_get_adjacency_size", "acvd_get_adjacency", "acvd_get_items",
-    "acvd_get_scales", "acvd_build",
+    "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing",
 ]
 
 _lib = None
@@ -97,6 +102,8 @@ def lib():
     L.acvd_set_initial_labels.argtypes = [P, i32, P]
     L.acvd_process_clustering.argtypes = [P]
     L.acvd_build.argtypes = [P]
+    L.acvd_initial_sampling.argtypes = [P]
+    L.acvd_set_timing.argtypes = [P, ctypes.c_int]
     for f in ("acvd_get_labels", "acvd_get_cluster_sizes", "acvd_get_cluster_points", "acvd_get_energy",
               "acvd_get_scales"):
         getattr(L, f).argtypes = [P, i32, P]

@@ -16,6 +16,9 @@
 namespace acvd {
 
 constexpr int kBlock = 256;
+// Work-list slices: blocks append to counter blockIdx % kSlices, which keeps
+// same-address atomics (slow at device scope) to a few per slice per pass.
+constexpr int kSlices = 64;
 
 struct DevMesh {
   int64_t item_off;
@@ -34,10 +37,13 @@ struct DevMesh {
 
 // Small control block (device), read back by the host between pass chunks.
 struct Ctrl {
-  int32_t cand_count;
   int32_t any_sweep;
   int32_t any_need_clean;
   int32_t fill_changed;
+  int32_t pad;
+  int32_t work_count[kSlices];  // items to evaluate in this pass, per slice
+  int32_t props[kSlices];       // proposals of this pass (single-mesh contexts)
+  int32_t mods[kSlices];        // applied moves of this pass (single-mesh contexts)
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
@@ -67,7 +73,10 @@ struct SweepArgs {
   int32_t* stamp;              // mesh-local loop of the last modification
   unsigned long long* prio;    // accept keys per cluster
   int32_t* cand_pass;          // per item: loop of its last proposal
-  int2* cand;                  // (item, target cluster) of this pass
+  int32_t* cand_target;        // per item: target cluster of its proposal
+  int32_t* work;               // items to evaluate in this pass (kSlices slices)
+  int32_t slice_cap;           // capacity of one work-list slice
+  int32_t pad2;
   Ctrl* ctrl;
 };
 
@@ -94,7 +103,8 @@ hipError_t launch_zero_clusters(hipStream_t s, const SweepArgs& a, const int32_t
 hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask,
                              unsigned long long* mesh_count);
 hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int set_stamp);
-hipError_t launch_propose(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem);
+hipError_t launch_scan(hipStream_t s, const SweepArgs& a);
+hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid);
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid);
 hipError_t launch_controller(hipStream_t s, const SweepArgs& a);
 hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* pending);

@@ -97,7 +97,9 @@ struct acvd_ctx {
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T;
-  DBuf<int2> d_cand;
+  DBuf<int32_t> d_cand_target;
+  DBuf<int32_t> d_work;
+  int32_t slice_cap = 0;
   DBuf<int32_t> d_mask_all, d_mask, d_ndisc;
   DBuf<unsigned long long> d_mesh_count;
   // clean scratch
@@ -113,6 +115,10 @@ struct acvd_ctx {
   hipGraph_t chunk_graph = nullptr;
   double build_ms = 0.0, converge_ms = 0.0;
   Ctrl last_ctrl{};
+  bool timing = false;
+  bool labels_seeded = false;  // device labels hold the initial sampling
+  double kernel_ms[4] = {0, 0, 0, 0};
+  int64_t kernel_launches[4] = {0, 0, 0, 0};
 
   ~acvd_ctx() {
     if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
@@ -145,7 +151,9 @@ struct acvd_ctx {
     a.stamp = d_stamp.p;
     a.prio = d_prio.p;
     a.cand_pass = d_cand_pass.p;
-    a.cand = d_cand.p;
+    a.cand_target = d_cand_target.p;
+    a.work = d_work.p;
+    a.slice_cap = slice_cap;
     a.ctrl = d_ctrl.p;
     return a;
   }
@@ -362,7 +370,13 @@ int build(acvd_ctx* c) {
   }
   HIP_TRY(c->d_prio.alloc(K));
   HIP_TRY(c->d_geom.alloc(K));
-  HIP_TRY(c->d_cand.alloc(N));
+  HIP_TRY(c->d_cand_target.alloc(N));
+  {
+    // blocks of the scan grid are dealt round-robin to the slices
+    const int64_t blocks = (N + kBlock - 1) / kBlock;
+    c->slice_cap = int32_t(((blocks + kSlices - 1) / kSlices) * kBlock);
+    HIP_TRY(c->d_work.alloc(size_t(c->slice_cap) * kSlices));
+  }
   HIP_TRY(c->d_parent.alloc(N));
   HIP_TRY(c->d_comp_size.alloc(N));
   HIP_TRY(c->d_ncomp.alloc(K));
@@ -392,6 +406,10 @@ int build(acvd_ctx* c) {
 }
 
 // ------------------------------------------------------------ minimisation
+int sweep_grid(const acvd_ctx* c) {
+  return int(std::max<int64_t>(1, std::min<int64_t>((c->N + kBlock - 1) / kBlock, 2048)));
+}
+
 int fill_until_stable(acvd_ctx* c, const int32_t* mask, int max_rounds) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
@@ -418,17 +436,49 @@ int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsign
 int enqueue_chunk(acvd_ctx* c) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
-  const int apply_grid = int(std::max<int64_t>(1, std::min<int64_t>((c->N + kBlock - 1) / kBlock, 2048)));
+  const int grid = sweep_grid(c);
   for (int p = 0; p < kChunk; ++p) {
-    HIP_TRY(launch_propose(s, a, c->any_iso, c->any_qem));
-    HIP_TRY(launch_apply(s, a, apply_grid));
+    HIP_TRY(launch_scan(s, a));
+    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
+    HIP_TRY(launch_apply(s, a, grid));
     HIP_TRY(launch_controller(s, a));
   }
   return ACVD_OK;
 }
 
+// Passes launched one by one with events around each kernel.
+int run_chunk_timed(acvd_ctx* c) {
+  hipStream_t s = c->stream;
+  SweepArgs a = c->args();
+  const int grid = sweep_grid(c);
+  std::vector<hipEvent_t> ev(5 * kChunk);
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  for (int p = 0; p < kChunk; ++p) {
+    HIP_TRY(hipEventRecord(ev[5 * p], s));
+    HIP_TRY(launch_scan(s, a));
+    HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
+    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
+    HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
+    HIP_TRY(launch_apply(s, a, grid));
+    HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
+    HIP_TRY(launch_controller(s, a));
+    HIP_TRY(hipEventRecord(ev[5 * p + 4], s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int p = 0; p < kChunk; ++p)
+    for (int k = 0; k < 4; ++k) {
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[5 * p + k], ev[5 * p + k + 1]));
+      c->kernel_ms[k] += ms;
+      c->kernel_launches[k] += (k == 1 && c->any_iso && c->any_qem) ? 2 : 1;
+    }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return ACVD_OK;
+}
+
 int run_chunk(acvd_ctx* c, bool use_graph) {
   hipStream_t s = c->stream;
+  if (c->timing) return run_chunk_timed(c);
   if (use_graph) {
     if (!c->chunk_exec) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -478,6 +528,10 @@ int minimize_energy(acvd_ctx* c) {
   HIP_TRY(launch_fill_i32(s, c->d_cand_pass.p, c->N, -2));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
+  for (int k = 0; k < 4; ++k) {
+    c->kernel_ms[k] = 0.0;
+    c->kernel_launches[k] = 0;
+  }
   std::vector<int32_t> mask(M), ndisc(M);
   int64_t guard = 0;
   while (true) {
@@ -722,11 +776,32 @@ int acvd_process_clustering(acvd_ctx* ctx) {
     S.build_ms = ctx->build_ms;
     S.n_items = h.nv;
     S.n_adjacency = h.n_adj;
+    for (int k = 0; k < 4; ++k) {
+      S.kernel_ms[k] = ctx->kernel_ms[k];
+      S.kernel_launches[k] = ctx->kernel_launches[k];
+    }
     if (h.params.console_output) {
       std::printf("The clustering took :%g seconds.\n", ms / 1000.0);
       std::printf("Number of loops:\t%lld\n", (long long)st.loops);
     }
   }
+  return ACVD_OK;
+}
+
+int acvd_initial_sampling(acvd_ctx* ctx) {
+  if (int rc = check_ctx(ctx)) return rc;
+  int rc = build(ctx);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  rc = initial_labels(ctx);
+  if (rc) return rc;
+  // the first FillHolesInClustering of MinimizeEnergy (vtkUniformClustering.h:727)
+  return fill_until_stable(ctx, ctx->d_mask_all.p, 1 << 20);
+}
+
+int acvd_set_timing(acvd_ctx* ctx, int enable) {
+  if (int rc = check_ctx(ctx)) return rc;
+  ctx->timing = enable != 0;
   return ACVD_OK;
 }
 

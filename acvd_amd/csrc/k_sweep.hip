@@ -32,26 +32,6 @@ __device__ inline int mesh_of_cluster(const SweepArgs& a, int64_t c) { return a.
 
 __device__ inline int64_t ld_i64(const unsigned long long* p) { return int64_t(*p); }
 
-// wave-aggregated atomic increment of a 32-bit counter; returns this lane's slot
-__device__ inline int32_t wave_append(int32_t* counter, bool want) {
-  const unsigned long long mask = __ballot(want);
-  if (!want) return -1;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  int32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-  base = __shfl(base, leader);
-  return base + rank;
-}
-
-__device__ inline void block_count_add(unsigned long long* dst, unsigned v) {
-  // wave reduction then one atomic per wave
-  unsigned x = v;
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  if (__lane_id() == 0 && x) atomicAdd(dst, (unsigned long long)x);
-}
-
 // ----------------------------------------------------------------- statistics
 __global__ void k_zero_clusters(SweepArgs a, const int32_t* __restrict__ mask) {
   int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -130,36 +110,130 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   return v;
 }
 
-template <int METRIC>
-__global__ __launch_bounds__(kBlock) void k_propose(SweepArgs a) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool evaluated = false;
-  bool proposed = false;
-  int32_t target = kNull;
-  if (i < a.n_items) {
-    const int m = mesh_of_item(a, i);
-    const DevMesh& dm = a.meshes[m];
-    const MeshState& st = a.state[m];
-    if (dm.metric == METRIC && st.phase == kPhaseSweep) {
-      const SweepView v = make_view(a);
-      const int32_t L = st.loops;
-      const int32_t lab = a.labels[i];
-      if (needs_eval(v, i, lab, L)) {
-        evaluated = true;
-        proposed = propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target);
-        if (proposed) {
-          const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
-          if (lab != kNull) atomicMin(&a.prio[lab], key);
-          atomicMin(&a.prio[target], key);
-          a.cand_pass[i] = L;
-        }
-      }
+// Block-wide append: every thread of the block must call it.  Returns the
+// slot of a thread with want == true (one atomic per block on `counter`).
+__device__ inline int32_t block_append(int32_t* counter, bool want) {
+  __shared__ int32_t s_wave[kBlock / 64];
+  __shared__ int32_t s_base;
+  const unsigned long long mask = __ballot(want);
+  const int lane = __lane_id();
+  const int wid = threadIdx.x / 64;
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wave[wid] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      const int32_t c = s_wave[w];
+      s_wave[w] = tot;
+      tot += c;
+    }
+    s_base = tot ? atomicAdd(counter, tot) : 0;
+  }
+  __syncthreads();
+  const int32_t slot = s_base + s_wave[wid] + rank;
+  __syncthreads();  // s_wave / s_base are reused by the next call
+  return want ? slot : -1;
+}
+
+// needs_eval of acvd_math.h with the neighbour loads issued together (up to 8
+// neighbours in registers) so that the label and stamp gathers overlap.
+__device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t lab, int32_t L) {
+  const int32_t r0 = a.row_ptr[i], r1 = a.row_ptr[i + 1];
+  bool dirty = (a.cand_pass[i] == L - 1);
+  if (lab != kNull) dirty |= (a.stamp[lab] + 1 >= L);
+  bool boundary = false;
+  const int32_t deg = r1 - r0;
+  int32_t nb[8], lb[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) nb[k] = k < deg ? a.col[r0 + k] : 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) lb[k] = k < deg ? a.labels[nb[k]] : lab;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (lb[k] != lab) {
+      boundary = true;
+      if (lb[k] != kNull && a.stamp[lb[k]] + 1 >= L) dirty = true;
     }
   }
-  const int32_t slot = wave_append(&a.ctrl->cand_count, proposed);
-  if (proposed) a.cand[slot] = make_int2(int32_t(i), target);
-  block_count_add(&a.ctrl->evaluated, evaluated ? 1u : 0u);
-  block_count_add(&a.ctrl->proposed, proposed ? 1u : 0u);
+  for (int32_t k = r0 + 8; k < r1; ++k) {
+    const int32_t b = a.labels[a.col[k]];
+    if (b != lab) {
+      boundary = true;
+      if (b != kNull && a.stamp[b] + 1 >= L) dirty = true;
+    }
+  }
+  return boundary && dirty;
+}
+
+// Scan: items of sweeping meshes that must be evaluated this pass are
+// compacted into the work list.
+__global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool want = false;
+  if (i < a.n_items) {
+    const int m = mesh_of_item(a, i);
+    const MeshState& st = a.state[m];
+    if (st.phase == kPhaseSweep) want = needs_eval_fast(a, i, a.labels[i], st.loops);
+  }
+  const int slice = blockIdx.x % kSlices;
+  const int32_t slot = block_append(&a.ctrl->work_count[slice], want);
+  if (want) a.work[int64_t(slice) * a.slice_cap + slot] = int32_t(i);
+}
+
+// Maps a position v of the concatenated work-list slices to its item.
+struct WorkIndex {
+  int32_t start[kSlices + 1];
+};
+
+__device__ inline void load_work_index(const SweepArgs& a, WorkIndex* wi) {
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int k = 0; k < kSlices; ++k) {
+      wi->start[k] = s;
+      s += a.ctrl->work_count[k];
+    }
+    wi->start[kSlices] = s;
+  }
+  __syncthreads();
+}
+
+__device__ inline int32_t work_item(const SweepArgs& a, const WorkIndex* wi, int32_t v) {
+  int lo = 0, hi = kSlices;  // find the slice k with start[k] <= v < start[k+1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (wi->start[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return a.work[int64_t(lo) * a.slice_cap + (v - wi->start[lo])];
+}
+
+// Evaluate: best move of each work item (shared rule of acvd_math.h); a
+// proposal registers its accept key at both clusters and joins the
+// candidate list.
+template <int METRIC>
+__global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
+  __shared__ WorkIndex wi;
+  load_work_index(a, &wi);
+  const int32_t n = wi.start[kSlices];
+  const SweepView v = make_view(a);
+  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const int64_t i = work_item(a, &wi, t);
+    const int m = mesh_of_item(a, i);
+    const DevMesh& dm = a.meshes[m];
+    if (dm.metric != METRIC) continue;
+    const MeshState& st = a.state[m];
+    const int32_t lab = a.labels[i];
+    int32_t target = kNull;
+    if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
+      const int32_t L = st.loops;
+      const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
+      if (lab != kNull) atomicMin(&a.prio[lab], key);
+      atomicMin(&a.prio[target], key);
+      a.cand_pass[i] = L;
+      a.cand_target[i] = target;
+    }
+  }
 }
 
 // --------------------------------------------------------------------- apply
@@ -180,19 +254,40 @@ __device__ void update_cluster(const SweepArgs& a, int64_t c, int sign, int64_t 
   }
 }
 
+__device__ inline int block_sum(int v) {
+  __shared__ int s_red[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (__lane_id() == 0) s_red[threadIdx.x / 64] = v;
+  __syncthreads();
+  int tot = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
+  __syncthreads();
+  return tot;  // valid in thread 0
+}
+
+// Apply: a work item that proposed in this pass and holds the minimum key at
+// both clusters moves.  Proposals and moves are counted per block into the
+// sliced counters (single mesh) or per mesh (batches).
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a) {
-  const int32_t n = a.ctrl->cand_count;
-  unsigned accepted = 0;
-  for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    const int2 cd = a.cand[t];
-    const int64_t i = cd.x;
-    const int32_t b = cd.y;
+  __shared__ WorkIndex wi;
+  load_work_index(a, &wi);
+  const int32_t n = wi.start[kSlices];
+  int props = 0, mods = 0;
+  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    const int32_t t = base + int32_t(threadIdx.x);
+    if (t >= n) continue;
+    const int64_t i = work_item(a, &wi, t);
     const int m = mesh_of_item(a, i);
-    const DevMesh dm = a.meshes[m];
     MeshState* st = &a.state[m];
     const int32_t L = st->loops;
+    if (a.cand_pass[i] != L || st->phase != kPhaseSweep) continue;
+    ++props;
+    if (a.item_mesh) atomicAdd(&st->pass_props, 1);
+    const DevMesh dm = a.meshes[m];
     const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
     const int32_t lab = a.labels[i];
+    const int32_t b = a.cand_target[i];
     if (lab != kNull && a.prio[lab] != key) continue;
     if (a.prio[b] != key) continue;
     // exclusive owner of clusters lab and b for this pass
@@ -207,16 +302,24 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a) {
     const int constrained = st->constrained;
     if (lab != kNull) {
       update_cluster(a, lab, -1, iw, is, iq, dm, constrained);
-      atomicSub(&a.sizes[lab], 1);
+      a.sizes[lab] -= 1;
       a.stamp[lab] = L;
     }
     update_cluster(a, b, +1, iw, is, iq, dm, constrained);
-    atomicAdd(&a.sizes[b], 1);
+    a.sizes[b] += 1;
     a.stamp[b] = L;
-    atomicAdd(&st->pass_mods, 1);
-    ++accepted;
+    ++mods;
+    if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
   }
-  block_count_add(&a.ctrl->accepted, accepted);
+  if (!a.item_mesh) {
+    const int tp = block_sum(props);
+    const int tm = block_sum(mods);
+    if (threadIdx.x == 0) {
+      const int slice = blockIdx.x % kSlices;
+      if (tp) atomicAdd(&a.ctrl->props[slice], tp);
+      if (tm) atomicAdd(&a.ctrl->mods[slice], tm);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- controller
@@ -227,11 +330,28 @@ __global__ void k_controller(SweepArgs a) {
     any_clean = 0;
   }
   __syncthreads();
+  __shared__ int s_work, s_props, s_mods;
+  if (threadIdx.x == 0) {
+    int w = 0, p = 0, d = 0;
+    for (int k = 0; k < kSlices; ++k) {
+      w += a.ctrl->work_count[k];
+      p += a.ctrl->props[k];
+      d += a.ctrl->mods[k];
+    }
+    s_work = w;
+    s_props = p;
+    s_mods = d;
+  }
+  __syncthreads();
   bool ran = false;
   for (int m = threadIdx.x; m < a.n_meshes; m += blockDim.x) {
     MeshState s = a.state[m];
     if (s.phase == kPhaseSweep) {
       ran = true;
+      if (!a.item_mesh) {  // one mesh: the sliced counters are its own
+        s.pass_props = s_props;
+        s.pass_mods = s_mods;
+      }
       controller_step(s);
       a.state[m] = s;
     }
@@ -240,8 +360,15 @@ __global__ void k_controller(SweepArgs a) {
   }
   if (ran) atomicOr(&any_sweep, 2);
   __syncthreads();
+  if (threadIdx.x < kSlices) {
+    a.ctrl->work_count[threadIdx.x] = 0;
+    a.ctrl->props[threadIdx.x] = 0;
+    a.ctrl->mods[threadIdx.x] = 0;
+  }
   if (threadIdx.x == 0) {
-    a.ctrl->cand_count = 0;
+    a.ctrl->evaluated += (unsigned long long)s_work;
+    a.ctrl->proposed += (unsigned long long)s_props;
+    a.ctrl->accepted += (unsigned long long)s_mods;
     a.ctrl->any_sweep = any_sweep & 1;
     a.ctrl->any_need_clean = any_clean;
     if (any_sweep & 2) a.ctrl->passes += 1;
@@ -460,10 +587,15 @@ hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mask,
   return hipGetLastError();
 }
 
-hipError_t launch_propose(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem) {
+hipError_t launch_scan(hipStream_t s, const SweepArgs& a) {
   if (a.n_items == 0) return hipSuccess;
-  if (any_iso) hipLaunchKernelGGL(k_propose<0>, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a);
-  if (any_qem) hipLaunchKernelGGL(k_propose<1>, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_scan, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid) {
+  if (any_iso) hipLaunchKernelGGL(k_eval<0>, dim3(grid), dim3(kBlock), 0, s, a);
+  if (any_qem) hipLaunchKernelGGL(k_eval<1>, dim3(grid), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

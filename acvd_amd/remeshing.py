@@ -85,6 +85,11 @@ class Context:
     def num_meshes(self):
         return len(self._nv)
 
+    def _n(self, mesh_id):
+        if not 0 <= mesh_id < len(self._nv):
+            raise AcvdError(f"mesh id {mesh_id} out of range (context holds {len(self._nv)} meshes)")
+        return self._nv[mesh_id]
+
     def set_params(self, mesh_id=-1, **kw):
         p = N.Params()
         if mesh_id >= 0:
@@ -104,24 +109,31 @@ class Context:
 
     def set_indicator(self, mesh_id, indicator):
         ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
-        if ind is not None and len(ind) != self._nv[mesh_id]:
+        if ind is not None and len(ind) != self._n(mesh_id):
             raise ValueError("indicator length must equal the number of vertices")
         check(N.lib().acvd_set_indicator(self._h, mesh_id, _p(ind)))
 
     def set_initial_labels(self, mesh_id, labels):
         lab = None if labels is None else np.ascontiguousarray(labels, np.int32)
-        if lab is not None and len(lab) != self._nv[mesh_id]:
+        if lab is not None and len(lab) != self._n(mesh_id):
             raise ValueError("initial clustering length must equal the number of vertices")
         check(N.lib().acvd_set_initial_labels(self._h, mesh_id, _p(lab)))
 
     def build(self):
         check(N.lib().acvd_build(self._h))
 
+    def initial_sampling(self):
+        """Device initial clustering of every mesh without user labels."""
+        check(N.lib().acvd_initial_sampling(self._h))
+
+    def set_timing(self, enable):
+        check(N.lib().acvd_set_timing(self._h, 1 if enable else 0))
+
     def process(self):
         check(N.lib().acvd_process_clustering(self._h))
 
     def labels(self, mesh_id=0):
-        out = np.empty(self._nv[mesh_id], np.int32)
+        out = np.empty(self._n(mesh_id), np.int32)
         check(N.lib().acvd_get_labels(self._h, mesh_id, _p(out)))
         return out
 
@@ -150,13 +162,13 @@ class Context:
     def adjacency(self, mesh_id=0):
         n = ctypes.c_int64()
         check(N.lib().acvd_get_adjacency_size(self._h, mesh_id, ctypes.byref(n)))
-        rp = np.empty(self._nv[mesh_id] + 1, np.int32)
+        rp = np.empty(self._n(mesh_id) + 1, np.int32)
         col = np.empty(n.value, np.int32)
         check(N.lib().acvd_get_adjacency(self._h, mesh_id, _p(rp), _p(col)))
         return rp, col
 
     def items(self, mesh_id=0):
-        nv = self._nv[mesh_id]
+        nv = self._n(mesh_id)
         xw = np.empty((nv, 4), np.float32)
         q = None
         if self.get_params(mesh_id)["metric"] == N.METRIC_QEM:

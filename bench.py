@@ -1,0 +1,271 @@
+"""Benchmark: ACVDQ vertex clustering on one MI355X.
+
+Workload (the metric's configuration): torus R=1, r=0.35 on a 4096 x 2048
+parametric grid (8,388,608 vertices, 16.8M triangles), tube radius displaced by
+32 seeded sinusoids, FP32 positions, density |H|^1.5 (gradation 1.5 on the
+discrete mean curvature), 100,000 clusters, QEM metric with unconstrained
+initialisation (ACVDQ).  Synthetic, generated from seed 3.
+
+A step is one full clustering convergence (MinimizeEnergy: all passes, cleans
+and fills until every stopping rule is met) from the same device-seeded
+initial labelling, with the mesh, CSR and items resident in HBM.  `value` is
+the median device time of a step in ms (HIP events inside the library);
+`ms_per_step` the wall time per step around the call.
+
+Extra legs:
+  roofline     the dominant sweep kernel's algorithmic bytes per launch over its
+               average launch duration (HIP events, one extra timed step);
+  cpu_baseline the threaded restatement of the reference (ACVDQP,
+               vtkThreadedClustering) on all host cores, on a bounded sample
+               of the same family (a 1024 x 512 torus, 84 items per cluster as in the
+               workload);
+  batched      (--batch) 384 star-shaped icospheres (levels 4-8, iso/QEM 50/50,
+               K = V/15) packed in one context vs clustered one by one.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def torus_workload(nu, nv, K):
+    from acvd_amd import meshgen
+    xyz, tri = meshgen.torus(nu, nv, seed=3)
+    H = meshgen.mean_curvature(xyz, tri)
+    return xyz, tri, H, K
+
+
+def make_ctx(xyz, tri, H, K, seed=0):
+    import acvd_amd
+    ctx = acvd_amd.Context(0)
+    ctx.set_input(xyz, tri)
+    ctx.set_params(0, number_of_clusters=K, metric=acvd_amd.METRIC_QEM, unconstrained_initialization=1,
+                   gradation=1.5, seed=seed)
+    ctx.set_indicator(0, H)
+    return ctx
+
+
+def scan_bytes_per_launch(N, E2, K, work_per_launch):
+    """Algorithmic (unique logical) bytes of one scan launch: each item reads
+    its label, row pointer and last-proposal stamp (12 B) and its CSR row
+    (4 B per entry); the cluster stamps are read once (4 B each); each item
+    to evaluate is written to the work list (4 B)."""
+    return 12.0 * N + 4.0 * E2 + 4.0 * K + 4.0 * work_per_launch
+
+
+def eval_bytes_per_launch(E2, N, K, work_per_launch, proposed_per_launch):
+    """Algorithmic bytes of one QEM evaluate launch: per work item its id,
+    label and row pointers (16 B), FP32 record (16 B position + weight, 40 B
+    quadric) and CSR row (4 B per entry, E2/N on average); each cluster
+    touched is read once (32 B centroid + W, 112 B fixed accumulators, 8 B T,
+    4 B size), at most 3 per work item; each proposal writes its candidate
+    (8 B), stamp (4 B) and two 8-B keys."""
+    touched = min(float(K), 3.0 * work_per_launch)
+    return work_per_launch * (72.0 + 4.0 * E2 / N) + 156.0 * touched + 28.0 * proposed_per_launch
+
+
+def run_gpu_leg(args):
+    import acvd_amd
+    xyz, tri, H, K = torus_workload(4096, 2048, 100000)
+    N = len(xyz)
+    log(f"workload: torus 4096x2048, {N} vertices, {len(tri)} triangles, K={K}")
+    ctx = make_ctx(xyz, tri, H, K)
+    t = time.time()
+    ctx.build()
+    log(f"device build (CSR + items + scales): {time.time() - t:.2f} s wall")
+    ctx.initial_sampling()
+    init = ctx.labels(0)
+    ctx.set_initial_labels(0, init)
+    for w in range(args.warmup):
+        ctx.process()
+        log(f"warmup {w}: {ctx.stats(0)['converge_ms']:.1f} ms")
+    dev_ms, wall_ms = [], []
+    for s in range(args.steps):
+        t = time.perf_counter()
+        ctx.process()
+        wall_ms.append((time.perf_counter() - t) * 1e3)
+        st = ctx.stats(0)
+        dev_ms.append(st["converge_ms"])
+        log(f"step {s}: device {st['converge_ms']:.1f} ms, wall {wall_ms[-1]:.1f} ms, passes {st['loops']}, "
+            f"moves {st['total_moves']}")
+    st = ctx.stats(0)
+    labels = ctx.labels(0)
+    energy = ctx.energy(0)
+    # one extra step with per-kernel events (no graph replay) for the roofline
+    ctx.set_timing(True)
+    ctx.process()
+    tst = ctx.stats(0)
+    ctx.set_timing(False)
+    names = ["k_scan", "k_eval<QEM>", "k_apply", "k_controller"]
+    launches = [max(1, v) for v in tst["kernel_launches"]]
+    avg_ms = [tst["kernel_ms"][k] / launches[k] for k in range(4)]
+    work_per = tst["ctx_evaluated"] / launches[0]
+    prop_per = tst["ctx_proposed"] / launches[0]
+    E2 = tst["n_adjacency"]
+    byts = {0: scan_bytes_per_launch(N, E2, K, work_per), 1: eval_bytes_per_launch(E2, N, K, work_per, prop_per)}
+    for k in range(4):
+        extra = ""
+        if k in byts:
+            extra = f", {byts[k] / 1e6:.1f} MB/launch -> {byts[k] / (avg_ms[k] * 1e-3) / 1e9:.0f} GB/s"
+        log(f"{names[k]}: {launches[k]} launches, total {tst['kernel_ms'][k]:.1f} ms, avg {avg_ms[k] * 1e3:.1f} us"
+            + extra)
+    log(f"work items/pass {work_per:.0f}, proposals/pass {prop_per:.0f}, timed-step total {sum(tst['kernel_ms']):.1f} ms")
+    dom = max((0, 1), key=lambda k: tst["kernel_ms"][k])
+    achieved = byts[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    return {
+        "N": N, "K": K, "init": init, "labels": labels, "dev_ms": dev_ms, "wall_ms": wall_ms, "stats": st,
+        "energy_gpu": energy, "xyz": xyz, "tri": tri, "H": H,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": names[dom], "avg_launch_us": round(avg_ms[dom] * 1e3, 2),
+                     "bytes_per_launch": round(byts[dom]), "launches": launches[dom],
+                     "kernel_ms_per_step": {names[k]: round(tst["kernel_ms"][k], 2) for k in range(4)}},
+    }
+
+
+def run_cpu_baseline(args):
+    """ACVDQP restatement (oracle B) on all host cores, on a bounded sample of
+    the workload family: torus 1024 x 512 (524,288 vertices) with K = 6,250
+    (same items per cluster as the workload), gradation 1.5, from the same
+    kind of device-seeded initial labelling.  The GPU runs the same sample."""
+    from oracle import pyoracle as po
+    xyz, tri, H, K = torus_workload(1024, 512, 6250)
+    ctx = make_ctx(xyz, tri, H, K)
+    ctx.initial_sampling()
+    init = ctx.labels(0)
+    ctx.set_initial_labels(0, init)
+    ctx.process()
+    ctx.process()
+    gpu_sample_ms = ctx.stats(0)["converge_ms"]
+    glab = ctx.labels(0)
+    threads = os.cpu_count() or 1
+    m = po.RefMesh(xyz, tri)
+    lab, st = m.threaded_run(po.QEM, K, 1.5, H, 3, 1, init, threads=threads, pooling=5)
+    cpu_ms = st["seconds"] * 1e3
+    e_cpu = m.energy(po.QEM, K, lab, 1.5, H)[0]
+    e_gpu = m.energy(po.QEM, K, glab, 1.5, H)[0]
+    log(f"cpu_baseline: ACVDQP restatement, {threads} threads, {cpu_ms:.0f} ms on the sample "
+        f"({st['loops']} loops, {st['convergences']} convergences); GPU on the same sample {gpu_sample_ms:.1f} ms; "
+        f"energy gap gpu vs ACVDQP {(e_gpu - e_cpu) / e_cpu * 100:+.3f}%")
+    return {"value": round(cpu_ms, 1), "unit": "ms to converge (sample)", "cores": threads, "kind": "port",
+            "sample": "torus 1024x512 (524,288 vertices), K=6250, QEM, gradation 1.5, same initial labels as the "
+                      "GPU; full convergence of vtkThreadedClustering restated (PoolingRatio 5)",
+            "gpu_same_sample_ms": round(gpu_sample_ms, 2),
+            "speedup_same_sample": round(cpu_ms / gpu_sample_ms, 1),
+            "energy_gap_gpu_vs_cpu_pct": round((e_gpu - e_cpu) / e_cpu * 100, 4),
+            "cpu_loops": st["loops"], "cpu_convergences": st["convergences"]}
+
+
+def run_batch_leg(args):
+    import acvd_amd
+    from acvd_amd import meshgen
+    levels = meshgen.batch_levels(args.batch_count, 4, 8, seed=4)
+    meshes = []
+    for j, L in enumerate(levels):
+        xyz, tri = meshgen.star_icosphere(int(L), seed=4000 + j)
+        K = len(xyz) // 15
+        metric = acvd_amd.METRIC_ISOTROPIC if j % 2 == 0 else acvd_amd.METRIC_QEM
+        meshes.append((xyz, tri, K, metric))
+    tot = sum(len(m[0]) for m in meshes)
+    log(f"batch: {len(meshes)} meshes, {tot} vertices")
+    ctx = acvd_amd.Context(0)
+    inits = []
+    for j, (xyz, tri, K, metric) in enumerate(meshes):
+        mid = ctx.add_input(xyz, tri)
+        ctx.set_params(mid, number_of_clusters=K, metric=metric, unconstrained_initialization=metric, seed=j)
+    ctx.build()
+    ctx.initial_sampling()
+    for j in range(len(meshes)):
+        inits.append(ctx.labels(j))
+        ctx.set_initial_labels(j, inits[-1])
+    ctx.process()
+    t = time.perf_counter()
+    ctx.process()
+    batched_s = time.perf_counter() - t
+    batched_dev = ctx.stats(0)["converge_ms"]
+    # one by one on the same device, same initial labels, contexts built beforehand
+    ctxs = []
+    for j, (xyz, tri, K, metric) in enumerate(meshes):
+        c = acvd_amd.Context(0)
+        c.set_input(xyz, tri)
+        c.set_params(0, number_of_clusters=K, metric=metric, unconstrained_initialization=metric, seed=j)
+        c.set_initial_labels(0, inits[j])
+        c.build()
+        ctxs.append(c)
+    t = time.perf_counter()
+    serial_dev = 0.0
+    for c in ctxs:
+        c.process()
+        serial_dev += c.stats(0)["converge_ms"]
+    serial_s = time.perf_counter() - t
+    # per-mesh equality of the packed run with the one-by-one run
+    same = sum(int(np.array_equal(ctx.labels(j), ctxs[j].labels(0))) for j in range(len(meshes)))
+    for c in ctxs:
+        c.close()
+    n = len(meshes)
+    log(f"batch: packed {batched_s * 1e3:.0f} ms wall ({batched_dev:.0f} ms device), one-by-one "
+        f"{serial_s * 1e3:.0f} ms wall ({serial_dev:.0f} ms device); identical labels {same}/{n}")
+    return {"meshes": n, "vertices": tot, "meshes_per_s": round(n / batched_s, 2),
+            "serial_meshes_per_s": round(n / serial_s, 2), "ratio": round(serial_s / batched_s, 2),
+            "batched_device_ms": round(batched_dev, 1), "serial_device_ms": round(serial_dev, 1),
+            "identical_labels": same}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch", action="store_true", help="also run the 384-mesh batched leg")
+    ap.add_argument("--batch-count", type=int, default=384)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    g = run_gpu_leg(args)
+    out = {
+        "metric": "10M->100k ACVDQ converge ms; batched vs serial meshes/s at <=0.1% energy gap",
+        "value": round(statistics.median(g["dev_ms"]), 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(statistics.mean(g["wall_ms"]), 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64solve+i64fixed",
+        "data": "synthetic",
+        "config": {"workload": "torus 4096x2048 (8,388,608 vertices, 16.8M triangles) -> 100,000 clusters, QEM "
+                               "(ACVDQ, unconstrained init), gradation 1.5 on |H|, FP32 positions",
+                   "items": g["N"], "clusters": g["K"], "parallelism": "single GPU"},
+        "passes": g["stats"]["loops"],
+        "moves": g["stats"]["total_moves"],
+        "item_moves_per_s": round(g["stats"]["total_moves"] / (statistics.median(g["dev_ms"]) * 1e-3)),
+        "energy_gpu": g["energy_gpu"],
+        "roofline": g["roofline"],
+    }
+    if not args.skip_cpu:
+        out["cpu_baseline"] = run_cpu_baseline(args)
+    if args.batch:
+        out["batched"] = run_batch_leg(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,10 @@ typedef struct acvd_stats {
   double build_ms;          /* device time of the CSR + item build */
   int64_t n_items;          /* items of this mesh */
   int64_t n_adjacency;      /* CSR entries of this mesh (2E) */
+  /* per-kernel device time when timing is on (acvd_set_timing), context-wide:
+   * [0] scan, [1] evaluate, [2] apply, [3] controller */
+  double kernel_ms[4];
+  int64_t kernel_launches[4];
 } acvd_stats;
 
 /* ---- library -------------------------------------------------------------- */
@@ -130,6 +134,19 @@ int acvd_set_initial_labels(acvd_ctx* ctx, int32_t mesh_id, const int32_t* label
  * initial sampling, then MinimizeEnergy (:724-830) with the boundary sweep on
  * the GPU.  Returns ACVD_ERR_ARG if a mesh has number_of_clusters <= 0. */
 int acvd_process_clustering(acvd_ctx* ctx);
+
+/* InitSamples (vtkUniformClustering.h:1123-1175) followed by the first
+ * FillHolesInClustering of MinimizeEnergy (:727), without clustering: meshes
+ * without user labels get the device's seeded initial clustering (weighted
+ * random seeds), then null items are grown into their neighbours' clusters.
+ * acvd_get_labels then returns this starting labelling. */
+int acvd_initial_sampling(acvd_ctx* ctx);
+
+/* Per-kernel timing (no counterpart in the reference): when enabled, passes
+ * are launched one by one with HIP events around each sweep kernel (instead
+ * of replaying the captured chunk graph) and the times are reported in
+ * acvd_stats.kernel_ms / kernel_launches. */
+int acvd_set_timing(acvd_ctx* ctx, int enable);
 
 /* ---- results (copies into caller buffers) --------------------------------- */
 /* GetClustering() (vtkUniformClustering.h:78): nv labels in 0..K. */

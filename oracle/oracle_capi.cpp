@@ -7,15 +7,18 @@
 //   B  threaded restatement of vtkThreadedClustering (threaded.h)
 //   C  replay of the GPU schedule (replay.h)
 //   E  FP64 unshifted energy of a labelling (energy below)
-// plus the reference's quadric tools and ring check for unit tests.
+// plus the reference's quadric tools and ring check for unit tests, and a
+// host build of the device sweep inputs (devinputs.h).
 
 #include <chrono>
+#include <random>
 #include <cstdint>
 #include <cstring>
 #include <exception>
 #include <string>
 #include <vector>
 
+#include "devinputs.h"
 #include "metrics.h"
 #include "quadric.h"
 #include "ref_mesh.h"
@@ -295,6 +298,38 @@ int orc_hdr_ring_problem(const int32_t* rp, const int32_t* col, const int32_t* l
 
 uint64_t orc_hdr_accept_key(uint32_t seed, int32_t loop, uint32_t item) { return acvd::accept_key(seed, loop, item); }
 
+// Hartigan terms of the shared header: out2 = {leave term w.r.t. (ca, Wa),
+// join term w.r.t. (cb, Wb)}.
+void orc_hdr_terms(const float* xw4, const double* ca, double Wa, const double* cb, double Wb, float* out2) {
+  acvd::ClusterGeom a{ca[0], ca[1], ca[2], Wa}, b{cb[0], cb[1], cb[2], Wb};
+  out2[0] = acvd::leave_term(xw4[0], xw4[1], xw4[2], xw4[3], a);
+  out2[1] = acvd::join_term(xw4[0], xw4[1], xw4[2], xw4[3], b);
+}
+
+// T = W |q - c|^2 and q of a cluster from fixed accumulators (shared header).
+double orc_hdr_qem_T(const int64_t* ws4, const int64_t* q10, const double* scales3, int level, double* q_out) {
+  acvd::Scales sc{scales3[0], scales3[1], scales3[2], 1.0 / scales3[0], 1.0 / scales3[1], 1.0 / scales3[2]};
+  acvd::ClusterGeom g = acvd::geom_from_fixed(ws4[0], ws4[1], ws4[2], ws4[3], sc);
+  return acvd::qem_T_from_fixed(q10, g, sc, level, q_out);
+}
+
+// Fixed-point image of one item (shared header): out = {w, wx, wy, wz, q0..q9}.
+void orc_hdr_fix_item(const float* xw4, const float* q10, const double* scales3, int64_t* out14) {
+  acvd::Scales sc{scales3[0], scales3[1], scales3[2], 1.0 / scales3[0], 1.0 / scales3[1], 1.0 / scales3[2]};
+  out14[0] = acvd::fix_w(xw4[3], sc);
+  for (int k = 0; k < 3; ++k) out14[1 + k] = acvd::fix_s(xw4[3], xw4[k], sc);
+  for (int k = 0; k < 10; ++k) out14[4 + k] = q10 ? acvd::fix_q(q10[k], sc) : 0;
+}
+
+// The C++ standard fixes the 10000th output of a default-seeded std::mt19937
+// (4123659995); the reference's initial sampling draws from this engine.
+uint32_t orc_mt19937_10000(void) {
+  std::mt19937 g;
+  uint32_t v = 0;
+  for (int i = 0; i < 10000; ++i) v = g();
+  return v;
+}
+
 // Replay C.  params: K, metric, level, seed, unconstrained, max_loops,
 // max_convergences.  scales3 = {sw, ss, sq}.  init/out in the reference
 // convention (K = null).
@@ -332,5 +367,23 @@ int orc_replay_run(int64_t n, const int32_t* rp, const int32_t* col, const float
   }
   return 0;
 }
+
+// Host build of the device sweep inputs (devinputs.h).
+void* orc_devinputs_create(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
+                           const double* ind, double gradation) {
+  DevInputs* d = new DevInputs();
+  build_device_inputs(nv, xyz, nf, tri, metric, ind, gradation, *d);
+  return d;
+}
+int64_t orc_devinputs_ncol(void* h) { return int64_t(static_cast<DevInputs*>(h)->col.size()); }
+void orc_devinputs_get(void* h, int32_t* rp, int32_t* col, float* xw, float* q, double* scales3) {
+  DevInputs* d = static_cast<DevInputs*>(h);
+  std::memcpy(rp, d->row_ptr.data(), d->row_ptr.size() * sizeof(int32_t));
+  std::memcpy(col, d->col.data(), d->col.size() * sizeof(int32_t));
+  std::memcpy(xw, d->xw.data(), d->xw.size() * sizeof(float));
+  if (q && !d->q.empty()) std::memcpy(q, d->q.data(), d->q.size() * sizeof(float));
+  for (int k = 0; k < 3; ++k) scales3[k] = d->scales[k];
+}
+void orc_devinputs_destroy(void* h) { delete static_cast<DevInputs*>(h); }
 
 }  // extern "C"

@@ -65,6 +65,17 @@ def lib():
         L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32]
         L.orc_replay_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
+        L.orc_devinputs_create.restype = P
+        L.orc_devinputs_create.argtypes = [ctypes.c_int64, P, ctypes.c_int64, P, ctypes.c_int, P, ctypes.c_double]
+        L.orc_devinputs_ncol.restype = ctypes.c_int64
+        L.orc_devinputs_ncol.argtypes = [P]
+        L.orc_devinputs_get.argtypes = [P, P, P, P, P, P]
+        L.orc_devinputs_destroy.argtypes = [P]
+        L.orc_hdr_terms.argtypes = [P, P, ctypes.c_double, P, ctypes.c_double, P]
+        L.orc_hdr_qem_T.restype = ctypes.c_double
+        L.orc_hdr_qem_T.argtypes = [P, P, P, ctypes.c_int, P]
+        L.orc_hdr_fix_item.argtypes = [P, P, P, P]
+        L.orc_mt19937_10000.restype = ctypes.c_uint32
         L.orc_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -217,3 +228,49 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
                          max_conv, _p(ini), _p(out), _p(st))
     keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
     return out, dict(zip(keys, (int(v) for v in st)))
+
+
+def device_inputs(xyz, tri, metric, indicator=None, gradation=0.0):
+    """Host build of the sweep inputs: (row_ptr, col, xw[n,4], q[n,10] or None, scales[3])."""
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+    ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+    h = lib().orc_devinputs_create(len(x), _p(x), len(t), _p(t), metric, _p(ind), gradation)
+    try:
+        ncol = lib().orc_devinputs_ncol(h)
+        rp = np.empty(len(x) + 1, np.int32)
+        col = np.empty(ncol, np.int32)
+        xw = np.empty((len(x), 4), np.float32)
+        q = np.empty((len(x), 10), np.float32) if metric == QEM else None
+        sc = np.empty(3, np.float64)
+        lib().orc_devinputs_get(h, _p(rp), _p(col), _p(xw), _p(q), _p(sc))
+    finally:
+        lib().orc_devinputs_destroy(h)
+    return rp, col, xw, q, sc
+
+
+def hdr_terms(xw4, ca, Wa, cb, Wb):
+    x = np.ascontiguousarray(xw4, np.float32)
+    out = np.empty(2, np.float32)
+    lib().orc_hdr_terms(_p(x), _p(np.ascontiguousarray(ca, np.float64)), float(Wa),
+                        _p(np.ascontiguousarray(cb, np.float64)), float(Wb), _p(out))
+    return float(out[0]), float(out[1])
+
+
+def hdr_qem_T(ws4, q10, scales, level=3):
+    q = np.empty(3)
+    T = lib().orc_hdr_qem_T(_p(np.ascontiguousarray(ws4, np.int64)), _p(np.ascontiguousarray(q10, np.int64)),
+                            _p(np.ascontiguousarray(scales, np.float64)), level, _p(q))
+    return T, q
+
+
+def hdr_fix_item(xw4, q10, scales):
+    out = np.empty(14, np.int64)
+    qq = None if q10 is None else np.ascontiguousarray(q10, np.float32)
+    lib().orc_hdr_fix_item(_p(np.ascontiguousarray(xw4, np.float32)), _p(qq),
+                           _p(np.ascontiguousarray(scales, np.float64)), _p(out))
+    return out
+
+
+def mt19937_10000():
+    return int(lib().orc_mt19937_10000())

@@ -173,6 +173,7 @@ struct Replay {
       mods++;
     }
     st.pass_mods = mods;
+    st.pass_props = int32_t(cands.size());
   }
 
   // Largest connected component per cluster (ties: smallest first item);

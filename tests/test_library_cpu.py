@@ -1,0 +1,64 @@
+"""The C-ABI library loads without a GPU, exports every entry point declared
+in include/acvd_c.h, and fails loudly (status + message) when no device is
+visible.  No compute calls here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import acvd_amd
+from acvd_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "acvd_c.h")).read()
+    return sorted(set(re.findall(r"\b(acvd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = N.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(N.EXPORTS) == syms
+
+
+def test_version_and_defaults():
+    L = N.lib()
+    assert b"gfx950" in L.acvd_version()
+    p = N.Params()
+    L.acvd_params_init(ctypes.byref(p))
+    assert p.number_of_clusters == 0 and p.metric == 0 and p.quadrics_optimization_level == 3
+    assert p.max_number_of_loops == 5000000 and p.max_number_of_convergences == 1000000000
+    assert p.gradation == 0.0 and p.unconstrained_initialization == 0
+
+
+@pytest.mark.skipif(N.lib().acvd_device_count() > 0, reason="a device is visible")
+def test_no_device_fails_loudly():
+    with pytest.raises(N.AcvdError, match="no HIP device"):
+        acvd_amd.Context()
+    r = acvd_amd.IsotropicDiscreteRemeshing.New()
+    xyz, tri = acvd_amd.meshgen.icosphere(2)
+    r.SetInput(acvd_amd.Mesh(xyz, tri))
+    r.SetNumberOfClusters(10)
+    with pytest.raises(N.AcvdError):
+        r.ProcessClustering()
+
+
+def test_zero_clusters_returns_none_like_the_reference():
+    r = acvd_amd.QIsotropicDiscreteRemeshing.New()
+    xyz, tri = acvd_amd.meshgen.icosphere(1)
+    r.SetInput(acvd_amd.Mesh(xyz, tri))
+    assert r.ProcessClustering() is None  # vtkUniformClustering.h:659-664
+
+
+def test_null_context_is_rejected():
+    L = N.lib()
+    assert L.acvd_process_clustering(None) == -1
+    assert b"null context" in L.acvd_last_error()
+    assert L.acvd_destroy(None) == 0

@@ -1,0 +1,100 @@
+"""Ring check (ConnexityConstraintProblemLocal, vtkVerticesProcessing.h:168-237):
+the device header's bitmask version against the oracle's restatement of the
+reference BFS and a direct graph search, on every label pattern of 1-rings of
+valence 3..8 (open fans and closed rings) and of special meshes where ring
+vertices have extra adjacency (tetrahedron, octahedron)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+
+def csr(n, tri):
+    adj = [set() for _ in range(n)]
+    for a, b, c in tri:
+        for u, v in ((a, b), (b, c), (c, a)):
+            adj[u].add(v)
+            adj[v].add(u)
+    rp = np.zeros(n + 1, np.int32)
+    col = []
+    for i in range(n):
+        row = sorted(adj[i])
+        col += row
+        rp[i + 1] = rp[i] + len(row)
+    return rp, np.array(col, np.int32), adj
+
+
+def direct(adj, labels, item, cluster):
+    ring = [v for v in adj[item] if labels[v] == cluster]
+    if not ring:
+        return 0
+    seen = {ring[0]}
+    stack = [ring[0]]
+    while stack:
+        u = stack.pop()
+        for v in adj[u]:
+            if v in ring and v not in seen:
+                seen.add(v)
+                stack.append(v)
+    return 0 if len(seen) == len(ring) else 1
+
+
+def fan(d, closed):
+    xyz = [[0, 0, 0]] + [[np.cos(2 * np.pi * k / d), np.sin(2 * np.pi * k / d), 0] for k in range(d)]
+    tri = [[0, 1 + k, 1 + (k + 1) % d] for k in range(d if closed else d - 1)]
+    return np.array(xyz, float), np.array(tri, np.int32)
+
+
+def special_meshes():
+    tet = (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float),
+           np.array([[0, 1, 2], [0, 3, 1], [0, 2, 3], [1, 3, 2]], np.int32))
+    octa = (np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float),
+            np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]],
+                     np.int32))
+    return [tet, octa]
+
+
+def check_all_patterns(xyz, tri):
+    n = len(xyz)
+    rp, col, adj = csr(n, tri)
+    m = po.RefMesh(xyz, tri)
+    for item in range(n):
+        ring = sorted(adj[item])
+        for bits in itertools.product([0, 1], repeat=len(ring)):
+            labels = np.ones(n, np.int32)  # cluster 1 everywhere
+            labels[item] = 0
+            for v, b in zip(ring, bits):
+                labels[v] = 0 if b else 1
+            want = direct(adj, labels, item, 0)
+            assert m.ring_problem(labels, item, 0) == want
+            assert po.hdr_ring_problem(rp, col, labels, item, 0) == want
+
+
+@pytest.mark.parametrize("d", range(3, 9))
+@pytest.mark.parametrize("closed", [True, False])
+def test_fans(d, closed):
+    xyz, tri = fan(d, closed)
+    check_all_patterns(xyz, tri)
+
+
+def test_special_meshes():
+    for xyz, tri in special_meshes():
+        check_all_patterns(xyz, tri)
+
+
+def test_random_labelings_on_a_sphere():
+    from acvd_amd import meshgen
+    xyz, tri = meshgen.icosphere(2)
+    n = len(xyz)
+    rp, col, adj = csr(n, tri)
+    m = po.RefMesh(xyz, tri)
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        labels = rng.integers(0, 3, n).astype(np.int32)
+        item = int(rng.integers(n))
+        c = int(labels[item])
+        want = direct(adj, labels, item, c)
+        assert m.ring_problem(labels, item, c) == want
+        assert po.hdr_ring_problem(rp, col, labels, item, c) == want
