@@ -67,7 +67,8 @@ EXPORTS = [
     "acvd_set_params", "acvd_get_params", "acvd_set_indicator", "acvd_set_initial_labels",
     "acvd_process_clustering", "acvd_get_labels", "acvd_get_cluster_sizes", "acvd_get_cluster_points",
     "acvd_get_energy", "acvd_get_stats", "acvd_get_adjacency_size", "acvd_get_adjacency", "acvd_get_items",
-    "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing",
+    "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
+    "acvd_get_output_sizes", "acvd_get_output",
 ]
 
 _lib = None
@@ -104,6 +105,9 @@ def lib():
     L.acvd_build.argtypes = [P]
     L.acvd_initial_sampling.argtypes = [P]
     L.acvd_set_timing.argtypes = [P, ctypes.c_int]
+    L.acvd_build_output.argtypes = [P, i32, i32]
+    L.acvd_get_output_sizes.argtypes = [P, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.acvd_get_output.argtypes = [P, i32, P, P]
     for f in ("acvd_get_labels", "acvd_get_cluster_sizes", "acvd_get_cluster_points", "acvd_get_energy",
               "acvd_get_scales"):
         getattr(L, f).argtypes = [P, i32, P]

@@ -71,6 +71,7 @@ struct SweepArgs {
   double* T;                   // W |q - c|^2 per cluster (QEM), may be null
   int32_t* sizes;
   int32_t* stamp;              // mesh-local loop of the last modification
+  int32_t* moved;              // mesh-local loop of the last applied move
   unsigned long long* prio;    // accept keys per cluster
   int32_t* cand_pass;          // per item: loop of its last proposal
   int32_t* cand_target;        // per item: target cluster of its proposal
@@ -119,6 +120,13 @@ hipError_t sort_pairs_u32(hipStream_t s, const uint32_t* kin, uint32_t* kout, co
 hipError_t launch_cluster_points(hipStream_t s, const SweepArgs& a, int32_t mesh, double* pts);
 hipError_t launch_item_energy(hipStream_t s, const SweepArgs& a, const double* pts_all, double* e_item);
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+// output mesh
+hipError_t build_dual_faces(hipStream_t s, int64_t nf, const int32_t* tri, const int32_t* labels, int64_t item_off,
+                            int64_t clus_off, int32_t* out_tri, int64_t* out_nf);
+hipError_t fix_empty_cluster_points(hipStream_t s, const int32_t* sizes, int64_t clus_off, int32_t K, int32_t* first,
+                                    double* pts);
+hipError_t quadric_placement(hipStream_t s, int64_t n, int64_t item_off, const int32_t* labels, int64_t clus_off,
+                             int32_t K, const float* q, double sq, unsigned long long* acc, int level, double* pts);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
 
 }  // namespace acvd

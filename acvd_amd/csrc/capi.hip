@@ -69,7 +69,11 @@ struct HostMesh {
   std::vector<int32_t> init_labels;
   acvd_params params;
   // layout in the packed context
-  int64_t item_off = 0, clus_off = 0, n_adj = 0, adj_off = 0;
+  int64_t item_off = 0, clus_off = 0, face_off = 0, n_adj = 0, adj_off = 0;
+  // output mesh (BuildDelaunayTriangulation)
+  std::vector<double> out_xyz;
+  std::vector<int32_t> out_tri;
+  bool clustered = false;
   acvd_stats stats;
 };
 
@@ -93,7 +97,7 @@ struct acvd_ctx {
   DBuf<int32_t> d_row_ptr, d_col, d_tri;
   DBuf<float4> d_xw;
   DBuf<float> d_q;
-  DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
+  DBuf<int32_t> d_labels, d_sizes, d_stamp, d_moved, d_cand_pass, d_pending;
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T;
@@ -141,7 +145,7 @@ struct acvd_ctx {
     a.row_ptr = d_row_ptr.p;
     a.col = d_col.p;
     a.xw = d_xw.p;
-    a.q = any_qem ? d_q.p : nullptr;
+    a.q = d_q.p;
     a.labels = d_labels.p;
     a.acc_ws = d_acc_ws.p;
     a.acc_q = any_qem ? d_acc_q.p : nullptr;
@@ -149,6 +153,7 @@ struct acvd_ctx {
     a.T = any_qem ? d_T.p : nullptr;
     a.sizes = d_sizes.p;
     a.stamp = d_stamp.p;
+    a.moved = d_moved.p;
     a.prio = d_prio.p;
     a.cand_pass = d_cand_pass.p;
     a.cand_target = d_cand_target.p;
@@ -264,6 +269,7 @@ int build(acvd_ctx* c) {
       std::copy(h.indicator.begin(), h.indicator.end(), ind.begin() + h.item_off);
     }
     for (int64_t k = 0; k < 3 * h.nf; ++k) tri[3 * foff + k] = int32_t(h.tri[k] + h.item_off);
+    h.face_off = foff;
     foff += h.nf;
     if (M > 1) {
       std::fill(item_mesh.begin() + h.item_off, item_mesh.begin() + h.item_off + h.nv, m);
@@ -332,11 +338,11 @@ int build(acvd_ctx* c) {
     HIP_TRY(launch_csr_compact(s, N, tptr.p, tcol.p, c->d_row_ptr.p, c->d_col.p));
     // ---- items
     HIP_TRY(c->d_xw.alloc(N));
-    if (c->any_qem) HIP_TRY(c->d_q.alloc(10 * N));
+    HIP_TRY(c->d_q.alloc(10 * N));  // vertex quadrics of every mesh
     HIP_TRY(d_raw.alloc(N));
     SweepArgs a = c->args();
     HIP_TRY(launch_item_build(s, a, d_pos.p, c->d_tri.p, vfptr.p, vfcol.p, any_ind ? d_ind.p : nullptr, d_raw.p,
-                              c->d_xw.p, c->any_qem ? c->d_q.p : nullptr));
+                              c->d_xw.p, c->d_q.p));
   }
   // ---- deterministic reductions: clamp then scales
   {
@@ -361,6 +367,7 @@ int build(acvd_ctx* c) {
   HIP_TRY(c->d_labels.alloc(N));
   HIP_TRY(c->d_sizes.alloc(K));
   HIP_TRY(c->d_stamp.alloc(K));
+  HIP_TRY(c->d_moved.alloc(K));
   HIP_TRY(c->d_cand_pass.alloc(N));
   HIP_TRY(c->d_pending.alloc(N));
   HIP_TRY(c->d_acc_ws.alloc(4 * K));
@@ -526,6 +533,7 @@ int minimize_energy(acvd_ctx* c) {
   for (int m = 0; m < M; ++m) c->h_state[m].early_stop_items = int64_t(cnt[m]);
   HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill_i32(s, c->d_cand_pass.p, c->N, -2));
+  HIP_TRY(launch_fill_i32(s, c->d_moved.p, c->K, -2));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
   for (int k = 0; k < 4; ++k) {
@@ -761,6 +769,9 @@ int acvd_process_clustering(acvd_ctx* ctx) {
   ctx->converge_ms = ms;
   for (int m = 0; m < M; ++m) {
     HostMesh& h = ctx->meshes[m];
+    h.clustered = true;
+    h.out_xyz.clear();
+    h.out_tri.clear();
     const MeshState& st = ctx->h_state[m];
     acvd_stats& S = h.stats;
     S.loops = st.loops;
@@ -802,6 +813,60 @@ int acvd_initial_sampling(acvd_ctx* ctx) {
 int acvd_set_timing(acvd_ctx* ctx, int enable) {
   if (int rc = check_ctx(ctx)) return rc;
   ctx->timing = enable != 0;
+  return ACVD_OK;
+}
+
+int acvd_build_output(acvd_ctx* ctx, int32_t mesh_id, int32_t quadrics_level) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  HostMesh& h = ctx->meshes[mesh_id];
+  if (!ctx->built || !h.clustered) return fail(ACVD_ERR_STATE, "no clustering computed");
+  if (quadrics_level < 0 || quadrics_level > 3) return fail(ACVD_ERR_ARG, "quadrics level must be 0..3");
+  if (h.params.number_of_clusters >= (1 << 21)) return fail(ACVD_ERR_ARG, "output mesh limited to 2^21 - 1 clusters");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int32_t K = h.params.number_of_clusters;
+  DBuf<double> pts;
+  DBuf<int32_t> first, otri;
+  HIP_TRY(pts.alloc(3 * size_t(K)));
+  HIP_TRY(first.alloc(1));
+  HIP_TRY(otri.alloc(3 * size_t(h.nf)));
+  HIP_TRY(launch_cluster_points(s, ctx->args(), mesh_id, pts.p));
+  HIP_TRY(launch_fill_i32(s, first.p, 1, K));
+  HIP_TRY(fix_empty_cluster_points(s, ctx->d_sizes.p, h.clus_off, K, first.p, pts.p));
+  if (quadrics_level > 0) {
+    DBuf<unsigned long long> acc;
+    HIP_TRY(acc.alloc(10 * size_t(K)));
+    HIP_TRY(quadric_placement(s, h.nv, h.item_off, ctx->d_labels.p, h.clus_off, K, ctx->d_q.p,
+                              ctx->h_meshes[mesh_id].scales.sq, acc.p, quadrics_level, pts.p));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  int64_t nf_out = 0;
+  HIP_TRY(build_dual_faces(s, h.nf, ctx->d_tri.p + 3 * h.face_off, ctx->d_labels.p, h.item_off, h.clus_off, otri.p,
+                           &nf_out));
+  h.out_xyz.resize(3 * size_t(K));
+  h.out_tri.resize(3 * size_t(nf_out));
+  HIP_TRY(hipMemcpyAsync(h.out_xyz.data(), pts.p, 3 * size_t(K) * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (nf_out)
+    HIP_TRY(hipMemcpyAsync(h.out_tri.data(), otri.p, 3 * size_t(nf_out) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return ACVD_OK;
+}
+
+int acvd_get_output_sizes(acvd_ctx* ctx, int32_t mesh_id, int64_t* nv, int64_t* nf) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  const HostMesh& h = ctx->meshes[mesh_id];
+  if (h.out_xyz.empty()) return fail(ACVD_ERR_STATE, "output mesh not built (acvd_build_output)");
+  *nv = int64_t(h.out_xyz.size() / 3);
+  *nf = int64_t(h.out_tri.size() / 3);
+  return ACVD_OK;
+}
+
+int acvd_get_output(acvd_ctx* ctx, int32_t mesh_id, double* xyz, int32_t* tri) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  const HostMesh& h = ctx->meshes[mesh_id];
+  if (h.out_xyz.empty()) return fail(ACVD_ERR_STATE, "output mesh not built (acvd_build_output)");
+  if (xyz) std::memcpy(xyz, h.out_xyz.data(), h.out_xyz.size() * sizeof(double));
+  if (tri) std::memcpy(tri, h.out_tri.data(), h.out_tri.size() * sizeof(int32_t));
   return ACVD_OK;
 }
 
@@ -895,10 +960,7 @@ int acvd_get_items(acvd_ctx* ctx, int32_t mesh_id, float* xw, float* q) {
   if (!ctx->built) return fail(ACVD_ERR_STATE, "device inputs not built");
   const HostMesh& h = ctx->meshes[mesh_id];
   if (xw) HIP_TRY(hipMemcpy(xw, ctx->d_xw.p + h.item_off, h.nv * sizeof(float4), hipMemcpyDeviceToHost));
-  if (q) {
-    if (h.params.metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "mesh has no quadrics (isotropic metric)");
-    HIP_TRY(hipMemcpy(q, ctx->d_q.p + 10 * h.item_off, 10 * h.nv * sizeof(float), hipMemcpyDeviceToHost));
-  }
+  if (q) HIP_TRY(hipMemcpy(q, ctx->d_q.p + 10 * h.item_off, 10 * h.nv * sizeof(float), hipMemcpyDeviceToHost));
   return ACVD_OK;
 }
 

@@ -104,6 +104,7 @@ __global__ void k_item_build(SweepArgs a, const double* __restrict__ pos, const 
   double area = 0.0;
   double Q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const bool qem = dm.metric == 1;
+  const bool quadrics = q != nullptr;  // vertex quadrics of every mesh (sweep and output placement)
   for (int32_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
     const int64_t f = vfcol[k];
     const int32_t i0 = tri[3 * f], i1 = tri[3 * f + 1], i2 = tri[3 * f + 2];
@@ -115,7 +116,7 @@ __global__ void k_item_build(SweepArgs a, const double* __restrict__ pos, const 
     double ec = (p3[0] - p1[0]) * (p3[0] - p1[0]) + (p3[1] - p1[1]) * (p3[1] - p1[1]) + (p3[2] - p1[2]) * (p3[2] - p1[2]);
     double t = 4.0 * ea * ec - (ea - eb + ec) * (ea - eb + ec);
     area += 0.25 * sqrt(t < 0 ? -t : t) / 3.0;
-    if (qem) {
+    if (quadrics) {
       // plane of the triangle: n = (p2-p1) x (p3-p1), d = -n.p1
       double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
       double w[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
@@ -132,7 +133,7 @@ __global__ void k_item_build(SweepArgs a, const double* __restrict__ pos, const 
   if (dm.has_indicator && (!qem || dm.gradation > 0.0)) wgt *= pow(indicator[v], dm.gradation);
   raw_w[v] = wgt;
   xw[v] = make_float4(float(pos[3 * v]), float(pos[3 * v + 1]), float(pos[3 * v + 2]), 0.0f);
-  if (qem)
+  if (quadrics)
     for (int k = 0; k < 10; ++k) q[10 * v + k] = float(Q[k]);
 }
 
@@ -156,12 +157,9 @@ __global__ void k_seg_reduce(const int2* __restrict__ block_tab, SweepArgs a, co
       v0 += w;
       v1 += w * mx;
       if (a.q) {
-        const int m = a.item_mesh ? a.item_mesh[i] : 0;
-        if (a.meshes[m].metric == 1) {
-          double qm = 0;
-          for (int k = 0; k < 10; ++k) qm = fmax(qm, fabs(double(a.q[10 * i + k])));
-          v2 += qm;
-        }
+        double qm = 0;
+        for (int k = 0; k < 10; ++k) qm = fmax(qm, fabs(double(a.q[10 * i + k])));
+        v2 += qm;
       }
       v3 = fmax(v3, w);
     }

@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
     int32_t target = kNull;
     if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
       const int32_t L = st.loops;
-      const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
+      const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
       if (lab != kNull) atomicMin(&a.prio[lab], key);
       atomicMin(&a.prio[target], key);
       a.cand_pass[i] = L;
@@ -266,10 +266,32 @@ __device__ inline int block_sum(int v) {
   return tot;  // valid in thread 0
 }
 
-// Apply: a work item that proposed in this pass and holds the minimum key at
-// both clusters moves.  Proposals and moves are counted per block into the
-// sliced counters (single mesh) or per mesh (batches).
-__global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a) {
+// Rematch (rounds >= 1): proposals of this pass that are not applied and
+// whose two clusters have not moved register their round key.
+__global__ __launch_bounds__(kBlock) void k_rematch(SweepArgs a, int round) {
+  __shared__ WorkIndex wi;
+  load_work_index(a, &wi);
+  const int32_t n = wi.start[kSlices];
+  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const int64_t i = work_item(a, &wi, t);
+    const int m = mesh_of_item(a, i);
+    const MeshState& st = a.state[m];
+    const int32_t L = st.loops;
+    if (a.cand_pass[i] != L || st.phase != kPhaseSweep) continue;
+    const int32_t lab = a.labels[i];
+    const int32_t b = a.cand_target[i];
+    if (lab == b) continue;  // applied in an earlier round
+    if ((lab != kNull && a.moved[lab] == L) || a.moved[b] == L) continue;
+    const unsigned long long key = accept_key(a.meshes[m].seed, L, round, uint32_t(i - a.meshes[m].item_off));
+    if (lab != kNull) atomicMin(&a.prio[lab], key);
+    atomicMin(&a.prio[b], key);
+  }
+}
+
+// Apply round `round`: a proposal of this pass holding the minimum round key
+// at both clusters moves.  Round 0 also counts the proposals.  Counts go to
+// the sliced counters (single mesh) or per mesh (batches).
+__global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ WorkIndex wi;
   load_work_index(a, &wi);
   const int32_t n = wi.start[kSlices];
@@ -282,12 +304,16 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a) {
     MeshState* st = &a.state[m];
     const int32_t L = st->loops;
     if (a.cand_pass[i] != L || st->phase != kPhaseSweep) continue;
-    ++props;
-    if (a.item_mesh) atomicAdd(&st->pass_props, 1);
-    const DevMesh dm = a.meshes[m];
-    const unsigned long long key = accept_key(dm.seed, L, uint32_t(i - dm.item_off));
     const int32_t lab = a.labels[i];
     const int32_t b = a.cand_target[i];
+    if (round == 0) {
+      ++props;
+      if (a.item_mesh) atomicAdd(&st->pass_props, 1);
+    } else if (lab == b) {
+      continue;  // applied in an earlier round
+    }
+    const DevMesh dm = a.meshes[m];
+    const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
     if (lab != kNull && a.prio[lab] != key) continue;
     if (a.prio[b] != key) continue;
     // exclusive owner of clusters lab and b for this pass
@@ -304,10 +330,12 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a) {
       update_cluster(a, lab, -1, iw, is, iq, dm, constrained);
       a.sizes[lab] -= 1;
       a.stamp[lab] = L;
+      a.moved[lab] = L;
     }
     update_cluster(a, b, +1, iw, is, iq, dm, constrained);
     a.sizes[b] += 1;
     a.stamp[b] = L;
+    a.moved[b] = L;
     ++mods;
     if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
   }
@@ -600,7 +628,11 @@ hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any
 }
 
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid) {
-  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, 0);
+  for (int r = 1; r < kMatchRounds; ++r) {
+    hipLaunchKernelGGL(k_rematch, dim3(grid), dim3(kBlock), 0, s, a, r);
+    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, r);
+  }
   return hipGetLastError();
 }
 

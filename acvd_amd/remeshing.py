@@ -170,11 +170,19 @@ class Context:
     def items(self, mesh_id=0):
         nv = self._n(mesh_id)
         xw = np.empty((nv, 4), np.float32)
-        q = None
-        if self.get_params(mesh_id)["metric"] == N.METRIC_QEM:
-            q = np.empty((nv, 10), np.float32)
+        q = np.empty((nv, 10), np.float32)
         check(N.lib().acvd_get_items(self._h, mesh_id, _p(xw), _p(q)))
         return xw, q
+
+    def output(self, mesh_id=0, quadrics_level=0):
+        """Output mesh (cluster points, triangles) of the last clustering."""
+        check(N.lib().acvd_build_output(self._h, mesh_id, quadrics_level))
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        check(N.lib().acvd_get_output_sizes(self._h, mesh_id, ctypes.byref(nv), ctypes.byref(nf)))
+        xyz = np.empty((nv.value, 3), np.float64)
+        tri = np.empty((nf.value, 3), np.int32)
+        check(N.lib().acvd_get_output(self._h, mesh_id, _p(xyz), _p(tri)))
+        return xyz, tri
 
     def scales(self, mesh_id=0):
         s = np.empty(3, np.float64)
@@ -218,6 +226,7 @@ class _DiscreteRemeshing:
         self._clustering = None
         self._ctx = None
         self._subsampling_threshold = 10
+        self._output = None
         self._metric = _MetricProxy(self)
 
     @classmethod
@@ -298,6 +307,29 @@ class _DiscreteRemeshing:
 
     def GetClustering(self):
         return self._clustering
+
+    def Remesh(self):
+        """vtkDiscreteRemeshing::Remesh (vtkDiscreteRemeshing.h:877-954):
+        ProcessClustering, then the output triangulation.  Subdivision of
+        meshes with fewer than SubsamplingThreshold * K vertices, curvature
+        estimation, ForceManifold and boundary fixing are not part of this path."""
+        n = self.GetNumberOfItems()
+        if n < self._subsampling_threshold * self._params["number_of_clusters"]:
+            raise AcvdError(f"input has {n} vertices, fewer than SubsamplingThreshold * NumberOfClusters; "
+                            "subdivide the input first")
+        if self.ProcessClustering() is None:
+            return
+        xyz, tri = self._ctx.output(0, 0)
+        self._output = Mesh(xyz, tri)
+
+    def GetOutput(self):
+        return self._output
+
+    def QuadricPostProcessing(self, level=1):
+        """Quadric placement of the output vertices (Examples/ACVD.cxx:217-291)."""
+        xyz, tri = self._ctx.output(0, level)
+        self._output = Mesh(xyz, tri)
+        return self._output
 
     def GetClusterSize(self, c):
         return int(self._ctx.cluster_sizes(0)[c])

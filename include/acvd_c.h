@@ -163,13 +163,27 @@ int acvd_get_cluster_points(acvd_ctx* ctx, int32_t mesh_id, double* xyz);
 int acvd_get_energy(acvd_ctx* ctx, int32_t mesh_id, double* energy);
 int acvd_get_stats(acvd_ctx* ctx, int32_t mesh_id, acvd_stats* st);
 
+/* ---- output mesh ---------------------------------------------------------- */
+/* BuildDelaunayTriangulation (vtkDiscreteRemeshing.h:1002-1152, vertex
+ * clustering branch), i.e. what GetOutput() (:61) returns after Remesh():
+ * K vertices at the cluster points (an empty cluster takes the first
+ * non-empty cluster's point), one triangle per input face spanning three
+ * distinct clusters, the first such face kept for each cluster triple.
+ * quadrics_level > 0 then applies ACVD's quadric placement
+ * (Examples/ACVD.cxx:217-291, level 1 by default there) to the vertices.
+ * ForceManifold / boundary fixing (:937-952) are not part of this call. */
+int acvd_build_output(acvd_ctx* ctx, int32_t mesh_id, int32_t quadrics_level);
+int acvd_get_output_sizes(acvd_ctx* ctx, int32_t mesh_id, int64_t* nv, int64_t* nf);
+/* xyz: 3*nv doubles, tri: 3*nf cluster ids (either may be NULL). */
+int acvd_get_output(acvd_ctx* ctx, int32_t mesh_id, double* xyz, int32_t* tri);
+
 /* ---- device inputs of the sweep (for replay and inspection) --------------- */
 /* Number of CSR adjacency entries of the mesh (valid after a build). */
 int acvd_get_adjacency_size(acvd_ctx* ctx, int32_t mesh_id, int64_t* n_adj);
 /* CSR with mesh-local vertex ids: row_ptr[nv+1], col[n_adj], rows sorted. */
 int acvd_get_adjacency(acvd_ctx* ctx, int32_t mesh_id, int32_t* row_ptr, int32_t* col);
-/* Items: xw[4*nv] = (x, y, z, w) in FP32; q[10*nv] quadrics (QEM only, may
- * be NULL).  Build the device inputs without clustering with acvd_build(). */
+/* Items: xw[4*nv] = (x, y, z, w) in FP32; q[10*nv] vertex quadrics (may be
+ * NULL).  Build the device inputs without clustering with acvd_build(). */
 int acvd_get_items(acvd_ctx* ctx, int32_t mesh_id, float* xw, float* q);
 /* Fixed-point scales of the mesh: {weight, weighted position, quadric}. */
 int acvd_get_scales(acvd_ctx* ctx, int32_t mesh_id, double* scales3);

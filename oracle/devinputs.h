@@ -76,7 +76,7 @@ inline void build_device_inputs(int64_t nv, const double* xyz, int64_t nf, const
   const bool qem = metric == 1;
   std::vector<double> raw(nv);
   out.xw.assign(4 * nv, 0.0f);
-  out.q.assign(qem ? 10 * nv : 0, 0.0f);
+  out.q.assign(10 * nv, 0.0f);  // vertex quadrics of every mesh
   for (int64_t v = 0; v < nv; ++v) {
     double area = 0.0;
     double Q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -89,7 +89,7 @@ inline void build_device_inputs(int64_t nv, const double* xyz, int64_t nf, const
       double ec = (p3[0] - p1[0]) * (p3[0] - p1[0]) + (p3[1] - p1[1]) * (p3[1] - p1[1]) + (p3[2] - p1[2]) * (p3[2] - p1[2]);
       double t = 4.0 * ea * ec - (ea - eb + ec) * (ea - eb + ec);
       area += 0.25 * std::sqrt(t < 0 ? -t : t) / 3.0;
-      if (qem) {
+      {
         double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
         double w[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
         double n0 = u[1] * w[2] - u[2] * w[1];
@@ -105,8 +105,7 @@ inline void build_device_inputs(int64_t nv, const double* xyz, int64_t nf, const
     if (indicator && (!qem || gradation > 0.0)) wgt *= std::pow(indicator[v], gradation);
     raw[v] = wgt;
     for (int k = 0; k < 3; ++k) out.xw[4 * v + k] = float(xyz[3 * v + k]);
-    if (qem)
-      for (int k = 0; k < 10; ++k) out.q[10 * v + k] = float(Q[k]);
+    for (int k = 0; k < 10; ++k) out.q[10 * v + k] = float(Q[k]);
   }
   const double avg = device_order_sum(nv, [&](int64_t i) { return raw[i]; }) / double(nv);
   const double ratio = qem ? 10000.0 : 100000.0;
@@ -123,13 +122,11 @@ inline void build_device_inputs(int64_t nv, const double* xyz, int64_t nf, const
                           std::max(std::fabs(double(out.xw[4 * i + 1])), std::fabs(double(out.xw[4 * i + 2]))));
     return double(out.xw[4 * i + 3]) * mxc;
   });
-  double sq = 0.0;
-  if (qem)
-    sq = device_order_sum(nv, [&](int64_t i) {
-      double m = 0;
-      for (int k = 0; k < 10; ++k) m = std::max(m, std::fabs(double(out.q[10 * i + k])));
-      return m;
-    });
+  double sq = device_order_sum(nv, [&](int64_t i) {
+    double m = 0;
+    for (int k = 0; k < 10; ++k) m = std::max(m, std::fabs(double(out.q[10 * i + k])));
+    return m;
+  });
   out.scales[0] = pow2_scale_below(sw);
   out.scales[1] = pow2_scale_below(ss);
   out.scales[2] = pow2_scale_below(sq);

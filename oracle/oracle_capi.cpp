@@ -296,7 +296,9 @@ int orc_hdr_ring_problem(const int32_t* rp, const int32_t* col, const int32_t* l
   return acvd::ring_problem(rp, col, labels, item, cluster) ? 1 : 0;
 }
 
-uint64_t orc_hdr_accept_key(uint32_t seed, int32_t loop, uint32_t item) { return acvd::accept_key(seed, loop, item); }
+uint64_t orc_hdr_accept_key(uint32_t seed, int32_t loop, int round, uint32_t item) {
+  return acvd::accept_key(seed, loop, round, item);
+}
 
 // Hartigan terms of the shared header: out2 = {leave term w.r.t. (ca, Wa),
 // join term w.r.t. (cb, Wb)}.

@@ -62,7 +62,7 @@ def lib():
         L.orc_hdr_rep_point_delta.argtypes = [P, P, ctypes.c_int, P]
         L.orc_hdr_ring_problem.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32]
         L.orc_hdr_accept_key.restype = ctypes.c_uint64
-        L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32]
+        L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int, ctypes.c_uint32]
         L.orc_replay_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
         L.orc_devinputs_create.restype = P
@@ -209,8 +209,8 @@ def hdr_ring_problem(row_ptr, col, labels, item, cluster):
                                       _p(np.ascontiguousarray(labels, np.int32)), int(item), int(cluster))
 
 
-def hdr_accept_key(seed, loop, item):
-    return lib().orc_hdr_accept_key(seed, loop, item)
+def hdr_accept_key(seed, loop, item, round=0):
+    return lib().orc_hdr_accept_key(seed, loop, round, item)
 
 
 def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, unconstrained=0, max_loops=5000000,
@@ -231,7 +231,7 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
 
 
 def device_inputs(xyz, tri, metric, indicator=None, gradation=0.0):
-    """Host build of the sweep inputs: (row_ptr, col, xw[n,4], q[n,10] or None, scales[3])."""
+    """Host build of the sweep inputs: (row_ptr, col, xw[n,4], q[n,10], scales[3])."""
     x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
     t = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
     ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
@@ -241,7 +241,7 @@ def device_inputs(xyz, tri, metric, indicator=None, gradation=0.0):
         rp = np.empty(len(x) + 1, np.int32)
         col = np.empty(ncol, np.int32)
         xw = np.empty((len(x), 4), np.float32)
-        q = np.empty((len(x), 10), np.float32) if metric == QEM else None
+        q = np.empty((len(x), 10), np.float32)
         sc = np.empty(3, np.float64)
         lib().orc_devinputs_get(h, _p(rp), _p(col), _p(xw), _p(q), _p(sc))
     finally:
@@ -274,3 +274,63 @@ def hdr_fix_item(xw4, q10, scales):
 
 def mt19937_10000():
     return int(lib().orc_mt19937_10000())
+
+
+# ---- output mesh (small meshes only: pure Python loops) -------------------
+def output_triangles(tri, labels, K):
+    """Triangles of the output mesh, vertex clustering branch of
+    BuildDelaunayTriangulation (DiscreteRemeshing/vtkDiscreteRemeshing.h:1041,
+    1096-1108): per face in order, the clusters of its vertices with
+    duplicates and the null label K removed in first-occurrence order
+    (GetDualItemNeighbourClusters :974-988); three of them give a triangle
+    unless one on the same three clusters exists already (AddFace :992-1000,
+    vtkSurfaceBase::IsFace Common/vtkSurfaceBase.cxx:723-763 matches the
+    vertex set regardless of orientation)."""
+    seen = set()
+    out = []
+    for f in np.asarray(tri):
+        ids = []
+        for v in f:
+            c = int(labels[int(v)])
+            if 0 <= c < K and c not in ids:
+                ids.append(c)
+        if len(ids) == 4:  # unreachable for triangles
+            raise AssertionError
+        if len(ids) == 3:
+            key = frozenset(ids)
+            if key not in seen:
+                seen.add(key)
+                out.append(ids)
+    return np.asarray(out, np.int32).reshape(-1, 3)
+
+
+def output_points(points, sizes):
+    """Output vertices (vtkDiscreteRemeshing.h:1012-1035): the cluster point,
+    an empty cluster taking the point of the first non-empty cluster."""
+    pts = np.array(points, np.float64, copy=True)
+    nonempty = np.flatnonzero(np.asarray(sizes) > 0)
+    valid = int(nonempty[0]) if len(nonempty) else 0
+    for i in np.flatnonzero(np.asarray(sizes) == 0):
+        pts[i] = pts[valid]
+    return pts
+
+
+def quadric_post_placement(xyz, tri, labels, K, points, level):
+    """ACVD quadric post-processing (DiscreteRemeshing/Examples/ACVD.cxx:237-276):
+    each cluster sums the plane quadrics of the faces around each of its
+    vertices (AddTriangleQuadric, Common/vtkQuadricTools.cxx:56-79), then
+    ComputeRepresentativePoint moves its output vertex."""
+    xyz = np.asarray(xyz, np.float64)
+    tri = np.asarray(tri)
+    fq = np.array([triangle_quadric(xyz[a], xyz[b], xyz[c]) for a, b, c in tri])
+    vq = np.zeros((len(xyz), 10))
+    for j in range(3):
+        np.add.at(vq, tri[:, j], fq)
+    cq = np.zeros((K, 10))
+    lab = np.asarray(labels)
+    ok = (lab >= 0) & (lab < K)
+    np.add.at(cq, lab[ok], vq[ok])
+    out = np.array(points, np.float64, copy=True)
+    for c in range(K):
+        out[c], _ = rep_point(cq[c, :9], out[c], level)
+    return out

@@ -36,7 +36,7 @@ struct Replay {
   const float* xw = nullptr;
   const float* q = nullptr;
 
-  std::vector<int32_t> labels, sizes, stamp, cand_pass, pending;
+  std::vector<int32_t> labels, sizes, stamp, moved, cand_pass, pending;
   std::vector<unsigned long long> acc_ws, acc_q;
   std::vector<acvd::ClusterGeom> geom;
   std::vector<double> T;
@@ -136,41 +136,51 @@ struct Replay {
       if (!ok) continue;
       cands.emplace_back(i, target);
     }
-    // accept keys (min at both clusters) and proposal stamps
-    for (auto& c : cands) {
-      const int64_t i = c.first;
-      const uint64_t key = acvd::accept_key(seed, L, uint32_t(i));
-      const int32_t lab = labels[i];
-      if (lab != acvd::kNull && key < prio[lab]) prio[lab] = key;
-      if (key < prio[c.second]) prio[c.second] = key;
-      cand_pass[i] = L;
-    }
+    // matching rounds: in each round the proposals whose clusters have not
+    // moved in this pass register their round key; winners are applied
     stats.proposed += int64_t(cands.size());
+    for (auto& c : cands) cand_pass[c.first] = L;
+    std::vector<int32_t> origlab(cands.size());
+    std::vector<char> applied(cands.size(), 0);
+    for (size_t t = 0; t < cands.size(); ++t) origlab[t] = labels[cands[t].first];
     int mods = 0;
-    for (auto& c : cands) {
-      const int64_t i = c.first;
-      const int32_t b = c.second;
-      const uint64_t key = acvd::accept_key(seed, L, uint32_t(i));
-      const int32_t lab = labels[i];
-      if (lab != acvd::kNull && prio[lab] != key) continue;
-      if (prio[b] != key) continue;
-      labels[i] = b;
-      const float w = xw[4 * i + 3];
-      const int64_t iw = acvd::fix_w(w, sc);
-      const int64_t is[3] = {acvd::fix_s(w, xw[4 * i], sc), acvd::fix_s(w, xw[4 * i + 1], sc),
-                             acvd::fix_s(w, xw[4 * i + 2], sc)};
-      int64_t iq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      if (metric == 1)
-        for (int k = 0; k < 10; ++k) iq[k] = acvd::fix_q(q[10 * i + k], sc);
-      if (lab != acvd::kNull) {
-        update(lab, -1, iw, is, iq);
-        sizes[lab]--;
-        stamp[lab] = L;
+    for (int round = 0; round < acvd::kMatchRounds; ++round) {
+      for (size_t t = 0; t < cands.size(); ++t) {
+        if (applied[t]) continue;
+        const int32_t a = origlab[t], b = cands[t].second;
+        if ((a != acvd::kNull && moved[a] == L) || moved[b] == L) continue;
+        const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(cands[t].first));
+        if (a != acvd::kNull && key < prio[a]) prio[a] = key;
+        if (key < prio[b]) prio[b] = key;
       }
-      update(b, +1, iw, is, iq);
-      sizes[b]++;
-      stamp[b] = L;
-      mods++;
+      for (size_t t = 0; t < cands.size(); ++t) {
+        if (applied[t]) continue;
+        const int64_t i = cands[t].first;
+        const int32_t lab = origlab[t], b = cands[t].second;
+        const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(i));
+        if (lab != acvd::kNull && prio[lab] != key) continue;
+        if (prio[b] != key) continue;
+        applied[t] = 1;
+        labels[i] = b;
+        const float w = xw[4 * i + 3];
+        const int64_t iw = acvd::fix_w(w, sc);
+        const int64_t is[3] = {acvd::fix_s(w, xw[4 * i], sc), acvd::fix_s(w, xw[4 * i + 1], sc),
+                               acvd::fix_s(w, xw[4 * i + 2], sc)};
+        int64_t iq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (metric == 1)
+          for (int k = 0; k < 10; ++k) iq[k] = acvd::fix_q(q[10 * i + k], sc);
+        if (lab != acvd::kNull) {
+          update(lab, -1, iw, is, iq);
+          sizes[lab]--;
+          stamp[lab] = L;
+          moved[lab] = L;
+        }
+        update(b, +1, iw, is, iq);
+        sizes[b]++;
+        stamp[b] = L;
+        moved[b] = L;
+        mods++;
+      }
     }
     st.pass_mods = mods;
     st.pass_props = int32_t(cands.size());
@@ -230,6 +240,7 @@ struct Replay {
     for (int64_t i = 0; i < N; ++i) labels[i] = (init[i] < 0 || init[i] >= K) ? acvd::kNull : init[i];
     sizes.assign(K, 0);
     stamp.assign(K, 0);
+    moved.assign(K, -2);
     cand_pass.assign(N, -2);
     pending.assign(N, acvd::kNull);
     acc_ws.assign(4 * size_t(K), 0ull);
