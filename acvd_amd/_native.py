@@ -68,7 +68,8 @@ EXPORTS = [
     "acvd_process_clustering", "acvd_get_labels", "acvd_get_cluster_sizes", "acvd_get_cluster_points",
     "acvd_get_energy", "acvd_get_stats", "acvd_get_adjacency_size", "acvd_get_adjacency", "acvd_get_items",
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
-    "acvd_get_output_sizes", "acvd_get_output",
+    "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
+    "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward",
 ]
 
 _lib = None
@@ -115,6 +116,12 @@ def lib():
     L.acvd_get_adjacency_size.argtypes = [P, i32, ctypes.POINTER(i64)]
     L.acvd_get_adjacency.argtypes = [P, i32, P, P]
     L.acvd_get_items.argtypes = [P, i32, P, P]
+    L.acvd_energy_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+    L.acvd_energy_destroy.argtypes = [P]
+    L.acvd_energy_add_mesh.argtypes = [P, i64, i64, P, i32, i32, i32, ctypes.POINTER(i32)]
+    L.acvd_energy_num_items.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i32)]
+    L.acvd_energy_forward.argtypes = [P, P, P, P, P, P]
+    L.acvd_energy_backward.argtypes = [P, P, P, P, P, P, P, P]
     _lib = L
     return L
 

@@ -19,6 +19,13 @@ constexpr int kBlock = 256;
 // Work-list slices: blocks append to counter blockIdx % kSlices, which keeps
 // same-address atomics (slow at device scope) to a few per slice per pass.
 constexpr int kSlices = 64;
+// Scan tiles: groups of kTileItems items of one mesh in spatial (Morton)
+// order.  Each tile keeps a superset of the clusters its items and their
+// neighbours belong to; a tile none of whose clusters changed in the last
+// pass (and none of whose proposals lost) has no item to evaluate and is
+// skipped after reading that set.
+constexpr int kTileItems = 256;
+constexpr int kTileSet = 64;  // set capacity; a larger set marks the tile always scanned
 
 struct DevMesh {
   int64_t item_off;
@@ -29,7 +36,7 @@ struct DevMesh {
   int32_t level;   // QEM quadrics optimisation level
   uint32_t seed;
   int32_t has_indicator;
-  int32_t pad;
+  int32_t tile_off;  // first scan tile of the mesh
   double gradation;
   double clamp_ratio;
   Scales scales;
@@ -42,7 +49,7 @@ struct Ctrl {
   int32_t fill_changed;
   int32_t pad;
   int32_t work_count[kSlices];  // items to evaluate in this pass, per slice
-  int32_t props[kSlices];       // proposals of this pass (single-mesh contexts)
+  int32_t cand_count[kSlices];  // proposals of this pass, per slice
   int32_t mods[kSlices];        // applied moves of this pass (single-mesh contexts)
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
@@ -55,7 +62,7 @@ struct SweepArgs {
   int64_t n_items;
   int64_t n_clusters;
   int32_t n_meshes;
-  int32_t pad;
+  int32_t n_tiles;
   const DevMesh* meshes;
   MeshState* state;
   const int32_t* item_mesh;  // null when n_meshes == 1
@@ -71,14 +78,21 @@ struct SweepArgs {
   double* T;                   // W |q - c|^2 per cluster (QEM), may be null
   int32_t* sizes;
   int32_t* stamp;              // mesh-local loop of the last modification
-  int32_t* moved;              // mesh-local loop of the last applied move
   unsigned long long* prio;    // accept keys per cluster
   int32_t* cand_pass;          // per item: loop of its last proposal
-  int32_t* cand_target;        // per item: target cluster of its proposal
+  int32_t* cand_target;        // per item: target of its live proposal (< 0: none)
   int32_t* work;               // items to evaluate in this pass (kSlices slices)
   int32_t slice_cap;           // capacity of one work-list slice
-  int32_t pad2;
+  int32_t cand_cap;            // capacity of one candidate-list slice
   Ctrl* ctrl;
+  int32_t* cand;               // proposals of this pass (kSlices slices)
+  // scan tiles
+  const int32_t* tile_items;   // n_tiles * kTileItems item ids, -1 padding
+  const int32_t* item_tile;    // tile of each item
+  const int32_t* tile_mesh;    // mesh of each tile
+  int32_t* tile_cnt;           // entries in the tile's cluster set (> kTileSet: overflow)
+  int32_t* tile_set;           // n_tiles * kTileSet cluster ids
+  int32_t* tile_prop;          // loop of the last proposal made in the tile
 };
 
 // ---- launchers (defined in the .hip files) --------------------------------
@@ -105,8 +119,12 @@ hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* m
                              unsigned long long* mesh_count);
 hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int set_stamp);
 hipError_t launch_scan(hipStream_t s, const SweepArgs& a);
+hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask);
+hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbox, int32_t* tile_items,
+                             int32_t* item_tile, void** tmp, size_t* tmp_bytes);
 hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid);
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid);
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round);
+hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid, int round);
 hipError_t launch_controller(hipStream_t s, const SweepArgs& a);
 hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* pending);
 hipError_t launch_cc(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* parent,

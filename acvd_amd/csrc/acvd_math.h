@@ -258,27 +258,29 @@ ACVD_HD uint64_t prio_key(uint32_t seed, uint32_t pass, uint32_t item) {
   return (uint64_t(prio_hash(seed, pass, item)) << 32) | uint64_t(item);
 }
 
-// Accept step.  A pass runs kMatchRounds matching rounds over its proposals:
-// in round r, every proposal whose two clusters have not moved yet in this
-// pass registers its key at both clusters (atomicMin), and a proposal holding
-// the minimum at both is applied.  Clusters touched by a round are skipped by
-// the later ones, so a cluster takes part in at most one move per pass and
-// every applied dE is exact with respect to the state at the pass start.
-constexpr int kMatchRounds = 3;
+// Accept step.  A pass runs kMatchRounds rounds over its proposals.  Round 0
+// takes the proposals of the pass's evaluation (state frozen at the pass
+// start); round r > 0 re-evaluates the proposals not applied yet against the
+// state left by round r-1.  In every round each live proposal registers its
+// key at both of its clusters (atomicMin) and a proposal holding the minimum
+// at both is applied.  The moves of one round touch disjoint clusters, so
+// every applied dE is exact with respect to the state at the start of its
+// round and the energy decreases monotonically.
+constexpr int kMatchRounds = 6;
 
 // Accept key:
-//   [63:40] 0xFFFFFF - ((loop << 2 | round) & 0xFFFFFF): newer rounds give
+//   [63:40] 0xFFFFFF - (loop * kMatchRounds + round): later rounds give
 //           smaller keys, so a slot never needs resetting
 //   [39:27] seeded hash of (loop, item)   random priority
 //   [26:0]  mesh-local item index         unique tie-break
 ACVD_HD uint64_t accept_key(uint32_t seed, int32_t loop, int round, uint32_t local_item) {
-  const uint32_t pr = (uint32_t(loop) << 2) | uint32_t(round);
+  const uint32_t pr = uint32_t(loop) * uint32_t(kMatchRounds) + uint32_t(round);
   uint64_t tag = uint64_t(0xFFFFFFu - (pr & 0xFFFFFFu));
   uint64_t h = uint64_t(prio_hash(seed, uint32_t(loop), local_item) & 0x1FFFu);
   return (tag << 40) | (h << 27) | uint64_t(local_item & 0x7FFFFFFu);
 }
 constexpr int64_t kMaxItemsPerMesh = int64_t(1) << 27;
-constexpr int32_t kMaxLoops = (1 << 22) - 2;
+constexpr int32_t kMaxLoops = int32_t(0xFFFFFF / kMatchRounds) - 2;
 
 // Isotropic acceptance: dE = w (tb - ta) < -eps * w (ta + tb).
 ACVD_HD bool iso_accept(float ta, float tb) { return (tb - ta) < -kAcceptEps * (ta + tb); }

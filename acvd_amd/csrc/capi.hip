@@ -97,13 +97,16 @@ struct acvd_ctx {
   DBuf<int32_t> d_row_ptr, d_col, d_tri;
   DBuf<float4> d_xw;
   DBuf<float> d_q;
-  DBuf<int32_t> d_labels, d_sizes, d_stamp, d_moved, d_cand_pass, d_pending;
+  DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T;
   DBuf<int32_t> d_cand_target;
-  DBuf<int32_t> d_work;
-  int32_t slice_cap = 0;
+  DBuf<int32_t> d_work, d_cand;
+  int32_t slice_cap = 0, cand_cap = 0;
+  // scan tiles
+  int32_t n_tiles = 0;
+  DBuf<int32_t> d_tile_items, d_item_tile, d_tile_mesh, d_tile_cnt, d_tile_set, d_tile_prop;
   DBuf<int32_t> d_mask_all, d_mask, d_ndisc;
   DBuf<unsigned long long> d_mesh_count;
   // clean scratch
@@ -138,6 +141,7 @@ struct acvd_ctx {
     a.n_items = N;
     a.n_clusters = K;
     a.n_meshes = int32_t(meshes.size());
+    a.n_tiles = n_tiles;
     a.meshes = d_meshes.p;
     a.state = d_state.p;
     a.item_mesh = meshes.size() > 1 ? d_item_mesh.p : nullptr;
@@ -153,13 +157,20 @@ struct acvd_ctx {
     a.T = any_qem ? d_T.p : nullptr;
     a.sizes = d_sizes.p;
     a.stamp = d_stamp.p;
-    a.moved = d_moved.p;
     a.prio = d_prio.p;
     a.cand_pass = d_cand_pass.p;
     a.cand_target = d_cand_target.p;
     a.work = d_work.p;
     a.slice_cap = slice_cap;
+    a.cand_cap = cand_cap;
+    a.cand = d_cand.p;
     a.ctrl = d_ctrl.p;
+    a.tile_items = d_tile_items.p;
+    a.item_tile = d_item_tile.p;
+    a.tile_mesh = d_tile_mesh.p;
+    a.tile_cnt = d_tile_cnt.p;
+    a.tile_set = d_tile_set.p;
+    a.tile_prop = d_tile_prop.p;
     return a;
   }
 
@@ -208,6 +219,9 @@ int add_mesh(acvd_ctx* c, int64_t nv, const P* xyz, int64_t nf, const int32_t* t
   return ACVD_OK;
 }
 
+// eval grid (blocks of kBlock threads, grid-stride over the work list)
+int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 2048)); }
+
 // ------------------------------------------------------------------- build
 int build(acvd_ctx* c) {
   if (c->built) return ACVD_OK;
@@ -231,6 +245,11 @@ int build(acvd_ctx* c) {
   }
   if (N >= (int64_t(1) << 31) - 1 || K >= (int64_t(1) << 31) - 1 || 3 * F >= (int64_t(1) << 31))
     return fail(ACVD_ERR_LIMIT, "context exceeds 2^31 items, clusters or face corners");
+  if (M >= (1 << 16)) return fail(ACVD_ERR_LIMIT, "context holds more than 65535 meshes");
+  int64_t n_tiles = 0;
+  for (auto& h : c->meshes) n_tiles += (h.nv + kTileItems - 1) / kTileItems;
+  if (n_tiles * kTileItems >= (int64_t(1) << 31)) return fail(ACVD_ERR_LIMIT, "context exceeds 2^31 tile slots");
+  c->n_tiles = int32_t(n_tiles);
   c->N = N;
   c->K = K;
   c->F = F;
@@ -248,8 +267,10 @@ int build(acvd_ctx* c) {
   }
   std::vector<double> pos(3 * N), ind(N, 0.0);
   std::vector<int32_t> tri(3 * F);
+  std::vector<int32_t> tile_mesh(c->n_tiles);
+  std::vector<float> bbox(6 * M);
   bool any_ind = false;
-  int64_t foff = 0;
+  int64_t foff = 0, toff = 0;
   for (int m = 0; m < M; ++m) {
     HostMesh& h = c->meshes[m];
     DevMesh& d = c->h_meshes[m];
@@ -263,6 +284,22 @@ int build(acvd_ctx* c) {
     d.has_indicator = h.indicator.empty() ? 0 : 1;
     d.gradation = h.params.gradation;
     d.clamp_ratio = d.metric == 1 ? 10000.0 : 100000.0;
+    d.tile_off = int32_t(toff);
+    const int64_t nt = (h.nv + kTileItems - 1) / kTileItems;
+    std::fill(tile_mesh.begin() + toff, tile_mesh.begin() + toff + nt, m);
+    toff += nt;
+    {
+      double lo[3] = {h.xyz[0], h.xyz[1], h.xyz[2]}, hi[3] = {lo[0], lo[1], lo[2]};
+      for (int64_t v = 0; v < h.nv; ++v)
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], h.xyz[3 * v + k]);
+          hi[k] = std::max(hi[k], h.xyz[3 * v + k]);
+        }
+      for (int k = 0; k < 3; ++k) {
+        bbox[6 * m + k] = float(lo[k]);
+        bbox[6 * m + 3 + k] = hi[k] > lo[k] ? float(65535.0 / (hi[k] - lo[k])) : 0.0f;
+      }
+    }
     std::copy(h.xyz.begin(), h.xyz.end(), pos.begin() + 3 * h.item_off);
     if (!h.indicator.empty()) {
       any_ind = true;
@@ -344,6 +381,22 @@ int build(acvd_ctx* c) {
     HIP_TRY(launch_item_build(s, a, d_pos.p, c->d_tri.p, vfptr.p, vfcol.p, any_ind ? d_ind.p : nullptr, d_raw.p,
                               c->d_xw.p, c->d_q.p));
   }
+  // ---- scan tiles in spatial order
+  {
+    const int64_t nt = c->n_tiles;
+    HIP_TRY(c->d_tile_items.alloc(nt * kTileItems));
+    HIP_TRY(c->d_item_tile.alloc(N));
+    HIP_TRY(c->d_tile_mesh.alloc(nt));
+    HIP_TRY(c->d_tile_cnt.alloc(nt));
+    HIP_TRY(c->d_tile_set.alloc(nt * kTileSet));
+    HIP_TRY(c->d_tile_prop.alloc(nt));
+    HIP_TRY(hipMemcpyAsync(c->d_tile_mesh.p, tile_mesh.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    DBuf<float> d_bbox;
+    HIP_TRY(d_bbox.alloc(6 * M));
+    HIP_TRY(hipMemcpyAsync(d_bbox.p, bbox.data(), 6 * M * sizeof(float), hipMemcpyHostToDevice, s));
+    SweepArgs a = c->args();
+    HIP_TRY(launch_tile_order(s, a, d_bbox.p, c->d_tile_items.p, c->d_item_tile.p, &c->cub_tmp, &c->cub_bytes));
+  }
   // ---- deterministic reductions: clamp then scales
   {
     std::vector<int2> tab;
@@ -367,7 +420,6 @@ int build(acvd_ctx* c) {
   HIP_TRY(c->d_labels.alloc(N));
   HIP_TRY(c->d_sizes.alloc(K));
   HIP_TRY(c->d_stamp.alloc(K));
-  HIP_TRY(c->d_moved.alloc(K));
   HIP_TRY(c->d_cand_pass.alloc(N));
   HIP_TRY(c->d_pending.alloc(N));
   HIP_TRY(c->d_acc_ws.alloc(4 * K));
@@ -379,10 +431,16 @@ int build(acvd_ctx* c) {
   HIP_TRY(c->d_geom.alloc(K));
   HIP_TRY(c->d_cand_target.alloc(N));
   {
-    // blocks of the scan grid are dealt round-robin to the slices
-    const int64_t blocks = (N + kBlock - 1) / kBlock;
-    c->slice_cap = int32_t(((blocks + kSlices - 1) / kSlices) * kBlock);
+    // scan tiles are dealt round-robin to the slices
+    const int64_t blocks = c->n_tiles;
+    c->slice_cap = int32_t(((blocks + kSlices - 1) / kSlices) * kTileItems);
     HIP_TRY(c->d_work.alloc(size_t(c->slice_cap) * kSlices));
+    // evaluate blocks are dealt round-robin to the candidate slices; a block
+    // appends at most kBlock proposals per grid-stride step
+    const int64_t grid = sweep_grid_for(N);
+    const int64_t steps = (N + grid * kBlock - 1) / (grid * kBlock);
+    c->cand_cap = int32_t(((grid + kSlices - 1) / kSlices) * steps * kBlock);
+    HIP_TRY(c->d_cand.alloc(size_t(c->cand_cap) * kSlices));
   }
   HIP_TRY(c->d_parent.alloc(N));
   HIP_TRY(c->d_comp_size.alloc(N));
@@ -413,9 +471,10 @@ int build(acvd_ctx* c) {
 }
 
 // ------------------------------------------------------------ minimisation
-int sweep_grid(const acvd_ctx* c) {
-  return int(std::max<int64_t>(1, std::min<int64_t>((c->N + kBlock - 1) / kBlock, 2048)));
-}
+int sweep_grid(const acvd_ctx* c) { return int(sweep_grid_for(c->N)); }
+
+// grid of the accept rounds (grid-stride over the candidate list)
+int round_grid(const acvd_ctx* c) { return std::min(sweep_grid(c), 1024); }
 
 int fill_until_stable(acvd_ctx* c, const int32_t* mask, int max_rounds) {
   hipStream_t s = c->stream;
@@ -444,10 +503,15 @@ int enqueue_chunk(acvd_ctx* c) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   const int grid = sweep_grid(c);
+  const int rgrid = round_grid(c);
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
-    HIP_TRY(launch_apply(s, a, grid));
+    HIP_TRY(launch_apply(s, a, rgrid, 0));
+    for (int r = 1; r < kMatchRounds; ++r) {
+      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, rgrid, r));
+      HIP_TRY(launch_apply(s, a, rgrid, r));
+    }
     HIP_TRY(launch_controller(s, a));
   }
   return ACVD_OK;
@@ -466,7 +530,11 @@ int run_chunk_timed(acvd_ctx* c) {
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
-    HIP_TRY(launch_apply(s, a, grid));
+    HIP_TRY(launch_apply(s, a, round_grid(c), 0));
+    for (int r = 1; r < kMatchRounds; ++r) {
+      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, round_grid(c), r));
+      HIP_TRY(launch_apply(s, a, round_grid(c), r));
+    }
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
     HIP_TRY(launch_controller(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 4], s));
@@ -477,7 +545,7 @@ int run_chunk_timed(acvd_ctx* c) {
       float ms = 0;
       HIP_TRY(hipEventElapsedTime(&ms, ev[5 * p + k], ev[5 * p + k + 1]));
       c->kernel_ms[k] += ms;
-      c->kernel_launches[k] += (k == 1 && c->any_iso && c->any_qem) ? 2 : 1;
+      c->kernel_launches[k] += (k == 1 && c->any_iso && c->any_qem) ? 2 : 1;  // [2]: one accept-round group
     }
   for (auto& e : ev) (void)hipEventDestroy(e);
   return ACVD_OK;
@@ -533,7 +601,9 @@ int minimize_energy(acvd_ctx* c) {
   for (int m = 0; m < M; ++m) c->h_state[m].early_stop_items = int64_t(cnt[m]);
   HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill_i32(s, c->d_cand_pass.p, c->N, -2));
-  HIP_TRY(launch_fill_i32(s, c->d_moved.p, c->K, -2));
+  HIP_TRY(launch_fill_i32(s, c->d_tile_prop.p, c->n_tiles, -2));
+  HIP_TRY(launch_tile_sets(s, a, nullptr));
+  HIP_TRY(launch_fill_i32(s, c->d_cand_target.p, c->N, -3));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
   for (int k = 0; k < 4; ++k) {
@@ -571,6 +641,7 @@ int minimize_energy(acvd_ctx* c) {
       // with every cluster marked modified
       rc = recompute_statistics(c, c->d_mask.p, 1, nullptr);
       if (rc) return rc;
+      HIP_TRY(launch_tile_sets(s, a, c->d_mask.p));
       HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
       HIP_TRY(hipStreamSynchronize(s));
       bool any = false;
@@ -971,6 +1042,196 @@ int acvd_get_scales(acvd_ctx* ctx, int32_t mesh_id, double* scales3) {
   scales3[0] = s.sw;
   scales3[1] = s.ss;
   scales3[2] = s.sq;
+  return ACVD_OK;
+}
+
+}  // extern "C"
+
+// ====================================================== fixed-assignment energy
+#include "acvd_energy.h"
+
+struct acvd_energy {
+  int device = 0;
+  struct Mesh {
+    int64_t nv, nf;
+    std::vector<int32_t> tri;
+    int32_t K, metric, level;
+  };
+  std::vector<Mesh> meshes;
+  bool built = false;
+  int64_t N = 0, K = 0, F = 0;
+  DBuf<EnergyMesh> d_meshes;
+  DBuf<int32_t> d_item_mesh, d_clus_mesh, d_tri, d_vfptr, d_vfcol;
+  DBuf<double> d_acc, d_pts, d_adj;
+  bool have_forward = false;
+  EnergyArgs args() const {
+    EnergyArgs a{};
+    a.n_items = N;
+    a.n_clusters = K;
+    a.meshes = d_meshes.p;
+    a.item_mesh = meshes.size() > 1 ? d_item_mesh.p : nullptr;
+    a.clus_mesh = meshes.size() > 1 ? d_clus_mesh.p : nullptr;
+    a.tri = d_tri.p;
+    a.vfptr = d_vfptr.p;
+    a.vfcol = d_vfcol.p;
+    a.acc = d_acc.p;
+    a.pts = d_pts.p;
+    a.adj = d_adj.p;
+    return a;
+  }
+};
+
+namespace {
+
+int energy_build(acvd_energy* e) {
+  if (e->built) return ACVD_OK;
+  if (e->meshes.empty()) return fail(ACVD_ERR_STATE, "no mesh added");
+  HIP_TRY(hipSetDevice(e->device));
+  const int M = int(e->meshes.size());
+  int64_t N = 0, K = 0, F = 0;
+  for (auto& m : e->meshes) {
+    N += m.nv;
+    K += m.K;
+    F += m.nf;
+  }
+  if (N >= (int64_t(1) << 31) - 1 || K >= (int64_t(1) << 31) - 1 || 3 * F >= (int64_t(1) << 31))
+    return fail(ACVD_ERR_LIMIT, "energy context exceeds 2^31 items, clusters or face corners");
+  std::vector<EnergyMesh> hm(M);
+  std::vector<int32_t> tri(3 * F), item_mesh, clus_mesh, vfptr(N + 1, 0), vfcol(3 * F);
+  if (M > 1) {
+    item_mesh.resize(N);
+    clus_mesh.resize(K);
+  }
+  int64_t io = 0, co = 0, fo = 0;
+  for (int m = 0; m < M; ++m) {
+    const auto& h = e->meshes[m];
+    hm[m] = EnergyMesh{io, h.nv, co, fo, h.K, h.metric, h.level, 0};
+    for (int64_t k = 0; k < 3 * h.nf; ++k) tri[3 * fo + k] = int32_t(h.tri[k] + io);
+    // vertex -> faces (mesh-local face ids), faces by increasing id
+    for (int64_t k = 0; k < 3 * h.nf; ++k) vfptr[io + h.tri[k] + 1]++;
+    if (M > 1) {
+      std::fill(item_mesh.begin() + io, item_mesh.begin() + io + h.nv, m);
+      std::fill(clus_mesh.begin() + co, clus_mesh.begin() + co + h.K, m);
+    }
+    io += h.nv;
+    co += h.K;
+    fo += h.nf;
+  }
+  for (int64_t i = 0; i < N; ++i) vfptr[i + 1] += vfptr[i];
+  {
+    std::vector<int32_t> fill(vfptr.begin(), vfptr.end() - 1);
+    for (int m = 0; m < M; ++m) {
+      const auto& h = e->meshes[m];
+      for (int64_t f = 0; f < h.nf; ++f)
+        for (int j = 0; j < 3; ++j) vfcol[fill[hm[m].item_off + h.tri[3 * f + j]]++] = int32_t(f);
+    }
+  }
+  HIP_TRY(e->d_meshes.alloc(M));
+  HIP_TRY(hipMemcpy(e->d_meshes.p, hm.data(), M * sizeof(EnergyMesh), hipMemcpyHostToDevice));
+  if (M > 1) {
+    HIP_TRY(e->d_item_mesh.alloc(N));
+    HIP_TRY(e->d_clus_mesh.alloc(K));
+    HIP_TRY(hipMemcpy(e->d_item_mesh.p, item_mesh.data(), N * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_clus_mesh.p, clus_mesh.data(), K * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(e->d_tri.alloc(3 * F));
+  HIP_TRY(hipMemcpy(e->d_tri.p, tri.data(), 3 * F * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(e->d_vfptr.alloc(N + 1));
+  HIP_TRY(hipMemcpy(e->d_vfptr.p, vfptr.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(e->d_vfcol.alloc(3 * F));
+  HIP_TRY(hipMemcpy(e->d_vfcol.p, vfcol.data(), 3 * F * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(e->d_acc.alloc(13 * K));
+  HIP_TRY(e->d_pts.alloc(6 * K));
+  HIP_TRY(e->d_adj.alloc(12 * K));
+  e->N = N;
+  e->K = K;
+  e->F = F;
+  e->built = true;
+  return ACVD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int acvd_energy_create(acvd_energy** out, int device) {
+  if (!out) return fail(ACVD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(ACVD_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(ACVD_ERR_ARG, "device index out of range");
+  auto* e = new (std::nothrow) acvd_energy();
+  if (!e) return fail(ACVD_ERR_ALLOC, "out of host memory");
+  e->device = device;
+  *out = e;
+  return ACVD_OK;
+}
+
+int acvd_energy_destroy(acvd_energy* e) {
+  if (!e) return fail(ACVD_ERR_ARG, "null energy context");
+  (void)hipSetDevice(e->device);
+  delete e;
+  return ACVD_OK;
+}
+
+int acvd_energy_add_mesh(acvd_energy* e, int64_t nv, int64_t nf, const int32_t* tri, int32_t n_clusters,
+                         int32_t metric, int32_t level, int32_t* mesh_id) {
+  if (!e) return fail(ACVD_ERR_ARG, "null energy context");
+  if (nv <= 0 || nf <= 0 || !tri) return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  if (n_clusters <= 0) return fail(ACVD_ERR_ARG, "number of clusters must be positive");
+  if (metric != ACVD_METRIC_ISOTROPIC && metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "unknown metric");
+  if (level < 0 || level > 3) return fail(ACVD_ERR_ARG, "quadrics level must be 0..3");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv) return fail(ACVD_ERR_ARG, "triangle references a missing vertex");
+  acvd_energy::Mesh m;
+  m.nv = nv;
+  m.nf = nf;
+  m.tri.assign(tri, tri + 3 * nf);
+  m.K = n_clusters;
+  m.metric = metric;
+  m.level = level;
+  e->meshes.push_back(std::move(m));
+  e->built = false;
+  e->have_forward = false;
+  if (mesh_id) *mesh_id = int32_t(e->meshes.size() - 1);
+  return ACVD_OK;
+}
+
+int acvd_energy_num_items(const acvd_energy* e, int64_t* n_items, int32_t* n_meshes) {
+  if (!e) return fail(ACVD_ERR_ARG, "null energy context");
+  int64_t n = 0;
+  for (auto& m : e->meshes) n += m.nv;
+  if (n_items) *n_items = n;
+  if (n_meshes) *n_meshes = int32_t(e->meshes.size());
+  return ACVD_OK;
+}
+
+int acvd_energy_forward(acvd_energy* e, const float* xyz, const float* w, const int32_t* labels, double* energy,
+                        void* stream) {
+  if (!e) return fail(ACVD_ERR_ARG, "null energy context");
+  if (!xyz || !w || !labels || !energy) return fail(ACVD_ERR_ARG, "null device buffer");
+  if (int rc = energy_build(e)) return rc;
+  HIP_TRY(hipSetDevice(e->device));
+  EnergyArgs a = e->args();
+  a.xyz = xyz;
+  a.w = w;
+  a.labels = labels;
+  HIP_TRY(energy_forward(static_cast<hipStream_t>(stream), a, energy, int(e->meshes.size())));
+  e->have_forward = true;
+  return ACVD_OK;
+}
+
+int acvd_energy_backward(acvd_energy* e, const float* xyz, const float* w, const int32_t* labels,
+                         const double* grad_energy, float* dxyz, float* dw, void* stream) {
+  if (!e) return fail(ACVD_ERR_ARG, "null energy context");
+  if (!e->have_forward) return fail(ACVD_ERR_STATE, "backward needs a forward on the same inputs first");
+  if (!xyz || !w || !labels || !grad_energy || !dxyz || !dw) return fail(ACVD_ERR_ARG, "null device buffer");
+  HIP_TRY(hipSetDevice(e->device));
+  EnergyArgs a = e->args();
+  a.xyz = xyz;
+  a.w = w;
+  a.labels = labels;
+  HIP_TRY(energy_backward(static_cast<hipStream_t>(stream), a, grad_energy, dxyz, dw));
   return ACVD_OK;
 }
 

@@ -313,4 +313,97 @@ hipError_t launch_clamp_and_scales(hipStream_t s, const SweepArgs& a, int n_bloc
   return hipGetLastError();
 }
 
+
+namespace {
+
+// 16 bits -> every third bit of 48
+__device__ inline uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0xFFFFu;
+  x = (x | (x << 32)) & 0x1F00000000FFFFull;
+  x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+// key = mesh << 48 | Morton code of the position quantised to 16 bits per
+// axis inside the mesh's bounding box (bbox: lo[3], scale[3] per mesh)
+__global__ void k_morton_keys(SweepArgs a, const float* __restrict__ bbox, unsigned long long* keys, int32_t* vals) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.n_items) return;
+  const int m = a.item_mesh ? a.item_mesh[i] : 0;
+  const float4 p = a.xw[i];
+  const float* bb = bbox + 6 * m;
+  const float c[3] = {p.x, p.y, p.z};
+  uint64_t key = uint64_t(m) << 48;
+  for (int k = 0; k < 3; ++k) {
+    float u = (c[k] - bb[k]) * bb[3 + k];
+    u = fminf(fmaxf(u, 0.0f), 65535.0f);
+    key |= spread3(uint32_t(u)) << k;
+  }
+  keys[i] = key;
+  vals[i] = int32_t(i);
+}
+
+// sorted position p -> tile slot (tiles of one mesh are contiguous)
+__global__ void k_tile_assign(SweepArgs a, const int32_t* __restrict__ sorted, int32_t* tile_items,
+                              int32_t* item_tile) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= a.n_items) return;
+  const int32_t i = sorted[p];
+  const int m = a.item_mesh ? a.item_mesh[i] : 0;
+  const DevMesh& dm = a.meshes[m];
+  const int64_t local = p - dm.item_off;  // meshes sort in order, so p lies in the mesh's range
+  const int64_t t = dm.tile_off + local / kTileItems;
+  tile_items[t * kTileItems + local % kTileItems] = i;
+  item_tile[i] = int32_t(t);
+}
+
+__global__ void k_fill_tiles(int64_t n, int32_t* p) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = -1;
+}
+
+}  // namespace
+
+hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbox, int32_t* tile_items,
+                             int32_t* item_tile, void** tmp, size_t* tmp_bytes) {
+  const int64_t n = a.n_items;
+  unsigned long long *k0 = nullptr, *k1 = nullptr;
+  int32_t *v0 = nullptr, *v1 = nullptr;
+  hipError_t e = hipSuccess;
+  auto cleanup = [&]() {
+    (void)hipFree(k0); (void)hipFree(k1); (void)hipFree(v0); (void)hipFree(v1);
+  };
+#define ACVD_TILE_TRY(x) do { e = (x); if (e != hipSuccess) { cleanup(); return e; } } while (0)
+  ACVD_TILE_TRY(hipMalloc(&k0, n * sizeof(unsigned long long)));
+  ACVD_TILE_TRY(hipMalloc(&k1, n * sizeof(unsigned long long)));
+  ACVD_TILE_TRY(hipMalloc(&v0, n * sizeof(int32_t)));
+  ACVD_TILE_TRY(hipMalloc(&v1, n * sizeof(int32_t)));
+  const int64_t nt = int64_t(a.n_tiles) * kTileItems;
+  hipLaunchKernelGGL(k_fill_tiles, dim3(grid_for(nt)), dim3(kBlock), 0, s, nt, tile_items);
+  ACVD_TILE_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_morton_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, a, bbox, k0, v0);
+  ACVD_TILE_TRY(hipGetLastError());
+  size_t need = 0;
+  ACVD_TILE_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, need, k0, k1, v0, v1, int(n), 0, 64, s));
+  if (need > *tmp_bytes) {
+    ACVD_TILE_TRY(hipStreamSynchronize(s));
+    if (*tmp) (void)hipFree(*tmp);
+    *tmp = nullptr;
+    *tmp_bytes = 0;
+    ACVD_TILE_TRY(hipMalloc(tmp, need));
+    *tmp_bytes = need;
+  }
+  size_t have = *tmp_bytes;
+  ACVD_TILE_TRY(hipcub::DeviceRadixSort::SortPairs(*tmp, have, k0, k1, v0, v1, int(n), 0, 64, s));
+  hipLaunchKernelGGL(k_tile_assign, dim3(grid_for(n)), dim3(kBlock), 0, s, a, v1, tile_items, item_tile);
+  ACVD_TILE_TRY(hipGetLastError());
+  ACVD_TILE_TRY(hipStreamSynchronize(s));
+  cleanup();
+#undef ACVD_TILE_TRY
+  return hipSuccess;
+}
+
 }  // namespace acvd

@@ -110,32 +110,6 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   return v;
 }
 
-// Block-wide append: every thread of the block must call it.  Returns the
-// slot of a thread with want == true (one atomic per block on `counter`).
-__device__ inline int32_t block_append(int32_t* counter, bool want) {
-  __shared__ int32_t s_wave[kBlock / 64];
-  __shared__ int32_t s_base;
-  const unsigned long long mask = __ballot(want);
-  const int lane = __lane_id();
-  const int wid = threadIdx.x / 64;
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  if (lane == 0) s_wave[wid] = __popcll(mask);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int32_t tot = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-      const int32_t c = s_wave[w];
-      s_wave[w] = tot;
-      tot += c;
-    }
-    s_base = tot ? atomicAdd(counter, tot) : 0;
-  }
-  __syncthreads();
-  const int32_t slot = s_base + s_wave[wid] + rank;
-  __syncthreads();  // s_wave / s_base are reused by the next call
-  return want ? slot : -1;
-}
-
 // needs_eval of acvd_math.h with the neighbour loads issued together (up to 8
 // neighbours in registers) so that the label and stamp gathers overlap.
 __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t lab, int32_t L) {
@@ -167,58 +141,196 @@ __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t la
 }
 
 // Scan: items of sweeping meshes that must be evaluated this pass are
-// compacted into the work list.
-__global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool want = false;
-  if (i < a.n_items) {
-    const int m = mesh_of_item(a, i);
-    const MeshState& st = a.state[m];
-    if (st.phase == kPhaseSweep) want = needs_eval_fast(a, i, a.labels[i], st.loops);
+// compacted into the work list.  One wavefront per tile: the lanes first test
+// the tile's cluster set against the cluster stamps (and the tile's last
+// proposal loop); only an active tile reads its items and CSR rows.  The test
+// is conservative: every item needs_eval would select lies in an active tile.
+__global__ __launch_bounds__(64) void k_scan(SweepArgs a) {
+  const int32_t t = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int m = a.item_mesh ? a.tile_mesh[t] : 0;
+  const MeshState& st = a.state[m];
+  if (st.phase != kPhaseSweep) return;
+  const int32_t L = st.loops;
+  const int32_t cnt = a.tile_cnt[t];
+  bool active = cnt > kTileSet || a.tile_prop[t] == L - 1;
+  if (!active) {  // wave-uniform branch
+    bool hit = false;
+    if (lane < cnt) {
+      const int32_t c = a.tile_set[int64_t(t) * kTileSet + lane];
+      hit = a.stamp[c] + 1 >= L;
+    }
+    active = __ballot(hit) != 0ull;
   }
-  const int slice = blockIdx.x % kSlices;
-  const int32_t slot = block_append(&a.ctrl->work_count[slice], want);
-  if (want) a.work[int64_t(slice) * a.slice_cap + slot] = int32_t(i);
+  if (!active) return;
+  constexpr int R = kTileItems / 64;
+  int32_t it[R];
+  bool want[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    it[r] = a.tile_items[int64_t(t) * kTileItems + r * 64 + lane];
+    want[r] = it[r] >= 0 && needs_eval_fast(a, it[r], a.labels[it[r]], L);
+  }
+  unsigned long long bal[R];
+  int32_t tot = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    bal[r] = __ballot(want[r]);
+    tot += __popcll(bal[r]);
+  }
+  if (tot == 0) return;
+  const int slice = t % kSlices;
+  int32_t base = 0;
+  if (lane == 0) base = atomicAdd(&a.ctrl->work_count[slice], tot);
+  base = __shfl(base, 0);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int32_t off = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (want[r]) a.work[int64_t(slice) * a.slice_cap + base + off + __popcll(bal[r] & below)] = it[r];
+    off += __popcll(bal[r]);
+  }
 }
 
-// Maps a position v of the concatenated work-list slices to its item.
-struct WorkIndex {
+// Cluster set of a tile, rebuilt from scratch: labels of its items and of
+// their neighbours, gathered in an LDS hash table (linear probing, bounded).
+__global__ __launch_bounds__(64) void k_tile_sets(SweepArgs a, const int32_t* __restrict__ mask) {
+  constexpr int kTab = 256;
+  constexpr int32_t kEmpty = -2;
+  __shared__ int32_t tab[kTab];
+  __shared__ int32_t n_uniq, overflow, n_out;
+  const int32_t t = blockIdx.x;
+  const int m = a.item_mesh ? a.tile_mesh[t] : 0;
+  if (mask && !mask[m]) return;
+  for (int k = threadIdx.x; k < kTab; k += 64) tab[k] = kEmpty;
+  if (threadIdx.x == 0) {
+    n_uniq = 0;
+    overflow = 0;
+    n_out = 0;
+  }
+  __syncthreads();
+  auto insert = [&](int32_t c) {
+    if (c == kNull) return;
+    uint32_t h = (uint32_t(c) * 2654435761u) >> 24;  // top 8 bits
+    for (int probe = 0; probe < kTab; ++probe) {
+      const int32_t old = atomicCAS(&tab[h], kEmpty, c);
+      if (old == kEmpty) {
+        atomicAdd(&n_uniq, 1);
+        return;
+      }
+      if (old == c) return;
+      h = (h + 1) & (kTab - 1);
+    }
+    atomicOr(&overflow, 1);
+  };
+  for (int r = 0; r < kTileItems / 64; ++r) {
+    const int32_t i = a.tile_items[int64_t(t) * kTileItems + r * 64 + threadIdx.x];
+    if (i < 0) continue;
+    insert(a.labels[i]);
+    for (int32_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) insert(a.labels[a.col[k]]);
+  }
+  __syncthreads();
+  const bool big = overflow || n_uniq > kTileSet;
+  if (!big) {
+    for (int k = threadIdx.x; k < kTab; k += 64) {
+      const int32_t c = tab[k];
+      if (c != kEmpty) a.tile_set[int64_t(t) * kTileSet + atomicAdd(&n_out, 1)] = c;
+    }
+  }
+  if (threadIdx.x == 0) a.tile_cnt[t] = big ? kTileSet + 1 : n_uniq;
+}
+// Maps a position v of the concatenation of a sliced list (kSlices slices of
+// capacity cap, fill counts in counts[]) to its entry.
+struct SliceIndex {
   int32_t start[kSlices + 1];
 };
 
-__device__ inline void load_work_index(const SweepArgs& a, WorkIndex* wi) {
+__device__ inline void load_slice_index(const int32_t* counts, SliceIndex* wi) {
   if (threadIdx.x == 0) {
     int32_t s = 0;
     for (int k = 0; k < kSlices; ++k) {
       wi->start[k] = s;
-      s += a.ctrl->work_count[k];
+      s += counts[k];
     }
     wi->start[kSlices] = s;
   }
   __syncthreads();
 }
 
-__device__ inline int32_t work_item(const SweepArgs& a, const WorkIndex* wi, int32_t v) {
+__device__ inline int32_t slice_entry(const int32_t* list, int32_t cap, const SliceIndex* wi, int32_t v) {
   int lo = 0, hi = kSlices;  // find the slice k with start[k] <= v < start[k+1]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (wi->start[mid] <= v) lo = mid;
     else hi = mid;
   }
-  return a.work[int64_t(lo) * a.slice_cap + (v - wi->start[lo])];
+  return list[int64_t(lo) * cap + (v - wi->start[lo])];
 }
 
-// Evaluate: best move of each work item (shared rule of acvd_math.h); a
-// proposal registers its accept key at both clusters and joins the
-// candidate list.
+// Wave-aggregated append of item i (when want) to slice `slice` of the
+// candidate list; every lane of the wave must call it.
+__device__ inline void cand_append(const SweepArgs& a, int slice, bool want, int32_t i) {
+  const unsigned long long bal = __ballot(want);
+  if (bal == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)bal) - 1;
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(&a.ctrl->cand_count[slice], __popcll(bal));
+  base = __shfl(base, leader);
+  if (want) a.cand[int64_t(slice) * a.cand_cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = i;
+}
+
+// Evaluate (round 0): best move of each work item (shared rule of
+// acvd_math.h) against the state at the pass start; a proposal registers its
+// round-0 accept key at both clusters and joins the candidate list.
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
-  __shared__ WorkIndex wi;
-  load_work_index(a, &wi);
+  __shared__ SliceIndex wi;
+  load_slice_index(a.ctrl->work_count, &wi);
   const int32_t n = wi.start[kSlices];
   const SweepView v = make_view(a);
+  const int slice = blockIdx.x % kSlices;
+  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    const int32_t t = base + int32_t(threadIdx.x);
+    bool want = false;
+    int64_t i = 0;
+    if (t < n) {
+      i = slice_entry(a.work, a.slice_cap, &wi, t);
+      const int m = mesh_of_item(a, i);
+      const DevMesh& dm = a.meshes[m];
+      if (dm.metric == METRIC) {
+        MeshState* st = &a.state[m];
+        const int32_t lab = a.labels[i];
+        int32_t target = kNull;
+        if (propose_item<METRIC>(v, i, lab, st->connexity, st->constrained, dm.level, dm.scales, &target)) {
+          const int32_t L = st->loops;
+          const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
+          if (lab != kNull) atomicMin(&a.prio[lab], key);
+          atomicMin(&a.prio[target], key);
+          a.cand_pass[i] = L;
+          a.cand_target[i] = target;
+          a.tile_prop[a.item_tile[i]] = L;
+          if (a.item_mesh) atomicAdd(&st->pass_props, 1);
+          want = true;
+        }
+      }
+    }
+    cand_append(a, slice, want, int32_t(i));
+  }
+}
+
+// Re-evaluate (round r > 0): each live candidate (cand_target >= 0) is
+// evaluated again against the state left by round r-1; it either registers
+// its round-r key with its (possibly new) target or is dropped (-3).
+template <int METRIC>
+__global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
+  __shared__ SliceIndex ci;
+  load_slice_index(a.ctrl->cand_count, &ci);
+  const int32_t n = ci.start[kSlices];
+  const SweepView v = make_view(a);
   for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    const int64_t i = work_item(a, &wi, t);
+    const int64_t i = slice_entry(a.cand, a.cand_cap, &ci, t);
+    if (a.cand_target[i] < 0) continue;
     const int m = mesh_of_item(a, i);
     const DevMesh& dm = a.meshes[m];
     if (dm.metric != METRIC) continue;
@@ -226,17 +338,29 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
     const int32_t lab = a.labels[i];
     int32_t target = kNull;
     if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
-      const int32_t L = st.loops;
-      const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
+      const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
       if (lab != kNull) atomicMin(&a.prio[lab], key);
       atomicMin(&a.prio[target], key);
-      a.cand_pass[i] = L;
       a.cand_target[i] = target;
+    } else {
+      a.cand_target[i] = -3;
     }
   }
 }
 
 // --------------------------------------------------------------------- apply
+// Adds cluster c to the set of tile t unless present (sets only grow between
+// rebuilds; duplicates from concurrent inserts only use capacity).
+__device__ inline void tile_insert(const SweepArgs& a, int32_t t, int32_t c) {
+  const int32_t cnt = a.tile_cnt[t];
+  if (cnt > kTileSet) return;
+  const int32_t* set = a.tile_set + int64_t(t) * kTileSet;
+  for (int32_t k = 0; k < cnt; ++k)
+    if (set[k] == c) return;
+  const int32_t slot = atomicAdd(&a.tile_cnt[t], 1);
+  if (slot < kTileSet) a.tile_set[int64_t(t) * kTileSet + slot] = c;
+}
+
 __device__ void update_cluster(const SweepArgs& a, int64_t c, int sign, int64_t iw, const int64_t* is,
                                const int64_t* iq, const DevMesh& dm, int constrained) {
   const unsigned long long s = (unsigned long long)int64_t(sign);
@@ -266,58 +390,31 @@ __device__ inline int block_sum(int v) {
   return tot;  // valid in thread 0
 }
 
-// Rematch (rounds >= 1): proposals of this pass that are not applied and
-// whose two clusters have not moved register their round key.
-__global__ __launch_bounds__(kBlock) void k_rematch(SweepArgs a, int round) {
-  __shared__ WorkIndex wi;
-  load_work_index(a, &wi);
-  const int32_t n = wi.start[kSlices];
-  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    const int64_t i = work_item(a, &wi, t);
-    const int m = mesh_of_item(a, i);
-    const MeshState& st = a.state[m];
-    const int32_t L = st.loops;
-    if (a.cand_pass[i] != L || st.phase != kPhaseSweep) continue;
-    const int32_t lab = a.labels[i];
-    const int32_t b = a.cand_target[i];
-    if (lab == b) continue;  // applied in an earlier round
-    if ((lab != kNull && a.moved[lab] == L) || a.moved[b] == L) continue;
-    const unsigned long long key = accept_key(a.meshes[m].seed, L, round, uint32_t(i - a.meshes[m].item_off));
-    if (lab != kNull) atomicMin(&a.prio[lab], key);
-    atomicMin(&a.prio[b], key);
-  }
-}
-
-// Apply round `round`: a proposal of this pass holding the minimum round key
-// at both clusters moves.  Round 0 also counts the proposals.  Counts go to
-// the sliced counters (single mesh) or per mesh (batches).
+// Apply round `round`: a live candidate holding the minimum round key at both
+// of its clusters moves.  Counts go to the sliced counters (single mesh) or
+// per mesh (batches).
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
-  __shared__ WorkIndex wi;
-  load_work_index(a, &wi);
-  const int32_t n = wi.start[kSlices];
-  int props = 0, mods = 0;
+  __shared__ SliceIndex ci;
+  load_slice_index(a.ctrl->cand_count, &ci);
+  const int32_t n = ci.start[kSlices];
+  int mods = 0;
   for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
     const int32_t t = base + int32_t(threadIdx.x);
     if (t >= n) continue;
-    const int64_t i = work_item(a, &wi, t);
+    const int64_t i = slice_entry(a.cand, a.cand_cap, &ci, t);
+    const int32_t b = a.cand_target[i];
+    if (b < 0) continue;
     const int m = mesh_of_item(a, i);
     MeshState* st = &a.state[m];
     const int32_t L = st->loops;
-    if (a.cand_pass[i] != L || st->phase != kPhaseSweep) continue;
     const int32_t lab = a.labels[i];
-    const int32_t b = a.cand_target[i];
-    if (round == 0) {
-      ++props;
-      if (a.item_mesh) atomicAdd(&st->pass_props, 1);
-    } else if (lab == b) {
-      continue;  // applied in an earlier round
-    }
     const DevMesh dm = a.meshes[m];
     const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
     if (lab != kNull && a.prio[lab] != key) continue;
     if (a.prio[b] != key) continue;
-    // exclusive owner of clusters lab and b for this pass
+    // exclusive owner of clusters lab and b for this round
     a.labels[i] = b;
+    a.cand_target[i] = -3;
     const float4 p = a.xw[i];
     const Scales sc = dm.scales;
     const int64_t iw = fix_w(p.w, sc);
@@ -330,23 +427,23 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
       update_cluster(a, lab, -1, iw, is, iq, dm, constrained);
       a.sizes[lab] -= 1;
       a.stamp[lab] = L;
-      a.moved[lab] = L;
     }
     update_cluster(a, b, +1, iw, is, iq, dm, constrained);
     a.sizes[b] += 1;
     a.stamp[b] = L;
-    a.moved[b] = L;
+    // item i now shows cluster b to its own tile and to its neighbours' tiles
+    const int32_t ti = a.item_tile[i];
+    tile_insert(a, ti, b);
+    for (int32_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
+      const int32_t tj = a.item_tile[a.col[k]];
+      if (tj != ti) tile_insert(a, tj, b);
+    }
     ++mods;
     if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
   }
   if (!a.item_mesh) {
-    const int tp = block_sum(props);
     const int tm = block_sum(mods);
-    if (threadIdx.x == 0) {
-      const int slice = blockIdx.x % kSlices;
-      if (tp) atomicAdd(&a.ctrl->props[slice], tp);
-      if (tm) atomicAdd(&a.ctrl->mods[slice], tm);
-    }
+    if (threadIdx.x == 0 && tm) atomicAdd(&a.ctrl->mods[blockIdx.x % kSlices], tm);
   }
 }
 
@@ -363,7 +460,7 @@ __global__ void k_controller(SweepArgs a) {
     int w = 0, p = 0, d = 0;
     for (int k = 0; k < kSlices; ++k) {
       w += a.ctrl->work_count[k];
-      p += a.ctrl->props[k];
+      p += a.ctrl->cand_count[k];
       d += a.ctrl->mods[k];
     }
     s_work = w;
@@ -390,7 +487,7 @@ __global__ void k_controller(SweepArgs a) {
   __syncthreads();
   if (threadIdx.x < kSlices) {
     a.ctrl->work_count[threadIdx.x] = 0;
-    a.ctrl->props[threadIdx.x] = 0;
+    a.ctrl->cand_count[threadIdx.x] = 0;
     a.ctrl->mods[threadIdx.x] = 0;
   }
   if (threadIdx.x == 0) {
@@ -615,9 +712,15 @@ hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mask,
   return hipGetLastError();
 }
 
+hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask) {
+  if (a.n_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_sets, dim3(unsigned(a.n_tiles)), dim3(64), 0, s, a, mesh_mask);
+  return hipGetLastError();
+}
+
 hipError_t launch_scan(hipStream_t s, const SweepArgs& a) {
   if (a.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_scan, dim3(unsigned(a.n_tiles)), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -627,12 +730,14 @@ hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any
   return hipGetLastError();
 }
 
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid) {
-  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, 0);
-  for (int r = 1; r < kMatchRounds; ++r) {
-    hipLaunchKernelGGL(k_rematch, dim3(grid), dim3(kBlock), 0, s, a, r);
-    hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, r);
-  }
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round) {
+  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, round);
+  return hipGetLastError();
+}
+
+hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid, int round) {
+  if (any_iso) hipLaunchKernelGGL(k_reeval<0>, dim3(grid), dim3(kBlock), 0, s, a, round);
+  if (any_qem) hipLaunchKernelGGL(k_reeval<1>, dim3(grid), dim3(kBlock), 0, s, a, round);
   return hipGetLastError();
 }
 

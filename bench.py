@@ -16,11 +16,13 @@ Extra legs:
   roofline     the dominant sweep kernel's algorithmic bytes per launch over its
                average launch duration (HIP events, one extra timed step);
   cpu_baseline the threaded restatement of the reference (ACVDQP,
-               vtkThreadedClustering) on all host cores, on a bounded sample
-               of the same family (a 1024 x 512 torus, 84 items per cluster as in the
-               workload);
+               vtkThreadedClustering) on all host cores, from the same initial
+               labels, on the full workload (or a 2048 x 1024 torus);
+               --gap adds the serial restatement (ACVDQ) for the energy gap;
   batched      (--batch) 384 star-shaped icospheres (levels 4-8, iso/QEM 50/50,
-               K = V/15) packed in one context vs clustered one by one.
+               K = V/15) packed in one context vs clustered one by one;
+  energy       (--energy) fixed-assignment energy forward and forward+backward
+               on 16 QEM tori 1024 x 1024 (K = 25,000 each) with an FD check.
 """
 import argparse
 import json
@@ -136,37 +138,53 @@ def run_gpu_leg(args):
     }
 
 
-def run_cpu_baseline(args):
-    """ACVDQP restatement (oracle B) on all host cores, on a bounded sample of
-    the workload family: torus 1024 x 512 (524,288 vertices) with K = 6,250
-    (same items per cluster as the workload), gradation 1.5, from the same
-    kind of device-seeded initial labelling.  The GPU runs the same sample."""
+def run_cpu_baseline(args, g):
+    """ACVDQP restatement (oracle B, vtkThreadedClustering) on all host cores,
+    from the GPU's initial labels.  Sample: the full workload when
+    --cpu-sample full (the default: about 10-30 s of CPU work on 16 cores),
+    or a 2048 x 1024 torus (K = 25,000, same items per cluster) otherwise."""
     from oracle import pyoracle as po
-    xyz, tri, H, K = torus_workload(1024, 512, 6250)
-    ctx = make_ctx(xyz, tri, H, K)
-    ctx.initial_sampling()
-    init = ctx.labels(0)
-    ctx.set_initial_labels(0, init)
-    ctx.process()
-    ctx.process()
-    gpu_sample_ms = ctx.stats(0)["converge_ms"]
-    glab = ctx.labels(0)
-    threads = os.cpu_count() or 1
+    if args.cpu_sample == "full":
+        xyz, tri, H, K, init = g["xyz"], g["tri"], g["H"], g["K"], g["init"]
+        glab, gpu_ms = g["labels"], statistics.median(g["dev_ms"])
+        sample = "the full workload (torus 4096x2048, 8,388,608 vertices, K=100000)"
+    else:
+        xyz, tri, H, K = torus_workload(2048, 1024, 25000)
+        ctx = make_ctx(xyz, tri, H, K)
+        ctx.initial_sampling()
+        init = ctx.labels(0)
+        ctx.set_initial_labels(0, init)
+        ctx.process()
+        ctx.process()
+        gpu_ms = ctx.stats(0)["converge_ms"]
+        glab = ctx.labels(0)
+        sample = "torus 2048x1024 (2,097,152 vertices), K=25000"
+    threads = int(os.environ.get("ACVD_CPU_THREADS", "0")) or (os.cpu_count() or 1)
+    t = time.time()
     m = po.RefMesh(xyz, tri)
+    log(f"cpu_baseline: reference mesh structure built in {time.time() - t:.1f} s")
     lab, st = m.threaded_run(po.QEM, K, 1.5, H, 3, 1, init, threads=threads, pooling=5)
     cpu_ms = st["seconds"] * 1e3
     e_cpu = m.energy(po.QEM, K, lab, 1.5, H)[0]
     e_gpu = m.energy(po.QEM, K, glab, 1.5, H)[0]
-    log(f"cpu_baseline: ACVDQP restatement, {threads} threads, {cpu_ms:.0f} ms on the sample "
-        f"({st['loops']} loops, {st['convergences']} convergences); GPU on the same sample {gpu_sample_ms:.1f} ms; "
-        f"energy gap gpu vs ACVDQP {(e_gpu - e_cpu) / e_cpu * 100:+.3f}%")
-    return {"value": round(cpu_ms, 1), "unit": "ms to converge (sample)", "cores": threads, "kind": "port",
-            "sample": "torus 1024x512 (524,288 vertices), K=6250, QEM, gradation 1.5, same initial labels as the "
-                      "GPU; full convergence of vtkThreadedClustering restated (PoolingRatio 5)",
-            "gpu_same_sample_ms": round(gpu_sample_ms, 2),
-            "speedup_same_sample": round(cpu_ms / gpu_sample_ms, 1),
-            "energy_gap_gpu_vs_cpu_pct": round((e_gpu - e_cpu) / e_cpu * 100, 4),
-            "cpu_loops": st["loops"], "cpu_convergences": st["convergences"]}
+    out = {"value": round(cpu_ms, 1), "unit": "ms to converge", "cores": threads, "kind": "port",
+           "sample": sample + ", QEM, gradation 1.5, same initial labels as the GPU; full convergence of "
+                     "vtkThreadedClustering restated (PoolingRatio 5), timed around ProcessClustering",
+           "gpu_same_sample_ms": round(gpu_ms, 2), "speedup_same_sample": round(cpu_ms / gpu_ms, 1),
+           "energy_gap_gpu_vs_acvdqp_pct": round((e_gpu - e_cpu) / e_cpu * 100, 4),
+           "cpu_loops": st["loops"], "cpu_convergences": st["convergences"]}
+    log(f"cpu_baseline: ACVDQP restatement, {threads} threads, {cpu_ms:.0f} ms ({st['loops']} loops, "
+        f"{st['convergences']} convergences); GPU on the same sample {gpu_ms:.1f} ms; energy gap gpu vs ACVDQP "
+        f"{out['energy_gap_gpu_vs_acvdqp_pct']:+.3f}%")
+    if args.gap:
+        t = time.time()
+        slab, sst = m.serial_run(po.QEM, K, 1.5, H, 3, 1, init)
+        e_ser = m.energy(po.QEM, K, slab, 1.5, H)[0]
+        out["serial_ms"] = round(sst["seconds"] * 1e3, 1)
+        out["energy_gap_gpu_vs_serial_pct"] = round((e_gpu - e_ser) / e_ser * 100, 4)
+        log(f"gap: serial ACVDQ restatement {sst['seconds']:.1f} s ({time.time() - t:.0f} s wall), energy gap "
+            f"gpu vs serial {out['energy_gap_gpu_vs_serial_pct']:+.4f}%")
+    return out
 
 
 def run_batch_leg(args):
@@ -224,14 +242,97 @@ def run_batch_leg(args):
             "identical_labels": same}
 
 
+def run_energy_leg(args):
+    """Differentiable fixed-assignment energy: 16 tori 1024 x 1024 (QEM, K =
+    25,000 each, |H|^1.5 weights), labels frozen from one batched GPU
+    clustering.  Times the forward and forward + backward with HIP events on
+    torch's stream, and checks the adjoint against FP64 central differences
+    (step 1e-6) at 1,000 seeded coordinates."""
+    import torch
+    import acvd_amd
+    from acvd_amd import meshgen
+    from acvd_amd.energy import ClusterEnergy
+    from oracle import pyoracle as po
+    count, nu, nv, K = args.energy_meshes, 1024, 1024, 25000
+    ctx = acvd_amd.Context(0)
+    meshes = []
+    for j in range(count):
+        xyz, tri = meshgen.torus(nu, nv, seed=500 + j)
+        H = meshgen.mean_curvature(xyz, tri)
+        mid = ctx.add_input(xyz, tri)
+        ctx.set_params(mid, number_of_clusters=K, metric=acvd_amd.METRIC_QEM, unconstrained_initialization=1,
+                       gradation=1.5, seed=j)
+        ctx.set_indicator(mid, H)
+        meshes.append((xyz, tri))
+    t = time.time()
+    ctx.process()
+    log(f"energy leg: {count} tori clustered in {time.time() - t:.1f} s wall")
+    ce = ClusterEnergy(0)
+    xs, ws, ls = [], [], []
+    for j, (xyz, tri) in enumerate(meshes):
+        ce.add_mesh(len(xyz), tri, K, acvd_amd.METRIC_QEM, 3)
+        xw, _ = ctx.items(j)
+        xs.append(np.asarray(xyz, np.float32))
+        ws.append(xw[:, 3].copy())
+        ls.append(ctx.labels(j))
+    dev = torch.device("cuda", 0)
+    x = torch.tensor(np.concatenate(xs), device=dev).contiguous()
+    w = torch.tensor(np.concatenate(ws), device=dev).contiguous()
+    lab = torch.tensor(np.concatenate(ls).astype(np.int32), device=dev).contiguous()
+    g = torch.ones(count, dtype=torch.float64, device=dev)
+    n = x.shape[0]
+    for _ in range(3):
+        ce.forward_raw(x, w, lab)
+        ce.backward_raw(x, w, lab, g)
+    torch.cuda.synchronize()
+    reps = 20
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    for _ in range(reps):
+        e = ce.forward_raw(x, w, lab)
+    ev[1].record()
+    for _ in range(reps):
+        e = ce.forward_raw(x, w, lab)
+        dx, dw = ce.backward_raw(x, w, lab, g)
+    ev[2].record()
+    torch.cuda.synchronize()
+    fwd_ms = ev[0].elapsed_time(ev[1]) / reps
+    fb_ms = ev[1].elapsed_time(ev[2]) / reps
+    # FD check on mesh 0 (1,000 seeded coordinates)
+    x0 = xs[0].astype(np.float64)
+    w0 = ws[0].astype(np.float64)
+    ref = po.EnergyRef(len(x0), meshes[0][1], ls[0], K, 1, 3)
+    rng = np.random.default_rng(5)
+    n0 = len(x0)
+    coords = rng.choice(4 * n0, 1000, replace=False)
+    fd = ref.central_differences(x0, w0, coords, 1e-6)
+    gg = np.concatenate([dx[:n0].cpu().numpy().reshape(-1), dw[:n0].cpu().numpy()]).astype(np.float64)
+    scale = np.abs(fd).max()
+    err = np.abs(gg[coords] - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    e_cpu0 = ref.total(x0, w0)
+    log(f"energy: forward {fwd_ms:.3f} ms, forward+backward {fb_ms:.3f} ms (ratio {fb_ms / fwd_ms:.2f}), "
+        f"{n} items; FD rel err median {np.median(err):.2e} max {err.max():.2e}; "
+        f"E0 gpu {e[0].item():.9e} cpu {e_cpu0:.9e}")
+    fwd_bytes = 24.0 * n
+    return {"meshes": count, "items": n, "forward_ms": round(fwd_ms, 3), "forward_backward_ms": round(fb_ms, 3),
+            "ratio": round(fb_ms / fwd_ms, 3), "forward_GBps_algorithmic": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9, 1),
+            "fd_rel_err_median": float(np.median(err)), "fd_rel_err_max": float(err.max()),
+            "energy_rel_diff_vs_cpu": float(abs(e[0].item() - e_cpu0) / e_cpu0)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", choices=["full", "half"], default="full",
+                    help="CPU baseline on the full workload or on a 2048x1024 torus")
+    ap.add_argument("--gap", action="store_true", help="also run the serial ACVDQ restatement for the energy gap")
     ap.add_argument("--batch", action="store_true", help="also run the 384-mesh batched leg")
     ap.add_argument("--batch-count", type=int, default=384)
+    ap.add_argument("--energy", action="store_true", help="also run the differentiable-energy leg (16 tori)")
+    ap.add_argument("--energy-meshes", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
@@ -261,9 +362,11 @@ def main():
         "roofline": g["roofline"],
     }
     if not args.skip_cpu:
-        out["cpu_baseline"] = run_cpu_baseline(args)
+        out["cpu_baseline"] = run_cpu_baseline(args, g)
     if args.batch:
         out["batched"] = run_batch_leg(args)
+    if args.energy:
+        out["energy"] = run_energy_leg(args)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,174 @@
+// C++ drop-in for the reference's remeshing classes, over the C-ABI of
+// libacvd_amd.so (include/acvd_c.h).  VTK-free: a Mesh value type replaces
+// vtkSurface, an int32 label vector replaces vtkIntArray.
+//
+//   reference (DiscreteRemeshing/)                         here
+//   vtkIsotropicDiscreteRemeshing (vtkIsotropicDiscreteRemeshing.h:42-69)  acvd::IsotropicDiscreteRemeshing
+//   vtkQIsotropicDiscreteRemeshing (:71-102)                acvd::QIsotropicDiscreteRemeshing
+//   ::New() / ->Delete()                                    New() / Delete()
+//   SetInput(vtkSurface*) (vtkSurfaceClustering.h:178-211)  SetInput(const Mesh*)
+//   SetNumberOfClusters (vtkUniformClustering.h:62-69)      same
+//   SetConsoleOutput (:95), SetUnconstrainedInitialization (:81),
+//   SetMaxNumberOfLoops (:103), SetMaxNumberOfConvergences (:99),
+//   SetInitialClustering (:111)                             same
+//   GetMetric()->SetGradation / SetQuadricsOptimizationLevel
+//     (vtkIsotropicMetricForClustering.h:73-76, vtkQEMetricForClustering.h:50-54)  same
+//   SetSubsamplingThreshold (vtkDiscreteRemeshing.h:71)     same
+//   ProcessClustering() -> vtkIntArray* (vtkUniformClustering.h:75)
+//                                                           ProcessClustering() -> const Labels*
+//   GetClustering() (:78)                                   same (borrowed, owned by the object)
+//   Remesh() (vtkDiscreteRemeshing.h:64, 877-954)           same (no subdivision: fails instead)
+//   GetOutput() -> vtkSurface* (:61)                        GetOutput() -> const Mesh* (borrowed)
+//
+// Error behaviour follows the reference: no exceptions; a failing call
+// returns 0 / nullptr and, with console output on, prints the reason
+// (vtkUniformClustering.h:659-664).  GetLastError() returns it as well.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../acvd_c.h"
+
+namespace acvd {
+
+struct Mesh {
+  std::vector<double> points;      // 3 per vertex
+  std::vector<int32_t> triangles;  // 3 per face
+  int64_t GetNumberOfPoints() const { return int64_t(points.size() / 3); }
+  int64_t GetNumberOfCells() const { return int64_t(triangles.size() / 3); }
+  void GetPoint(int64_t i, double p[3]) const {
+    for (int k = 0; k < 3; ++k) p[k] = points[3 * i + k];
+  }
+};
+
+using Labels = std::vector<int32_t>;  // 0..K-1, K = null
+
+template <int METRIC>
+class DiscreteRemeshing {
+ public:
+  class MetricHandle {
+   public:
+    void SetGradation(double g) { p_->gradation = g; }
+    double GetGradation() const { return p_->gradation; }
+    void SetQuadricsOptimizationLevel(int l) { p_->quadrics_optimization_level = l; }
+    int GetQuadricsOptimizationLevel() const { return p_->quadrics_optimization_level; }
+    // per-vertex curvature indicator (vtkIsotropicMetricForClustering.h:66,
+    // vtkQEMetricForClustering.h:75 take it as a vtkDataArrayCollection)
+    void SetCurvatureInfo(const std::vector<double>* ind) { *ind_ = ind; }
+
+   private:
+    friend class DiscreteRemeshing;
+    MetricHandle(acvd_params* p, const std::vector<double>** ind) : p_(p), ind_(ind) {}
+    acvd_params* p_;
+    const std::vector<double>** ind_;
+  };
+
+  static DiscreteRemeshing* New() { return new DiscreteRemeshing(); }
+  void Delete() { delete this; }
+
+  void SetInput(const Mesh* m) { input_ = m; }
+  const Mesh* GetInput() const { return input_; }
+  void SetNumberOfClusters(int k) { params_.number_of_clusters = k; }
+  int GetNumberOfClusters() const { return params_.number_of_clusters; }
+  void SetConsoleOutput(int v) { params_.console_output = v; }
+  void SetUnconstrainedInitialization(int v) { params_.unconstrained_initialization = v; }
+  void SetMaxNumberOfLoops(int v) { params_.max_number_of_loops = v; }
+  void SetMaxNumberOfConvergences(int v) { params_.max_number_of_convergences = v; }
+  void SetInitialClustering(const Labels* l) { initial_ = l; }
+  void SetSubsamplingThreshold(int t) { subsampling_threshold_ = t; }
+  void SetSeed(uint32_t s) { params_.seed = s; }
+  void SetDevice(int d) { device_ = d; }
+  MetricHandle* GetMetric() { return &metric_; }
+  int64_t GetNumberOfItems() const { return input_ ? input_->GetNumberOfPoints() : 0; }
+  int GetClusteringType() const { return 1; }  // items are vertices
+  const std::string& GetLastError() const { return error_; }
+
+  // ProcessClustering (vtkUniformClustering.h:654-714).  Returns the labels
+  // (borrowed) or nullptr when K == 0 or on error.
+  const Labels* ProcessClustering() {
+    if (params_.number_of_clusters <= 0) return fail("NumberOfClusters is 0"), nullptr;
+    if (!input_) return fail("no input"), nullptr;
+    if (!ensure_ctx()) return nullptr;
+    const int64_t nv = input_->GetNumberOfPoints(), nf = input_->GetNumberOfCells();
+    if (!ok(acvd_set_input_f64(ctx_, nv, input_->points.data(), nf, input_->triangles.data()))) return nullptr;
+    acvd_params p = params_;
+    p.metric = METRIC;
+    if (!ok(acvd_set_params(ctx_, 0, &p))) return nullptr;
+    if (indicator_ && !ok(acvd_set_indicator(ctx_, 0, indicator_->data()))) return nullptr;
+    if (initial_ && !ok(acvd_set_initial_labels(ctx_, 0, initial_->data()))) return nullptr;
+    if (!ok(acvd_process_clustering(ctx_))) return nullptr;
+    clustering_.resize(size_t(nv));
+    if (!ok(acvd_get_labels(ctx_, 0, clustering_.data()))) return nullptr;
+    clustered_ = true;
+    return &clustering_;
+  }
+  const Labels* GetClustering() const { return clustered_ ? &clustering_ : nullptr; }
+
+  // Remesh (vtkDiscreteRemeshing.h:877-954): clustering, then the output mesh.
+  // Returns 1 on success, 0 on failure.
+  int Remesh() {
+    if (input_ && GetNumberOfItems() < int64_t(subsampling_threshold_) * params_.number_of_clusters)
+      return fail("input has fewer than SubsamplingThreshold * NumberOfClusters vertices; subdivide it first"), 0;
+    if (!ProcessClustering()) return 0;
+    return BuildOutput(0);
+  }
+  // Output mesh with vertices moved by the quadric placement of
+  // Examples/ACVD.cxx:217-291 at `level` (0 = cluster points).
+  int BuildOutput(int level) {
+    if (!clustered_) return fail("no clustering computed"), 0;
+    if (!ok(acvd_build_output(ctx_, 0, level))) return 0;
+    int64_t nv = 0, nf = 0;
+    if (!ok(acvd_get_output_sizes(ctx_, 0, &nv, &nf))) return 0;
+    output_.points.resize(size_t(3 * nv));
+    output_.triangles.resize(size_t(3 * nf));
+    if (!ok(acvd_get_output(ctx_, 0, output_.points.data(), output_.triangles.data()))) return 0;
+    return 1;
+  }
+  const Mesh* GetOutput() const { return clustered_ ? &output_ : nullptr; }
+
+  int GetStats(acvd_stats* st) const { return ctx_ ? acvd_get_stats(ctx_, 0, st) == ACVD_OK : 0; }
+
+ protected:
+  DiscreteRemeshing() : metric_(&params_, &indicator_) { acvd_params_init(&params_); }
+  ~DiscreteRemeshing() {
+    if (ctx_) acvd_destroy(ctx_);
+  }
+  DiscreteRemeshing(const DiscreteRemeshing&) = delete;
+  DiscreteRemeshing& operator=(const DiscreteRemeshing&) = delete;
+
+ private:
+  bool ensure_ctx() {
+    if (ctx_) return true;
+    return ok(acvd_create(&ctx_, device_));
+  }
+  bool ok(int rc) {
+    if (rc == ACVD_OK) return true;
+    fail(acvd_last_error());
+    return false;
+  }
+  void fail(const std::string& msg) {
+    error_ = msg;
+    if (params_.console_output) std::fprintf(stdout, "%s\n", msg.c_str());
+  }
+
+  acvd_params params_{};
+  MetricHandle metric_;
+  const Mesh* input_ = nullptr;
+  const Labels* initial_ = nullptr;
+  const std::vector<double>* indicator_ = nullptr;
+  int subsampling_threshold_ = 10;
+  int device_ = 0;
+  acvd_ctx* ctx_ = nullptr;
+  Labels clustering_;
+  Mesh output_;
+  bool clustered_ = false;
+  std::string error_;
+};
+
+using IsotropicDiscreteRemeshing = DiscreteRemeshing<ACVD_METRIC_ISOTROPIC>;
+using QIsotropicDiscreteRemeshing = DiscreteRemeshing<ACVD_METRIC_QEM>;
+
+}  // namespace acvd

@@ -80,7 +80,8 @@ typedef struct acvd_stats {
   int64_t n_items;          /* items of this mesh */
   int64_t n_adjacency;      /* CSR entries of this mesh (2E) */
   /* per-kernel device time when timing is on (acvd_set_timing), context-wide:
-   * [0] scan, [1] evaluate, [2] apply, [3] controller */
+   * [0] scan, [1] evaluate, [2] accept rounds (apply, re-evaluate, apply ...),
+   * [3] controller; launches count one per pass for [2] */
   double kernel_ms[4];
   int64_t kernel_launches[4];
 } acvd_stats;
@@ -189,6 +190,34 @@ int acvd_get_items(acvd_ctx* ctx, int32_t mesh_id, float* xw, float* q);
 int acvd_get_scales(acvd_ctx* ctx, int32_t mesh_id, double* scales3);
 /* Runs the device build (CSR, items, scales) without clustering. */
 int acvd_build(acvd_ctx* ctx);
+
+/* ---- fixed-assignment energy and its adjoint -------------------------------
+ * No reference counterpart: the reference minimises the clustering energy but
+ * never differentiates it.  For frozen labels, per mesh:
+ *   isotropic: E = sum_i w_i |x_i - c|^2 with c = S/W of the item's cluster
+ *              (the unshifted energy of vtkIsotropicMetricForClustering.h:114-121);
+ *   QEM:       E = sum_i w_i |x_i - q|^2 with q the rank-truncated quadric point
+ *              of the cluster (vtkQEMetricForClustering.h:194-208,
+ *              Common/vtkQuadricTools.cxx:83-177), the cluster quadric being the
+ *              sum of the plane quadrics of the faces around its vertices,
+ *              recomputed from x so that E is differentiable in x.
+ * All buffers of forward/backward are DEVICE pointers: xyz float[3N], w
+ * float[N], labels int32[N] (mesh-local ids; K or out of range = no cluster),
+ * energy / grad_energy double[M], dxyz float[3N], dw float[N]; meshes are
+ * packed in the order they were added.  stream is a hipStream_t (NULL: the
+ * null stream); calls are asynchronous on it.  backward uses the per-cluster
+ * state of the last forward, which must have run on the same inputs. */
+typedef struct acvd_energy acvd_energy;
+int acvd_energy_create(acvd_energy** out, int device);
+int acvd_energy_destroy(acvd_energy* e);
+/* tri: 3*nf mesh-local vertex ids (host memory, copied). */
+int acvd_energy_add_mesh(acvd_energy* e, int64_t nv, int64_t nf, const int32_t* tri, int32_t n_clusters,
+                         int32_t metric, int32_t quadrics_level, int32_t* mesh_id);
+int acvd_energy_num_items(const acvd_energy* e, int64_t* n_items, int32_t* n_meshes);
+int acvd_energy_forward(acvd_energy* e, const float* xyz, const float* w, const int32_t* labels, double* energy,
+                        void* stream);
+int acvd_energy_backward(acvd_energy* e, const float* xyz, const float* w, const int32_t* labels,
+                         const double* grad_energy, float* dxyz, float* dw, void* stream);
 
 #ifdef __cplusplus
 }

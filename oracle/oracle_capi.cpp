@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "devinputs.h"
+#include "energy_ref.h"
 #include "metrics.h"
 #include "quadric.h"
 #include "ref_mesh.h"
@@ -389,3 +390,24 @@ void orc_devinputs_get(void* h, int32_t* rp, int32_t* col, float* xw, float* q, 
 void orc_devinputs_destroy(void* h) { delete static_cast<DevInputs*>(h); }
 
 }  // extern "C"
+
+// ---- fixed-assignment energy reference
+extern "C" void* orc_energy_create(int64_t nv, int64_t nf, const int32_t* tri, const int32_t* labels, int K,
+                                   int metric, int level) {
+  return new oracle::EnergyRef(nv, nf, tri, labels, K, metric, level);
+}
+extern "C" void orc_energy_destroy(void* h) { delete static_cast<oracle::EnergyRef*>(h); }
+extern "C" double orc_energy_total(void* h, const double* x, const double* w) {
+  return static_cast<oracle::EnergyRef*>(h)->total(x, w);
+}
+// coords: 3*i+k for x_i[k], 3*nv+i for w_i
+extern "C" void orc_energy_derivatives(void* h, const double* x, const double* w, int64_t n, const int64_t* coords,
+                                       double* out) {
+  auto* e = static_cast<oracle::EnergyRef*>(h);
+  for (int64_t k = 0; k < n; ++k) out[k] = e->derivative(x, w, coords[k]);
+}
+extern "C" void orc_energy_central_differences(void* h, const double* x, const double* w, int64_t n,
+                                               const int64_t* coords, double step, double* out) {
+  auto* e = static_cast<oracle::EnergyRef*>(h);
+  for (int64_t k = 0; k < n; ++k) out[k] = e->central_difference(x, w, coords[k], step);
+}

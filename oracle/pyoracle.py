@@ -334,3 +334,54 @@ def quadric_post_placement(xyz, tri, labels, K, points, level):
     for c in range(K):
         out[c], _ = rep_point(cq[c, :9], out[c], level)
     return out
+
+
+# ---- fixed-assignment energy reference (oracle/energy_ref.h)
+class EnergyRef:
+    """FP64 fixed-assignment energy of one mesh (isotropic CVT or QEM) with
+    dual-number derivatives and central differences per input coordinate
+    (3*i+k: x_i[k]; 3*nv+i: w_i)."""
+
+    def __init__(self, nv, tri, labels, K, metric, level=3):
+        L = lib()
+        L.orc_energy_create.restype = ctypes.c_void_p
+        L.orc_energy_create.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_energy_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_energy_total.restype = ctypes.c_double
+        L.orc_energy_total.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_energy_derivatives.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_energy_central_differences.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_int64, ctypes.c_void_p, ctypes.c_double,
+                                                     ctypes.c_void_p]
+        self.nv = int(nv)
+        t = np.ascontiguousarray(tri, np.int32)
+        lab = np.ascontiguousarray(labels, np.int32)
+        self.h = L.orc_energy_create(self.nv, len(t), _p(t), _p(lab), int(K), int(metric), int(level))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_energy_destroy(self.h)
+            self.h = None
+
+    def total(self, x, w):
+        x = np.ascontiguousarray(x, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        return lib().orc_energy_total(self.h, _p(x), _p(w))
+
+    def derivatives(self, x, w, coords):
+        x = np.ascontiguousarray(x, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        c = np.ascontiguousarray(coords, np.int64)
+        out = np.empty(len(c))
+        lib().orc_energy_derivatives(self.h, _p(x), _p(w), len(c), _p(c), _p(out))
+        return out
+
+    def central_differences(self, x, w, coords, step=1e-6):
+        x = np.ascontiguousarray(x, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        c = np.ascontiguousarray(coords, np.int64)
+        out = np.empty(len(c))
+        lib().orc_energy_central_differences(self.h, _p(x), _p(w), len(c), _p(c), float(step), _p(out))
+        return out

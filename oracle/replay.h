@@ -6,7 +6,8 @@
 // arithmetic (acvd_amd/csrc/acvd_math.h, compiled here with
 // -ffp-contract=off), so its final labels must equal the device's bit for bit.
 // What it restates independently of the device code: the pass structure
-// (propose over a frozen state, min-key accept, exclusive apply), the
+// (propose over a frozen state, rounds of min-key accept and exclusive apply
+// with re-evaluation of the proposals left), the
 // synchronous fill rounds, the largest-component clean and the host-side
 // phase decisions between convergences.
 #pragma once
@@ -36,7 +37,7 @@ struct Replay {
   const float* xw = nullptr;
   const float* q = nullptr;
 
-  std::vector<int32_t> labels, sizes, stamp, moved, cand_pass, pending;
+  std::vector<int32_t> labels, sizes, stamp, cand_pass, pending;
   std::vector<unsigned long long> acc_ws, acc_q;
   std::vector<acvd::ClusterGeom> geom;
   std::vector<double> T;
@@ -122,68 +123,80 @@ struct Replay {
     derive(c);
   }
 
+  void apply_move(int64_t i, int32_t lab, int32_t b, int32_t L) {
+    labels[i] = b;
+    const float w = xw[4 * i + 3];
+    const int64_t iw = acvd::fix_w(w, sc);
+    const int64_t is[3] = {acvd::fix_s(w, xw[4 * i], sc), acvd::fix_s(w, xw[4 * i + 1], sc),
+                           acvd::fix_s(w, xw[4 * i + 2], sc)};
+    int64_t iq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (metric == 1)
+      for (int k = 0; k < 10; ++k) iq[k] = acvd::fix_q(q[10 * i + k], sc);
+    if (lab != acvd::kNull) {
+      update(lab, -1, iw, is, iq);
+      sizes[lab]--;
+      stamp[lab] = L;
+    }
+    update(b, +1, iw, is, iq);
+    sizes[b]++;
+    stamp[b] = L;
+  }
+
+  bool propose(int64_t i, int32_t* target) {
+    const acvd::SweepView v = view();
+    const int32_t lab = labels[i];
+    return metric == 0 ? acvd::propose_item<0>(v, i, lab, st.connexity, st.constrained, level, sc, target)
+                       : acvd::propose_item<1>(v, i, lab, st.connexity, st.constrained, level, sc, target);
+  }
+
   void pass() {
     const int32_t L = st.loops;
     const acvd::SweepView v = view();
-    std::vector<std::pair<int64_t, int32_t>> cands;
+    // round 0 proposals: every item needs_eval selects, against the frozen state
+    std::vector<int64_t> cand;
+    std::vector<int32_t> target;
     for (int64_t i = 0; i < N; ++i) {
-      const int32_t lab = labels[i];
-      if (!acvd::needs_eval(v, i, lab, L)) continue;
+      if (!acvd::needs_eval(v, i, labels[i], L)) continue;
       stats.evaluated++;
-      int32_t target = acvd::kNull;
-      bool ok = metric == 0 ? acvd::propose_item<0>(v, i, lab, st.connexity, st.constrained, level, sc, &target)
-                            : acvd::propose_item<1>(v, i, lab, st.connexity, st.constrained, level, sc, &target);
-      if (!ok) continue;
-      cands.emplace_back(i, target);
+      int32_t t = acvd::kNull;
+      if (!propose(i, &t)) continue;
+      cand.push_back(i);
+      target.push_back(t);
     }
-    // matching rounds: in each round the proposals whose clusters have not
-    // moved in this pass register their round key; winners are applied
-    stats.proposed += int64_t(cands.size());
-    for (auto& c : cands) cand_pass[c.first] = L;
-    std::vector<int32_t> origlab(cands.size());
-    std::vector<char> applied(cands.size(), 0);
-    for (size_t t = 0; t < cands.size(); ++t) origlab[t] = labels[cands[t].first];
+    stats.proposed += int64_t(cand.size());
+    for (int64_t i : cand) cand_pass[i] = L;
     int mods = 0;
+    std::vector<int32_t> from(cand.size());
     for (int round = 0; round < acvd::kMatchRounds; ++round) {
-      for (size_t t = 0; t < cands.size(); ++t) {
-        if (applied[t]) continue;
-        const int32_t a = origlab[t], b = cands[t].second;
-        if ((a != acvd::kNull && moved[a] == L) || moved[b] == L) continue;
-        const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(cands[t].first));
-        if (a != acvd::kNull && key < prio[a]) prio[a] = key;
-        if (key < prio[b]) prio[b] = key;
-      }
-      for (size_t t = 0; t < cands.size(); ++t) {
-        if (applied[t]) continue;
-        const int64_t i = cands[t].first;
-        const int32_t lab = origlab[t], b = cands[t].second;
-        const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(i));
-        if (lab != acvd::kNull && prio[lab] != key) continue;
-        if (prio[b] != key) continue;
-        applied[t] = 1;
-        labels[i] = b;
-        const float w = xw[4 * i + 3];
-        const int64_t iw = acvd::fix_w(w, sc);
-        const int64_t is[3] = {acvd::fix_s(w, xw[4 * i], sc), acvd::fix_s(w, xw[4 * i + 1], sc),
-                               acvd::fix_s(w, xw[4 * i + 2], sc)};
-        int64_t iq[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (metric == 1)
-          for (int k = 0; k < 10; ++k) iq[k] = acvd::fix_q(q[10 * i + k], sc);
-        if (lab != acvd::kNull) {
-          update(lab, -1, iw, is, iq);
-          sizes[lab]--;
-          stamp[lab] = L;
-          moved[lab] = L;
+      if (round > 0) {
+        // live proposals are re-evaluated against the state left by the last round
+        for (size_t t = 0; t < cand.size(); ++t) {
+          if (target[t] < 0) continue;
+          int32_t nt = acvd::kNull;
+          target[t] = propose(cand[t], &nt) ? nt : -3;
         }
-        update(b, +1, iw, is, iq);
-        sizes[b]++;
-        stamp[b] = L;
-        moved[b] = L;
+        // (all re-evaluations of a round see the same state: none is applied before
+        // the keys below are compared)
+      }
+      for (size_t t = 0; t < cand.size(); ++t) from[t] = labels[cand[t]];
+      for (size_t t = 0; t < cand.size(); ++t) {
+        if (target[t] < 0) continue;
+        const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(cand[t]));
+        if (from[t] != acvd::kNull && key < prio[from[t]]) prio[from[t]] = key;
+        if (key < prio[target[t]]) prio[target[t]] = key;
+      }
+      for (size_t t = 0; t < cand.size(); ++t) {
+        if (target[t] < 0) continue;
+        const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(cand[t]));
+        if (from[t] != acvd::kNull && prio[from[t]] != key) continue;
+        if (prio[target[t]] != key) continue;
+        apply_move(cand[t], from[t], target[t], L);
+        target[t] = -3;
         mods++;
       }
     }
     st.pass_mods = mods;
-    st.pass_props = int32_t(cands.size());
+    st.pass_props = int32_t(cand.size());
   }
 
   // Largest connected component per cluster (ties: smallest first item);
@@ -240,7 +253,6 @@ struct Replay {
     for (int64_t i = 0; i < N; ++i) labels[i] = (init[i] < 0 || init[i] >= K) ? acvd::kNull : init[i];
     sizes.assign(K, 0);
     stamp.assign(K, 0);
-    moved.assign(K, -2);
     cand_pass.assign(N, -2);
     pending.assign(N, acvd::kNull);
     acc_ws.assign(4 * size_t(K), 0ull);

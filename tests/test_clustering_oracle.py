@@ -154,9 +154,11 @@ def test_accept_keys_order_newer_passes_first():
     k_old = po.hdr_accept_key(5, 10, 123)
     k_new = po.hdr_accept_key(5, 11, 123)
     assert k_new < k_old
-    # later matching rounds of a pass rank before earlier ones, and before the next pass
+    # later matching rounds of a pass rank before earlier ones, and the next
+    # pass before every round of this one (6 rounds per pass)
     assert po.hdr_accept_key(5, 10, 123, round=1) < k_old
     assert po.hdr_accept_key(5, 10, 7, round=2) < po.hdr_accept_key(5, 10, 123, round=1)
-    assert k_new < po.hdr_accept_key(5, 10, 7, round=2)
+    assert po.hdr_accept_key(5, 10, 7, round=5) < po.hdr_accept_key(5, 10, 123, round=4)
+    assert k_new < po.hdr_accept_key(5, 10, 7, round=5)
     keys = {po.hdr_accept_key(5, 7, i) for i in range(5000)}
     assert len(keys) == 5000  # unique per item within a pass

@@ -101,23 +101,26 @@ def test_labels_bit_exact_against_replay(gpu_ready, name):
 def test_energy_against_serial_reference(gpu_ready, name):
     """Final unshifted FP64 energy of the GPU labels against oracle A run from
     the same initial labels.  Distinct local minima of meshes this small
-    scatter by about 1 % (measured over initial labellings), so the bound here
-    is 0.5 %; the 0.1 % target is measured at the benchmark sizes."""
+    scatter by about 1 % (graded: the serial reference itself lands 5-45 %
+    apart across initial labellings), so the statistic is the median gap over
+    five accept seeds, bounded by 0.5 % (1 % graded); the 0.1 % target is
+    measured at the benchmark sizes."""
     xyz, tri, metric, K, unc, ind, grad, level = case_inputs(name)
     m = po.RefMesh(xyz, tri)
     init = m.initial_sampling(metric, K, grad, ind)
-    ctx = run_gpu(xyz, tri, metric, K, unc, init, ind, grad)
-    lab = ctx.labels(0)
     slab, _ = m.serial_run(metric, K, grad, ind, 3, unc, init)
-    eg = m.energy(metric, K, lab, grad, ind)[0]
     es = m.energy(metric, K, slab, grad, ind)[0]
-    gap = (eg - es) / es
-    print(f"{name}: gpu {eg:.9e} serial {es:.9e} gap {gap * 100:+.4f}%")
-    # graded QEM on 8k vertices: the serial reference itself scatters by
-    # 10-40 % across initial labellings, so only gross failures are caught here
+    gaps = []
+    for seed in (7, 8, 9, 10, 11):
+        ctx = run_gpu(xyz, tri, metric, K, unc, init, ind, grad, seed=seed)
+        lab = ctx.labels(0)
+        eg = m.energy(metric, K, lab, grad, ind)[0]
+        gaps.append((eg - es) / es)
+        # the device's own FP64 energy (FP32 items) agrees with the oracle's
+        assert np.isclose(ctx.energy(0), eg, rtol=1e-5)
+    gap = float(np.median(gaps))
+    print(f"{name}: serial {es:.9e} gaps {np.round(np.array(gaps) * 100, 4)} % median {gap * 100:+.4f}%")
     assert gap < (1e-2 if grad > 0 else 5e-3)
-    # the device's own FP64 energy (FP32 items) agrees with the oracle's
-    assert np.isclose(ctx.energy(0), eg, rtol=1e-5)
 
 
 @pytest.mark.parametrize("metric,unc", [(ISO, 0), (QEM, 1)])

@@ -62,3 +62,15 @@ def test_null_context_is_rejected():
     assert L.acvd_process_clustering(None) == -1
     assert b"null context" in L.acvd_last_error()
     assert L.acvd_destroy(None) == 0
+
+
+DROPIN = os.path.join(ROOT, "tests", "cpp", "_build", "dropin")
+
+
+@pytest.mark.skipif(N.lib().acvd_device_count() > 0, reason="a device is visible")
+def test_cpp_dropin_compiles_links_and_fails_loudly_without_device():
+    import subprocess
+    assert os.path.exists(DROPIN), "build() compiles tests/cpp"
+    r = subprocess.run([DROPIN], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert r.stdout.startswith("fail") and "no HIP device" in r.stdout

@@ -134,3 +134,18 @@ def test_gpu_remesh_class_api(gpu_ready):
     assert np.array_equal(out.triangles, po.output_triangles(tri, r.GetClustering(), 300))
     r.QuadricPostProcessing(1)
     assert r.GetOutput().points.shape == (300, 3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["iso", "qem"])
+def test_gpu_cpp_dropin_program(gpu_ready, mode):
+    """include/acvd/remeshing.hpp used as ACVD.cxx / ACVDQ.cxx use the
+    reference classes: icosphere L4 (2562 vertices) -> 200 clusters."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "dropin")
+    r = subprocess.run([exe] + (["q"] if mode == "qem" else []), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tag, nv, nf, nnull = r.stdout.split()
+    assert tag == "ok" and int(nv) == 200 and int(nnull) == 0
+    assert abs(int(nf) - 396) <= 20
