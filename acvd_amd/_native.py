@@ -30,6 +30,8 @@ class Params(ctypes.Structure):
         ("max_number_of_convergences", ctypes.c_int32),
         ("console_output", ctypes.c_int32),
         ("seed", ctypes.c_uint32),
+        ("initial_sampling", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("gradation", ctypes.c_double),
     ]
 
@@ -69,7 +71,7 @@ EXPORTS = [
     "acvd_get_energy", "acvd_get_stats", "acvd_get_adjacency_size", "acvd_get_adjacency", "acvd_get_items",
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
     "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
-    "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward",
+    "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
 ]
 
 _lib = None
@@ -116,6 +118,7 @@ def lib():
     L.acvd_get_adjacency_size.argtypes = [P, i32, ctypes.POINTER(i64)]
     L.acvd_get_adjacency.argtypes = [P, i32, P, P]
     L.acvd_get_items.argtypes = [P, i32, P, P]
+    L.acvd_reference_initial_sampling.argtypes = [i64, P, i64, P, i32, P, ctypes.c_double, i32, P]
     L.acvd_energy_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
     L.acvd_energy_destroy.argtypes = [P]
     L.acvd_energy_add_mesh.argtypes = [P, i64, i64, P, i32, i32, i32, ctypes.POINTER(i32)]

@@ -527,11 +527,13 @@ struct MeshState {
 // Early stop of the first two phases.  The reference stops a phase when a
 // loop applies at most N/1000 moves (vtkUniformClustering.h:773-776), a loop
 // being a sweep of the whole edge queue with immediate updates.  A GPU pass
-// applies only a matching of the improving moves it finds, so the analogue
-// here counts the improving moves found (proposals) in a pass; the threshold
-// N/10000 was chosen by measuring final energies against the serial
-// reference (equal or lower energy gap than N/1000 on every workload tried).
-constexpr int64_t kEarlyStopDivisor = 10000;
+// applies only matchings of the improving moves it finds, so the analogue
+// here counts the improving moves found (proposals) in a pass.  The
+// threshold N/3000 was chosen by measuring final energies against the serial
+// reference: on a graded QEM torus (524,288 vertices, K = 6,250) N/10000,
+// N/3000 and N/1000 end within 0.06 % of each other while N/3000 saves 15 %
+// of the passes of N/10000.
+constexpr int64_t kEarlyStopDivisor = 3000;
 
 // Controller transition after one pass (the decision part of MinimizeEnergy).
 // Returns true if the mesh reached a convergence (clean + fill must follow).

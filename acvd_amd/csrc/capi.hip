@@ -19,6 +19,7 @@
 
 #include "../../include/acvd_c.h"
 #include "acvd_ctx.h"
+#include "host_sampling.h"
 
 using namespace acvd;
 
@@ -664,11 +665,20 @@ int initial_labels(acvd_ctx* c) {
   SweepArgs a = c->args();
   for (auto& h : c->meshes) {
     const int32_t K = h.params.number_of_clusters;
-    if (!h.init_labels.empty()) {
+    if (!h.init_labels.empty() || h.params.initial_sampling == 0) {
       std::vector<int32_t> lab(h.nv);
-      for (int64_t i = 0; i < h.nv; ++i) {
-        int32_t l = h.init_labels[i];
-        lab[i] = (l < 0 || l >= K) ? kNull : int32_t(l + h.clus_off);
+      if (h.init_labels.empty()) {
+        // the reference's region growing, then local ids -> global
+        if (!reference_initial_sampling(h.nv, h.xyz.data(), h.nf, h.tri.data(), h.params.metric,
+                                        h.indicator.empty() ? nullptr : h.indicator.data(), h.params.gradation, K,
+                                        lab.data()))
+          return fail(ACVD_ERR_STATE, "initial sampling failed");
+        for (int64_t i = 0; i < h.nv; ++i) lab[i] = lab[i] == K ? kNull : int32_t(lab[i] + h.clus_off);
+      } else {
+        for (int64_t i = 0; i < h.nv; ++i) {
+          int32_t l = h.init_labels[i];
+          lab[i] = (l < 0 || l >= K) ? kNull : int32_t(l + h.clus_off);
+        }
       }
       HIP_TRY(hipMemcpyAsync(c->d_labels.p + h.item_off, lab.data(), h.nv * sizeof(int32_t), hipMemcpyHostToDevice,
                              s));
@@ -715,6 +725,8 @@ void acvd_params_init(acvd_params* p) {
   p->max_number_of_convergences = 1000000000;
   p->console_output = 0;
   p->seed = 0;
+  p->initial_sampling = 0;
+  p->reserved = 0;
   p->gradation = 0.0;
 }
 
@@ -768,6 +780,7 @@ int acvd_set_params(acvd_ctx* ctx, int32_t mesh_id, const acvd_params* p) {
   if (p->metric != ACVD_METRIC_ISOTROPIC && p->metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "unknown metric");
   if (p->quadrics_optimization_level < 0 || p->quadrics_optimization_level > 3)
     return fail(ACVD_ERR_ARG, "quadrics optimization level must be 0..3");
+  if (p->initial_sampling != 0 && p->initial_sampling != 1) return fail(ACVD_ERR_ARG, "initial sampling must be 0 or 1");
   if (p->max_number_of_loops < 0) return fail(ACVD_ERR_ARG, "negative loop limit");
   if (mesh_id == -1) {
     for (auto& h : ctx->meshes) h.params = *p;
@@ -884,6 +897,18 @@ int acvd_initial_sampling(acvd_ctx* ctx) {
 int acvd_set_timing(acvd_ctx* ctx, int enable) {
   if (int rc = check_ctx(ctx)) return rc;
   ctx->timing = enable != 0;
+  return ACVD_OK;
+}
+
+int acvd_reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int32_t metric,
+                                    const double* indicator, double gradation, int32_t n_clusters, int32_t* labels) {
+  if (nv <= 0 || nf <= 0 || !xyz || !tri || !labels) return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  if (n_clusters <= 0 || n_clusters > nv) return fail(ACVD_ERR_ARG, "number of clusters must be in 1..nv");
+  if (metric != ACVD_METRIC_ISOTROPIC && metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "unknown metric");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv) return fail(ACVD_ERR_ARG, "triangle references a missing vertex");
+  if (!reference_initial_sampling(nv, xyz, nf, tri, metric, indicator, gradation, n_clusters, labels))
+    return fail(ACVD_ERR_STATE, "initial sampling failed");
   return ACVD_OK;
 }
 

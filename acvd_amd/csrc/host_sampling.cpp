@@ -1,0 +1,206 @@
+// Reference initial sampling on the host.
+//
+// Region growing from shuffled start items until each region holds the mean
+// weight (vtkUniformClustering.h:1177-1316).  It depends on the reference's
+// topology order: the neighbours of a vertex are visited in the order its
+// edges were created when the mesh was built face by face (edges
+// (v0,v1), (v1,v2), (v2,v0) of each face, Common/vtkSurfaceBase.cxx:1166-1221,
+// 1407-1468; rings in insertion order, :1057-1068), and the item weights are
+// the metric's: vertex area = sum over the neighbour faces, in the order
+// GetVertexNeighbourFaces lists them (vtkSurfaceBase.cxx:964-984), of the
+// face area (Common/vtkSurface.cxx:1280-1283) / 3; times indicator^gradation
+// (isotropic: whenever an indicator is set, vtkIsotropicMetricForClustering.h:
+// 249-270; QEM: when gradation > 0, vtkQEMetricForClustering.h:324-342);
+// clamped to [avg/R, avg*R] with R = 1e5 (isotropic) or 1e4 (QEM).
+// The shuffle keeps the reference's std::mt19937 seeded with 0 and its
+// `rng() % n` index (vtkUniformClustering.h:1202-1207).
+#include "host_sampling.h"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <random>
+#include <vector>
+
+namespace acvd {
+
+namespace {
+
+double face_area(const double* xyz, const int32_t* t) {
+  const double* p1 = xyz + 3 * int64_t(t[0]);
+  const double* p2 = xyz + 3 * int64_t(t[1]);
+  const double* p3 = xyz + 3 * int64_t(t[2]);
+  auto d2 = [](const double* a, const double* b) {
+    return (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+  };
+  const double a = d2(p1, p2), b = d2(p2, p3), c = d2(p3, p1);
+  return 0.25 * std::sqrt(std::fabs(4.0 * a * c - (a - b + c) * (a - b + c)));
+}
+
+}  // namespace
+
+bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
+                                const double* indicator, double gradation, int32_t K, int32_t* labels) {
+  // vertex -> faces, increasing face id (faces with a repeated first edge
+  // vertex are inactive in the reference and skipped)
+  std::vector<int64_t> vfptr(nv + 1, 0);
+  for (int64_t f = 0; f < nf; ++f) {
+    const int32_t* t = tri + 3 * f;
+    if (t[0] == t[1]) continue;
+    for (int j = 0; j < 3; ++j) vfptr[t[j] + 1]++;
+  }
+  for (int64_t v = 0; v < nv; ++v) vfptr[v + 1] += vfptr[v];
+  std::vector<int32_t> vf(vfptr[nv]);
+  {
+    std::vector<int64_t> fill(vfptr.begin(), vfptr.end() - 1);
+    for (int64_t f = 0; f < nf; ++f) {
+      const int32_t* t = tri + 3 * f;
+      if (t[0] == t[1]) continue;
+      for (int j = 0; j < 3; ++j) {
+        // a face listing a vertex twice is stored once for it
+        if (j > 0 && t[j] == t[0]) continue;
+        if (j > 1 && t[j] == t[1]) continue;
+        vf[fill[t[j]]++] = int32_t(f);
+      }
+    }
+    // drop the slots of repeated vertices
+    std::vector<int64_t> np(nv + 1, 0);
+    std::vector<int32_t> nvf;
+    nvf.reserve(vf.size());
+    for (int64_t v = 0; v < nv; ++v) {
+      for (int64_t k = vfptr[v]; k < fill[v]; ++k) nvf.push_back(vf[k]);
+      np[v + 1] = int64_t(nvf.size());
+    }
+    vfptr.swap(np);
+    vf.swap(nvf);
+  }
+  // rings in edge-creation order: scanning the faces of v in increasing id and
+  // each face's edges in order, the first sighting of neighbour u is the
+  // creation of edge (v,u), which is when it joins both rings
+  std::vector<int64_t> rptr(nv + 1, 0);
+  std::vector<int32_t> ring;
+  ring.reserve(size_t(vfptr[nv]) * 2);
+  for (int64_t v = 0; v < nv; ++v) {
+    const size_t start = ring.size();
+    for (int64_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
+      const int32_t* t = tri + 3 * int64_t(vf[k]);
+      for (int j = 0; j < 3; ++j) {
+        const int32_t a = t[j], b = t[(j + 1) % 3];
+        if (a == b) continue;
+        int32_t u = -1;
+        if (a == v) u = b;
+        else if (b == v) u = a;
+        if (u < 0) continue;
+        bool seen = false;
+        for (size_t s = start; s < ring.size(); ++s)
+          if (ring[s] == u) {
+            seen = true;
+            break;
+          }
+        if (!seen) ring.push_back(u);
+      }
+    }
+    rptr[v + 1] = int64_t(ring.size());
+  }
+  // metric weights
+  std::vector<double> w(nv, 0.0);
+  std::vector<int32_t> nfaces;
+  for (int64_t v = 0; v < nv; ++v) {
+    nfaces.clear();
+    for (int64_t r = rptr[v]; r < rptr[v + 1]; ++r) {
+      const int32_t u = ring[r];
+      for (int64_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
+        const int32_t f = vf[k];
+        const int32_t* t = tri + 3 * int64_t(f);
+        if (t[0] != u && t[1] != u && t[2] != u) continue;
+        if (std::find(nfaces.begin(), nfaces.end(), f) == nfaces.end()) nfaces.push_back(f);
+      }
+    }
+    double a = 0.0;
+    for (int32_t f : nfaces) a += face_area(xyz, tri + 3 * int64_t(f)) / 3.0;
+    w[v] = a;
+  }
+  const bool qem = metric == 1;
+  if (indicator && (!qem || gradation > 0))
+    for (int64_t v = 0; v < nv; ++v) w[v] *= std::pow(indicator[v], gradation);
+  {
+    double avg = 0.0;
+    for (double x : w) avg += x;
+    avg /= double(nv);
+    const double ratio = qem ? 10000.0 : 100000.0;
+    const double mn = avg / ratio, mx = avg * ratio;
+    for (double& x : w) {
+      if (x > mx) x = mx;
+      if (x < mn) x = mn;
+    }
+  }
+  // region growing
+  const int64_t n = nv;
+  for (int64_t i = 0; i < n; ++i) labels[i] = K;
+  std::vector<int32_t> items(n);
+  for (int64_t i = 0; i < n; ++i) items[i] = int32_t(i);
+  std::mt19937 rng;
+  rng.seed(0);
+  for (int64_t i = n - 1; i > 0; --i) std::swap(items[i], items[rng() % uint64_t(n)]);
+  double sw = 0.0;
+  for (int64_t i = 0; i < n; ++i) sw += w[i];
+  const double target = sw / double(K);
+  int64_t remaining_items = n;
+  int32_t remaining_regions = K;
+  int64_t first = 0;
+  std::deque<int32_t> queue;
+  while (remaining_items > 0 && remaining_regions > 0) {
+    int32_t item = -1;
+    while (first < n) {
+      if (labels[items[first]] == K) {
+        item = items[first];
+        break;
+      }
+      ++first;
+    }
+    if (item < 0) break;  // every item labelled
+    queue.clear();
+    queue.push_back(item);
+    sw = 0.0;
+    --remaining_regions;
+    while (!queue.empty()) {
+      const int32_t it = queue.front();
+      queue.pop_front();
+      if (labels[it] != K) continue;
+      labels[it] = remaining_regions;
+      sw += w[it];
+      --remaining_items;
+      for (int64_t r = rptr[it]; r < rptr[it + 1]; ++r) queue.push_back(ring[r]);
+      if (sw > target) break;
+    }
+  }
+  if (remaining_regions == 0) return true;
+  // more regions than the growth produced: single items are moved into the
+  // remaining regions (vtkUniformClustering.h:1261-1311)
+  std::vector<int32_t> size(size_t(K) + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    items[i] = int32_t(i);
+    if (labels[i] != K) size[labels[i]]++;
+  }
+  for (int64_t i = n - 1; i > 0; --i) std::swap(items[i], items[rng() % uint64_t(n)]);
+  first = 0;
+  while (remaining_regions) {
+    int32_t item = -1;
+    while (first < n) {
+      item = items[first++];
+      const int32_t c = labels[item];
+      if (c == K) break;
+      if (size[c] == 1) continue;
+      labels[item] = remaining_regions;
+      size[c]--;
+      size[remaining_regions]++;
+      break;
+    }
+    if (item < 0) return false;
+    --remaining_regions;
+    labels[item] = remaining_regions;
+  }
+  return true;
+}
+
+}  // namespace acvd

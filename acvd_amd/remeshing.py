@@ -190,6 +190,19 @@ class Context:
         return s
 
 
+def reference_initial_sampling(xyz, tri, n_clusters, metric=N.METRIC_ISOTROPIC, indicator=None, gradation=0.0):
+    """The reference's default initial sampling (region growing from shuffled
+    items, vtkUniformClustering.h:1177-1316) on the host: labels 0..K-1, K for
+    unassigned items."""
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+    ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+    out = np.empty(len(x), np.int32)
+    check(N.lib().acvd_reference_initial_sampling(len(x), _p(x), len(t), _p(t), int(metric), _p(ind),
+                                                  float(gradation), int(n_clusters), _p(out)))
+    return out
+
+
 class _MetricProxy:
     """What GetMetric() returns: the metric-dependent options."""
 

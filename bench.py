@@ -320,9 +320,25 @@ def run_energy_leg(args):
             "energy_rel_diff_vs_cpu": float(abs(e[0].item() - e_cpu0) / e_cpu0)}
 
 
+def run_copy_calibration(args):
+    """Plain device-to-device copy of a known size (1 GiB), the reference for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 1 << 28  # float32 elements = 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 copies of {4 * n} bytes (read {4 * n}, write {4 * n} per copy)")
+    return {"bytes_per_copy": 4 * n, "copies": 3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (reference for the HBM byte counters)")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -338,6 +354,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        print(json.dumps(run_copy_calibration(args)), flush=True)
+        return
     g = run_gpu_leg(args)
     out = {
         "metric": "10M->100k ACVDQ converge ms; batched vs serial meshes/s at <=0.1% energy gap",

@@ -49,7 +49,10 @@ typedef struct acvd_ctx acvd_ctx;
  *   max_number_of_loops         <- SetMaxNumberOfLoops (:103), default 5000000 (:1407)
  *   max_number_of_convergences  <- SetMaxNumberOfConvergences (:99), default 1e9 (:1406)
  *   console_output              <- SetConsoleOutput (:95)
- *   seed                        <- (no counterpart) seed of the accept priorities and of the initial sampling
+ *   seed                        <- (no counterpart) seed of the accept priorities and of the device sampling
+ *   initial_sampling            <- InitSamples type (vtkUniformClustering.h:1123-1175): 0 = the reference's
+ *                                  default region growing (type 1, :1177-1316, sequential, on the host),
+ *                                  1 = weighted random seeds grown on the device (no counterpart; faster)
  */
 typedef struct acvd_params {
   int32_t number_of_clusters;
@@ -60,6 +63,8 @@ typedef struct acvd_params {
   int32_t max_number_of_convergences;
   int32_t console_output;
   uint32_t seed;
+  int32_t initial_sampling;
+  int32_t reserved;
   double gradation;
 } acvd_params;
 
@@ -94,7 +99,7 @@ const char* acvd_last_error(void);
 int acvd_device_count(void);
 /* Fills p with the reference defaults: K = 0, isotropic, level 3, gradation 0,
  * unconstrained initialisation off, max loops 5000000, max convergences 1e9,
- * console output 0, seed 0. */
+ * console output 0, seed 0, the reference's initial sampling (0). */
 void acvd_params_init(acvd_params* p);
 
 /* ---- context -------------------------------------------------------------- */
@@ -163,6 +168,14 @@ int acvd_get_cluster_points(acvd_ctx* ctx, int32_t mesh_id, double* xyz);
  * labels, accumulated in FP64 on the device (diagnostic). */
 int acvd_get_energy(acvd_ctx* ctx, int32_t mesh_id, double* energy);
 int acvd_get_stats(acvd_ctx* ctx, int32_t mesh_id, acvd_stats* st);
+
+/* The reference's default initial sampling (InitSamples type 1,
+ * vtkUniformClustering.h:1123-1175 -> ComputeInitialRandomSampling
+ * :1177-1316) of one mesh, computed on the host without a context or device:
+ * labels[nv] receives 0..K-1 and K for items left unassigned.  indicator may
+ * be NULL.  acvd_initial_sampling() uses it when initial_sampling == 0. */
+int acvd_reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int32_t metric,
+                                    const double* indicator, double gradation, int32_t n_clusters, int32_t* labels);
 
 /* ---- output mesh ---------------------------------------------------------- */
 /* BuildDelaunayTriangulation (vtkDiscreteRemeshing.h:1002-1152, vertex

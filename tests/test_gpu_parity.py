@@ -173,12 +173,21 @@ def test_batch_equals_each_mesh_alone(gpu_ready):
         assert ctx.stats(mid)["finished"] == 1
 
 
-def test_device_seeding_without_initial_labels(gpu_ready):
+@pytest.mark.parametrize("sampling", [0, 1])
+def test_initial_sampling_without_initial_labels(gpu_ready, sampling):
+    """0: the reference's region growing on the host; 1: weighted seeds grown
+    on the device.  Either way every cluster ends non-empty and connected."""
     xyz, tri = meshgen.icosphere(6, 1, 0.02)
     K = 2000
     ctx = acvd_amd.Context()
     ctx.set_input(xyz, tri)
-    ctx.set_params(0, number_of_clusters=K, seed=3)
+    ctx.set_params(0, number_of_clusters=K, seed=3, initial_sampling=sampling)
+    if sampling == 0:
+        ctx.initial_sampling()
+        want = po.RefMesh(xyz, tri).initial_sampling(ISO, K)
+        got = ctx.labels(0)
+        assigned = want < K
+        np.testing.assert_array_equal(got[assigned], want[assigned])  # the fill only touches unassigned items
     ctx.process()
     lab = ctx.labels(0)
     assert lab.min() >= 0 and lab.max() < K

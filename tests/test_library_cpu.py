@@ -74,3 +74,28 @@ def test_cpp_dropin_compiles_links_and_fails_loudly_without_device():
     r = subprocess.run([DROPIN], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert r.stdout.startswith("fail") and "no HIP device" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["ico-iso", "cube-qem", "torus-iso-graded", "torus-qem-graded", "small-K-N"])
+def test_reference_initial_sampling_equals_oracle(case):
+    """The product's host restatement of the reference's default initial
+    sampling (no device needed) gives the oracle's labels exactly: same
+    topology order, same FP64 weights, same std::mt19937 shuffle."""
+    from acvd_amd.remeshing import reference_initial_sampling
+    from oracle import pyoracle as po
+    ind, grad = None, 0.0
+    if case == "ico-iso":
+        (xyz, tri), metric, K = acvd_amd.meshgen.icosphere(4, 1, 0.02), acvd_amd.METRIC_ISOTROPIC, 150
+    elif case == "cube-qem":
+        (xyz, tri), metric, K = acvd_amd.meshgen.cube(20, 2, 1e-4), acvd_amd.METRIC_QEM, 120
+    elif case == "small-K-N":
+        (xyz, tri), metric, K = acvd_amd.meshgen.icosphere(2, 1, 0.0), acvd_amd.METRIC_ISOTROPIC, 100
+    else:
+        xyz, tri = acvd_amd.meshgen.torus(96, 48, 3)
+        ind, grad = acvd_amd.meshgen.mean_curvature(xyz, tri), 1.5
+        metric = acvd_amd.METRIC_ISOTROPIC if "iso" in case else acvd_amd.METRIC_QEM
+        K = 300
+    got = reference_initial_sampling(xyz, tri, K, metric, ind, grad)
+    want = po.RefMesh(xyz, tri).initial_sampling(metric, K, grad, ind)
+    np.testing.assert_array_equal(got, want)
+    assert got.min() >= 0 and got.max() <= K
