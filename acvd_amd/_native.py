@@ -85,6 +85,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AcvdError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (make -C acvd_amd/csrc)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # soname as ROCm's).  Loading torch first makes this library bind to that
+    # runtime, so device pointers and streams from torch tensors are valid
+    # here; loading ROCm's copy first would leave torch without a device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     i32, i64 = ctypes.c_int32, ctypes.c_int64

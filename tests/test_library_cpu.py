@@ -99,3 +99,14 @@ def test_reference_initial_sampling_equals_oracle(case):
     want = po.RefMesh(xyz, tri).initial_sampling(metric, K, grad, ind)
     np.testing.assert_array_equal(got, want)
     assert got.min() >= 0 and got.max() <= K
+
+
+def test_one_hip_runtime_per_process():
+    """The library binds to the HIP runtime torch loaded (same soname), so
+    device pointers and streams of torch tensors are valid in it."""
+    N.lib()
+    import torch  # noqa: F401
+    maps = open("/proc/self/maps").read().splitlines()
+    hip = {line.split()[-1] for line in maps if "libamdhip64" in line}
+    hsa = {line.split()[-1] for line in maps if "libhsa-runtime64" in line}
+    assert len(hip) == 1 and len(hsa) == 1, (hip, hsa)
