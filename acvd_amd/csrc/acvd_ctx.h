@@ -81,6 +81,8 @@ struct SweepArgs {
   unsigned long long* prio;    // accept keys per cluster
   int32_t* cand_pass;          // per item: loop of its last proposal
   int32_t* cand_target;        // per item: target of its live proposal (< 0: none)
+  int32_t* cand_round;         // per item: loop * kMatchRounds + round of its last evaluation
+  int32_t* cl_round;           // per cluster: loop * kMatchRounds + round of its last move
   int32_t* work;               // items to evaluate in this pass (kSlices slices)
   int32_t slice_cap;           // capacity of one work-list slice
   int32_t cand_cap;            // capacity of one candidate-list slice

@@ -102,7 +102,7 @@ struct acvd_ctx {
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T;
-  DBuf<int32_t> d_cand_target;
+  DBuf<int32_t> d_cand_target, d_cand_round, d_cl_round;
   DBuf<int32_t> d_work, d_cand;
   int32_t slice_cap = 0, cand_cap = 0;
   // scan tiles
@@ -161,6 +161,8 @@ struct acvd_ctx {
     a.prio = d_prio.p;
     a.cand_pass = d_cand_pass.p;
     a.cand_target = d_cand_target.p;
+    a.cand_round = d_cand_round.p;
+    a.cl_round = d_cl_round.p;
     a.work = d_work.p;
     a.slice_cap = slice_cap;
     a.cand_cap = cand_cap;
@@ -431,6 +433,8 @@ int build(acvd_ctx* c) {
   HIP_TRY(c->d_prio.alloc(K));
   HIP_TRY(c->d_geom.alloc(K));
   HIP_TRY(c->d_cand_target.alloc(N));
+  HIP_TRY(c->d_cand_round.alloc(N));
+  HIP_TRY(c->d_cl_round.alloc(K));
   {
     // scan tiles are dealt round-robin to the slices
     const int64_t blocks = c->n_tiles;
@@ -605,6 +609,7 @@ int minimize_energy(acvd_ctx* c) {
   HIP_TRY(launch_fill_i32(s, c->d_tile_prop.p, c->n_tiles, -2));
   HIP_TRY(launch_tile_sets(s, a, nullptr));
   HIP_TRY(launch_fill_i32(s, c->d_cand_target.p, c->N, -3));
+  HIP_TRY(launch_fill_i32(s, c->d_cl_round.p, c->K, -1));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
   for (int k = 0; k < 4; ++k) {

@@ -43,27 +43,60 @@ __global__ void k_zero_clusters(SweepArgs a, const int32_t* __restrict__ mask) {
   a.sizes[c] = 0;
 }
 
-// Item contributions in int64 fixed point (order-independent sums).
-__global__ void k_accumulate(SweepArgs a, const int32_t* __restrict__ mask, unsigned long long* mesh_count) {
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= a.n_items) return;
-  const int m = mesh_of_item(a, i);
-  if (!mask[m]) return;
-  const int32_t c = a.labels[i];
-  if (c == kNull) return;
+// Item contributions in int64 fixed point (order-independent sums).  One
+// block per scan tile (256 items in spatial order, a few clusters): the items
+// first add into an LDS table keyed by cluster, then each table entry is
+// flushed with one global atomic per value, instead of one per item.
+static_assert(kBlock == kTileItems, "one thread per tile item");
+__global__ __launch_bounds__(kBlock) void k_accumulate(SweepArgs a, const int32_t* __restrict__ mask,
+                                                       unsigned long long* mesh_count) {
+  constexpr int kVals = 15;  // W, S[3], Q[10], count
+  __shared__ int32_t keys[kTileItems];
+  __shared__ unsigned long long vals[kTileItems * kVals];
+  __shared__ unsigned long long s_count;
+  const int32_t t = blockIdx.x;
+  const int m = a.item_mesh ? a.tile_mesh[t] : 0;
+  if (!mask[m]) return;  // whole block
+  keys[threadIdx.x] = -1;
+  for (int k = 0; k < kVals; ++k) vals[threadIdx.x * kVals + k] = 0ull;
+  if (threadIdx.x == 0) s_count = 0ull;
+  __syncthreads();
   const DevMesh& dm = a.meshes[m];
-  const float4 p = a.xw[i];
-  const Scales sc = dm.scales;
-  atomicAdd(&a.acc_ws[4 * int64_t(c)], (unsigned long long)fix_w(p.w, sc));
-  atomicAdd(&a.acc_ws[4 * int64_t(c) + 1], (unsigned long long)fix_s(p.w, p.x, sc));
-  atomicAdd(&a.acc_ws[4 * int64_t(c) + 2], (unsigned long long)fix_s(p.w, p.y, sc));
-  atomicAdd(&a.acc_ws[4 * int64_t(c) + 3], (unsigned long long)fix_s(p.w, p.z, sc));
-  if (dm.metric == 1) {
-    for (int k = 0; k < 10; ++k)
-      atomicAdd(&a.acc_q[10 * int64_t(c) + k], (unsigned long long)fix_q(a.q[10 * i + k], sc));
+  const int32_t i = a.tile_items[int64_t(t) * kTileItems + threadIdx.x];
+  const int32_t c = i >= 0 ? a.labels[i] : kNull;
+  if (c != kNull) {
+    uint32_t h = (uint32_t(c) * 2654435761u) >> 24;
+    int slot = 0;
+    for (int probe = 0; probe < kTileItems; ++probe) {  // at most 256 keys: always finds a slot
+      const int32_t old = atomicCAS(&keys[h], -1, c);
+      if (old == -1 || old == c) {
+        slot = int(h);
+        break;
+      }
+      h = (h + 1) & (kTileItems - 1);
+    }
+    const float4 p = a.xw[i];
+    const Scales sc = dm.scales;
+    unsigned long long* v = vals + slot * kVals;
+    atomicAdd(&v[0], (unsigned long long)fix_w(p.w, sc));
+    atomicAdd(&v[1], (unsigned long long)fix_s(p.w, p.x, sc));
+    atomicAdd(&v[2], (unsigned long long)fix_s(p.w, p.y, sc));
+    atomicAdd(&v[3], (unsigned long long)fix_s(p.w, p.z, sc));
+    if (dm.metric == 1)
+      for (int k = 0; k < 10; ++k) atomicAdd(&v[4 + k], (unsigned long long)fix_q(a.q[10 * int64_t(i) + k], sc));
+    atomicAdd(&v[14], 1ull);
+    atomicAdd(&s_count, 1ull);
   }
-  atomicAdd(&a.sizes[c], 1);
-  if (mesh_count) atomicAdd(&mesh_count[m], 1ull);
+  __syncthreads();
+  const int32_t key = keys[threadIdx.x];
+  if (key >= 0) {
+    const unsigned long long* v = vals + threadIdx.x * kVals;
+    for (int k = 0; k < 4; ++k) atomicAdd(&a.acc_ws[4 * int64_t(key) + k], v[k]);
+    if (dm.metric == 1)
+      for (int k = 0; k < 10; ++k) atomicAdd(&a.acc_q[10 * int64_t(key) + k], v[4 + k]);
+    atomicAdd(&a.sizes[key], int32_t(v[14]));
+  }
+  if (threadIdx.x == 0 && mesh_count && s_count) atomicAdd(&mesh_count[m], s_count);
 }
 
 __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm, int constrained) {
@@ -309,6 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
           atomicMin(&a.prio[target], key);
           a.cand_pass[i] = L;
           a.cand_target[i] = target;
+          a.cand_round[i] = L * kMatchRounds;
           a.tile_prop[a.item_tile[i]] = L;
           if (a.item_mesh) atomicAdd(&st->pass_props, 1);
           want = true;
@@ -321,7 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
 
 // Re-evaluate (round r > 0): each live candidate (cand_target >= 0) is
 // evaluated again against the state left by round r-1; it either registers
-// its round-r key with its (possibly new) target or is dropped (-3).
+// its round-r key with its (possibly new) target or is dropped (-3).  A
+// candidate none of whose clusters (its own and its neighbours' labels) was
+// modified since its last evaluation would get the same answer, so it only
+// registers its key again.
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ SliceIndex ci;
@@ -336,9 +373,21 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
     if (dm.metric != METRIC) continue;
     const MeshState& st = a.state[m];
     const int32_t lab = a.labels[i];
+    const int32_t since = a.cand_round[i];
+    bool stale = lab != kNull && a.cl_round[lab] >= since;
+    for (int32_t k = a.row_ptr[i]; !stale && k < a.row_ptr[i + 1]; ++k) {
+      const int32_t b = a.labels[a.col[k]];
+      stale = b != kNull && a.cl_round[b] >= since;
+    }
+    const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
+    if (!stale) {
+      if (lab != kNull) atomicMin(&a.prio[lab], key);
+      atomicMin(&a.prio[a.cand_target[i]], key);
+      continue;
+    }
     int32_t target = kNull;
+    a.cand_round[i] = st.loops * kMatchRounds + round;
     if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
-      const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
       if (lab != kNull) atomicMin(&a.prio[lab], key);
       atomicMin(&a.prio[target], key);
       a.cand_target[i] = target;
@@ -423,14 +472,17 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
     if (dm.metric == 1)
       for (int k = 0; k < 10; ++k) iq[k] = fix_q(a.q[10 * i + k], sc);
     const int constrained = st->constrained;
+    const int32_t rtag = L * kMatchRounds + round;
     if (lab != kNull) {
       update_cluster(a, lab, -1, iw, is, iq, dm, constrained);
       a.sizes[lab] -= 1;
       a.stamp[lab] = L;
+      a.cl_round[lab] = rtag;
     }
     update_cluster(a, b, +1, iw, is, iq, dm, constrained);
     a.sizes[b] += 1;
     a.stamp[b] = L;
+    a.cl_round[b] = rtag;
     // item i now shows cluster b to its own tile and to its neighbours' tiles
     const int32_t ti = a.item_tile[i];
     tile_insert(a, ti, b);
@@ -702,7 +754,7 @@ hipError_t launch_zero_clusters(hipStream_t s, const SweepArgs& a, const int32_t
 
 hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* mask, unsigned long long* mesh_count) {
   if (a.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, mask, mesh_count);
+  hipLaunchKernelGGL(k_accumulate, dim3(unsigned(a.n_tiles)), dim3(kBlock), 0, s, a, mask, mesh_count);
   return hipGetLastError();
 }
 
