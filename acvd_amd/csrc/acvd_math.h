@@ -390,6 +390,118 @@ ACVD_HD void load_acc(const SweepView& v, int32_t c, int64_t* ws, int64_t* q10) 
   for (int k = 0; k < 10; ++k) q10[k] = int64_t(v.acc_q[10 * int64_t(c) + k]);
 }
 
+// ---------------------------------------------------------------------------
+// QEM evaluation steps.  QemItem holds what the choice needs besides the
+// solves; everything is read from the state frozen for the current round, so
+// recomputing it in another kernel gives the same bits.
+struct QemItem {
+  float px, py, pz, pw;
+  float ta;      // leave term of the item's cluster
+  double Ta;     // T of the item's cluster (0 when unconstrained)
+  int32_t r0, r1;
+};
+
+ACVD_HD void qem_item_terms(const SweepView& v, int64_t i, int32_t lab, int constrained, QemItem* s) {
+  s->px = v.xw[4 * i];
+  s->py = v.xw[4 * i + 1];
+  s->pz = v.xw[4 * i + 2];
+  s->pw = v.xw[4 * i + 3];
+  s->r0 = v.row_ptr[i];
+  s->r1 = v.row_ptr[i + 1];
+  s->ta = leave_term(s->px, s->py, s->pz, s->pw, v.geom[lab]);
+  s->Ta = constrained ? v.T[lab] : 0.0;
+}
+
+// Exact prune before any FP64 solve: T' >= 0, so with monotone rounding the
+// computed dE of a move to b is at least LB_b = double(w (tb - ta)) + (-(Ta + Tb));
+// a target with LB_b >= 0 can neither pass the acceptance test nor be the
+// best of a set that contains an acceptable move, so it is skipped.
+ACVD_HD bool qem_open(const SweepView& v, const QemItem& s, int32_t b, int constrained) {
+  const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
+  const double Tb = constrained ? v.T[b] : 0.0;
+  return double(s.pw * (tb - s.ta)) + (-(s.Ta + Tb)) < 0.0;
+}
+
+// CSR position of the next open target after position k (each neighbouring
+// cluster once, at its first position), or s.r1.
+ACVD_HD int32_t qem_next_open(const SweepView& v, int32_t lab, const QemItem& s, int constrained, int32_t k) {
+  for (++k; k < s.r1; ++k) {
+    const int32_t b = v.labels[v.col[k]];
+    if (b == kNull || b == lab) continue;
+    bool seen = false;
+    for (int32_t t = s.r0; t < k; ++t)
+      if (v.labels[v.col[t]] == b) {
+        seen = true;
+        break;
+      }
+    if (seen) continue;
+    if (qem_open(v, s, b, constrained)) return k;
+  }
+  return s.r1;
+}
+
+// Guards (size, ring check, leave condition) and prune of a labelled item.
+// Returns true when at least one target is open (solves and choice follow).
+ACVD_HD bool qem_screen(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained, QemItem* s) {
+  if (v.sizes[lab] <= 1) return false;
+  if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
+  if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
+  qem_item_terms(v, i, lab, constrained, s);
+  for (int32_t k = s->r0; k < s->r1; ++k) {
+    const int32_t b = v.labels[v.col[k]];
+    if (b == kNull || b == lab) continue;
+    if (qem_open(v, *s, b, constrained)) return true;
+  }
+  return false;
+}
+
+// T of the item's cluster without the item, and of cluster b with it.
+ACVD_HD void item_fixed(const SweepView& v, int64_t i, const Scales& sc, int64_t* iw, int64_t* is, int64_t* iq) {
+  const float px = v.xw[4 * i], py = v.xw[4 * i + 1], pz = v.xw[4 * i + 2], pw = v.xw[4 * i + 3];
+  *iw = fix_w(pw, sc);
+  is[0] = fix_s(pw, px, sc);
+  is[1] = fix_s(pw, py, sc);
+  is[2] = fix_s(pw, pz, sc);
+  for (int k = 0; k < 10; ++k) iq[k] = fix_q(v.q[10 * i + k], sc);
+}
+ACVD_HD double qem_leave_T(const SweepView& v, int64_t i, int32_t lab, const Scales& sc, int level) {
+  int64_t iw, is[3], iq[10], ws[4], q10[10];
+  item_fixed(v, i, sc, &iw, is, iq);
+  load_acc(v, lab, ws, q10);
+  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level);
+}
+ACVD_HD double qem_join_T(const SweepView& v, int64_t i, int32_t b, const Scales& sc, int level) {
+  int64_t iw, is[3], iq[10], ws[4], q10[10];
+  item_fixed(v, i, sc, &iw, is, iq);
+  load_acc(v, b, ws, q10);
+  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level);
+}
+
+// Choice among the open targets (CSR order; ties: lowest cluster id) and the
+// acceptance test.  tb_new(k, b) returns T of cluster b (first seen at CSR
+// position k) with the item.
+template <class TbNew>
+ACVD_HD bool qem_choose(const SweepView& v, int32_t lab, const QemItem& s, int constrained, double Ta_new,
+                        TbNew tb_new, int32_t* target_out) {
+  int32_t target = kNull;
+  double best_d = 0.0, best_m = 0.0;
+  for (int32_t k = qem_next_open(v, lab, s, constrained, s.r0 - 1); k < s.r1;
+       k = qem_next_open(v, lab, s, constrained, k)) {
+    const int32_t b = v.labels[v.col[k]];
+    const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
+    const double Tb = constrained ? v.T[b] : 0.0;
+    const double Tb_new = tb_new(k, b);
+    const double d = qem_delta(s.pw, s.ta, tb, Ta_new, s.Ta, Tb_new, Tb);
+    if (target == kNull || d < best_d || (d == best_d && b < target)) {
+      best_d = d;
+      best_m = qem_mag(s.pw, s.ta, tb, Ta_new, s.Ta, Tb_new, Tb);
+      target = b;
+    }
+  }
+  *target_out = target;
+  return target != kNull && qem_accept(best_d, best_m);
+}
+
 // Best move of an evaluated item (label lab, loop L).  Returns true and the
 // target cluster if the move is proposed.
 //  - null item: joins the neighbouring cluster with the smallest join term
@@ -418,82 +530,33 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
     *target_out = target;
     return target != kNull;
   }
+  if (METRIC == 1) {
+    // QEM: screen, solves, choice (the device runs the three steps in
+    // separate kernels so that the FP64 solves of one item run on parallel lanes)
+    QemItem s;
+    if (!qem_screen(v, i, lab, connexity, constrained, &s)) return false;
+    const double Ta_new = constrained ? qem_leave_T(v, i, lab, sc, level) : 0.0;
+    return qem_choose(
+        v, lab, s, constrained, Ta_new,
+        [&](int32_t, int32_t b) { return constrained ? qem_join_T(v, i, b, sc, level) : 0.0; }, target_out);
+  }
   if (v.sizes[lab] <= 1) return false;
   if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
   const ClusterGeom ga = v.geom[lab];
   if (!can_leave(pw, ga)) return false;
   const float ta = leave_term(px, py, pz, pw, ga);
-  if (METRIC == 0) {
-    float best = 3.0e38f;
-    for (int32_t k = r0; k < r1; ++k) {
-      const int32_t b = v.labels[v.col[k]];
-      if (b == kNull || b == lab) continue;
-      const float tb = join_term(px, py, pz, pw, v.geom[b]);
-      if (tb < best || (tb == best && b < target)) {
-        best = tb;
-        target = b;
-      }
-    }
-    *target_out = target;
-    return target != kNull && iso_accept(ta, best);
-  }
-  // QEM.  Exact prune before any FP64 solve: T' >= 0, so with monotone
-  // rounding the computed dE of a move to b is at least
-  //   LB_b = double(w (tb - ta)) + (-(Ta + Tb));
-  // a target with LB_b >= 0 can neither pass the acceptance test nor be the
-  // best of a set that contains an acceptable move, so it is skipped.
-  const double Ta = constrained ? v.T[lab] : 0.0;
-  bool any_open = false;
+  float best = 3.0e38f;
   for (int32_t k = r0; k < r1; ++k) {
     const int32_t b = v.labels[v.col[k]];
     if (b == kNull || b == lab) continue;
     const float tb = join_term(px, py, pz, pw, v.geom[b]);
-    const double Tb = constrained ? v.T[b] : 0.0;
-    if (double(pw * (tb - ta)) + (-(Ta + Tb)) < 0.0) {
-      any_open = true;
-      break;
-    }
-  }
-  if (!any_open) return false;
-  const int64_t iw = fix_w(pw, sc);
-  const int64_t is[3] = {fix_s(pw, px, sc), fix_s(pw, py, sc), fix_s(pw, pz, sc)};
-  int64_t iq[10];
-  for (int k = 0; k < 10; ++k) iq[k] = fix_q(v.q[10 * i + k], sc);
-  double Ta_new = 0.0;
-  if (constrained) {
-    int64_t ws[4], q10[10];
-    load_acc(v, lab, ws, q10);
-    Ta_new = qem_T_trial(ws, q10, iw, is, iq, -1, sc, level);
-  }
-  double best_d = 0.0, best_m = 0.0;
-  for (int32_t k = r0; k < r1; ++k) {
-    const int32_t b = v.labels[v.col[k]];
-    if (b == kNull || b == lab) continue;
-    bool seen = false;  // each neighbouring cluster is evaluated once
-    for (int32_t t = r0; t < k; ++t)
-      if (v.labels[v.col[t]] == b) {
-        seen = true;
-        break;
-      }
-    if (seen) continue;
-    const float tb = join_term(px, py, pz, pw, v.geom[b]);
-    const double Tb = constrained ? v.T[b] : 0.0;
-    if (!(double(pw * (tb - ta)) + (-(Ta + Tb)) < 0.0)) continue;  // pruned
-    double Tb_new = 0.0;
-    if (constrained) {
-      int64_t ws[4], q10[10];
-      load_acc(v, b, ws, q10);
-      Tb_new = qem_T_trial(ws, q10, iw, is, iq, +1, sc, level);
-    }
-    const double d = qem_delta(pw, ta, tb, Ta_new, Ta, Tb_new, Tb);
-    if (target == kNull || d < best_d || (d == best_d && b < target)) {
-      best_d = d;
-      best_m = qem_mag(pw, ta, tb, Ta_new, Ta, Tb_new, Tb);
+    if (tb < best || (tb == best && b < target)) {
+      best = tb;
       target = b;
     }
   }
   *target_out = target;
-  return target != kNull && qem_accept(best_d, best_m);
+  return target != kNull && iso_accept(ta, best);
 }
 
 // Per-mesh convergence state, advanced once per pass by the controller with
