@@ -51,7 +51,6 @@ struct Ctrl {
   int32_t work_count[kSlices];  // items to evaluate in this pass, per slice
   int32_t cand_count[kSlices];  // proposals of this pass, per slice
   int32_t mods[kSlices];        // applied moves of this pass (single-mesh contexts)
-  int32_t pend_count[kMatchRounds][kSlices];  // constrained QEM items awaiting solves, per round
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
@@ -89,11 +88,6 @@ struct SweepArgs {
   int32_t cand_cap;            // capacity of one candidate-list slice
   Ctrl* ctrl;
   int32_t* cand;               // proposals of this pass (kSlices slices)
-  int32_t* pend;               // items awaiting QEM solves in the current round (kSlices slices)
-  int32_t pend_cap;            // capacity of one pending-list slice
-  int32_t pad3;
-  double* res_leave;           // per item: T of its cluster without it
-  double* res_join;            // per CSR position: T of that neighbour's cluster with the item
   // scan tiles
   const int32_t* tile_items;   // n_tiles * kTileItems item ids, -1 padding
   const int32_t* item_tile;    // tile of each item
@@ -133,7 +127,6 @@ hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbo
 hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid);
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round);
 hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid, int round);
-hipError_t launch_qem_solve_choose(hipStream_t s, const SweepArgs& a, int grid, int round);
 hipError_t launch_controller(hipStream_t s, const SweepArgs& a);
 hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* pending);
 hipError_t launch_cc(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* parent,

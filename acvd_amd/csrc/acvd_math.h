@@ -391,9 +391,9 @@ ACVD_HD void load_acc(const SweepView& v, int32_t c, int64_t* ws, int64_t* q10) 
 }
 
 // ---------------------------------------------------------------------------
-// QEM evaluation steps.  QemItem holds what the choice needs besides the
-// solves; everything is read from the state frozen for the current round, so
-// recomputing it in another kernel gives the same bits.
+// QEM evaluation steps: screen (guards and exact prune), FP64 solves, choice.
+// QemItem holds what the choice needs besides the solves; everything is read
+// from the state frozen for the current round.
 struct QemItem {
   float px, py, pz, pw;
   float ta;      // leave term of the item's cluster
@@ -531,8 +531,7 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
     return target != kNull;
   }
   if (METRIC == 1) {
-    // QEM: screen, solves, choice (the device runs the three steps in
-    // separate kernels so that the FP64 solves of one item run on parallel lanes)
+    // QEM: screen, solves, choice
     QemItem s;
     if (!qem_screen(v, i, lab, connexity, constrained, &s)) return false;
     const double Ta_new = constrained ? qem_leave_T(v, i, lab, sc, level) : 0.0;

@@ -103,9 +103,8 @@ struct acvd_ctx {
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T;
   DBuf<int32_t> d_cand_target, d_cand_round, d_cl_round;
-  DBuf<int32_t> d_work, d_cand, d_pend;
-  DBuf<double> d_res_leave, d_res_join;
-  int32_t slice_cap = 0, cand_cap = 0, pend_cap = 0;
+  DBuf<int32_t> d_work, d_cand;
+  int32_t slice_cap = 0, cand_cap = 0;
   // scan tiles
   int32_t n_tiles = 0;
   DBuf<int32_t> d_tile_items, d_item_tile, d_tile_mesh, d_tile_cnt, d_tile_set, d_tile_prop;
@@ -168,10 +167,6 @@ struct acvd_ctx {
     a.slice_cap = slice_cap;
     a.cand_cap = cand_cap;
     a.cand = d_cand.p;
-    a.pend = d_pend.p;
-    a.pend_cap = pend_cap;
-    a.res_leave = d_res_leave.p;
-    a.res_join = d_res_join.p;
     a.ctrl = d_ctrl.p;
     a.tile_items = d_tile_items.p;
     a.item_tile = d_item_tile.p;
@@ -452,19 +447,8 @@ int build(acvd_ctx* c) {
       return ((grid + kSlices - 1) / kSlices) * steps * kBlock;
     };
     const int64_t grid = sweep_grid_for(N);
-    const int64_t rgrid = std::min<int64_t>(grid, 1024);
-    // round-0 candidates are appended by the evaluate grid (direct proposals)
-    // and by the round grid (QEM choices): a slice holds the bound of both
-    c->cand_cap = int32_t(slice_bound(grid) + slice_bound(rgrid));
+    c->cand_cap = int32_t(slice_bound(grid));
     HIP_TRY(c->d_cand.alloc(size_t(c->cand_cap) * kSlices));
-    // pending lists are filled by the evaluate grid (round 0) or by the
-    // round grid (later rounds)
-    c->pend_cap = int32_t(std::max(slice_bound(grid), slice_bound(rgrid)));
-    if (c->any_qem) {
-      HIP_TRY(c->d_pend.alloc(size_t(c->pend_cap) * kSlices));
-      HIP_TRY(c->d_res_leave.alloc(N));
-      HIP_TRY(c->d_res_join.alloc(std::max<int64_t>(c->E2, 1)));
-    }
   }
   HIP_TRY(c->d_parent.alloc(N));
   HIP_TRY(c->d_comp_size.alloc(N));
@@ -498,7 +482,7 @@ int build(acvd_ctx* c) {
 int sweep_grid(const acvd_ctx* c) { return int(sweep_grid_for(c->N)); }
 
 // grid of the accept rounds (grid-stride over the candidate list)
-int round_grid(const acvd_ctx* c) { return std::min(sweep_grid(c), 1024); }  // as rgrid in build()
+int round_grid(const acvd_ctx* c) { return std::min(sweep_grid(c), 1024); }
 
 int fill_until_stable(acvd_ctx* c, const int32_t* mask, int max_rounds) {
   hipStream_t s = c->stream;
@@ -531,11 +515,9 @@ int enqueue_chunk(acvd_ctx* c) {
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
-    if (c->any_qem) HIP_TRY(launch_qem_solve_choose(s, a, rgrid, 0));
     HIP_TRY(launch_apply(s, a, rgrid, 0));
     for (int r = 1; r < kMatchRounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, rgrid, r));
-      if (c->any_qem) HIP_TRY(launch_qem_solve_choose(s, a, rgrid, r));
       HIP_TRY(launch_apply(s, a, rgrid, r));
     }
     HIP_TRY(launch_controller(s, a));
@@ -555,12 +537,10 @@ int run_chunk_timed(acvd_ctx* c) {
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
-    if (c->any_qem) HIP_TRY(launch_qem_solve_choose(s, a, round_grid(c), 0));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
     HIP_TRY(launch_apply(s, a, round_grid(c), 0));
     for (int r = 1; r < kMatchRounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, round_grid(c), r));
-      if (c->any_qem) HIP_TRY(launch_qem_solve_choose(s, a, round_grid(c), r));
       HIP_TRY(launch_apply(s, a, round_grid(c), r));
     }
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));

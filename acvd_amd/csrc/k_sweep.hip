@@ -313,19 +313,6 @@ __device__ inline void cand_append(const SweepArgs& a, int slice, bool want, int
   if (want) a.cand[int64_t(slice) * a.cand_cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = i;
 }
 
-// Wave-aggregated append of item i (when want) to slice `slice` of the
-// pending list of round `round` (constrained QEM items waiting for solves).
-__device__ inline void pend_append(const SweepArgs& a, int round, int slice, bool want, int32_t i) {
-  const unsigned long long bal = __ballot(want);
-  if (bal == 0ull) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)bal) - 1;
-  int32_t base = 0;
-  if (lane == leader) base = atomicAdd(&a.ctrl->pend_count[round][slice], __popcll(bal));
-  base = __shfl(base, leader);
-  if (want) a.pend[int64_t(slice) * a.pend_cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = i;
-}
-
 // Round-0 bookkeeping of a proposal: round-0 key at both clusters, candidate
 // state, tile activity, per-mesh proposal count (batches).
 __device__ inline void register_round0(const SweepArgs& a, MeshState* st, const DevMesh& dm, int64_t i, int32_t lab,
@@ -343,10 +330,7 @@ __device__ inline void register_round0(const SweepArgs& a, MeshState* st, const 
 
 // Evaluate (round 0): best move of each work item (shared rule of
 // acvd_math.h) against the state at the pass start; a proposal registers its
-// round-0 accept key at both clusters and joins the candidate list.  A
-// labelled item of a constrained QEM mesh that passes the screen goes to the
-// pending list instead: its FP64 solves run in k_qem_solve and its choice in
-// k_qem_choose.
+// round-0 accept key at both clusters and joins the candidate list.
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
   __shared__ SliceIndex wi;
@@ -356,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
   const int slice = blockIdx.x % kSlices;
   for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
     const int32_t t = base + int32_t(threadIdx.x);
-    bool want = false, pend = false;
+    bool want = false;
     int64_t i = 0;
     if (t < n) {
       i = slice_entry(a.work, a.slice_cap, &wi, t);
@@ -366,145 +350,52 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
         MeshState* st = &a.state[m];
         const int32_t lab = a.labels[i];
         int32_t target = kNull;
-        if (METRIC == 1 && st->constrained && lab != kNull) {
-          QemItem s;
-          pend = qem_screen(v, i, lab, st->connexity, 1, &s);
-        } else if (propose_item<METRIC>(v, i, lab, st->connexity, st->constrained, dm.level, dm.scales, &target)) {
+        if (propose_item<METRIC>(v, i, lab, st->connexity, st->constrained, dm.level, dm.scales, &target)) {
           register_round0(a, st, dm, i, lab, target);
           want = true;
         }
       }
     }
     cand_append(a, slice, want, int32_t(i));
-    if (METRIC == 1) pend_append(a, 0, slice, pend, int32_t(i));
   }
 }
 
-// FP64 solves of the pending items of a round: four lanes per item, lane 0
-// the item's own cluster without it, lanes 1-3 its open targets with it
-// (every third one each).  Results go to res_leave[item] and res_join[CSR
-// position of the target].
-__global__ __launch_bounds__(kBlock) void k_qem_solve(SweepArgs a, int round) {
-  __shared__ SliceIndex pi;
-  load_slice_index(a.ctrl->pend_count[round], &pi);
-  const int32_t n = pi.start[kSlices];
-  const SweepView v = make_view(a);
-  const int lane4 = threadIdx.x & 3;
-  const int32_t groups = int32_t(gridDim.x * kBlock / 4);
-  for (int32_t p = int32_t((blockIdx.x * kBlock + threadIdx.x) >> 2); p < n; p += groups) {
-    const int64_t i = slice_entry(a.pend, a.pend_cap, &pi, p);
-    const int m = mesh_of_item(a, i);
-    const DevMesh& dm = a.meshes[m];
-    const int32_t lab = a.labels[i];
-    if (lane4 == 0) {
-      a.res_leave[i] = qem_leave_T(v, i, lab, dm.scales, dm.level);
-      continue;
-    }
-    QemItem s;
-    qem_item_terms(v, i, lab, 1, &s);
-    int j = 0;
-    for (int32_t k = qem_next_open(v, lab, s, 1, s.r0 - 1); k < s.r1; k = qem_next_open(v, lab, s, 1, k), ++j)
-      if (j % 3 == lane4 - 1) a.res_join[k] = qem_join_T(v, i, a.labels[a.col[k]], dm.scales, dm.level);
-  }
-}
-
-// Choice of the pending items of a round from the solves (qem_choose of
-// acvd_math.h, the same arithmetic as the sequential rule).  Round 0: a
-// proposal registers like k_eval's; later rounds: the candidate takes its new
-// target and round key, or is dropped.
-__global__ __launch_bounds__(kBlock) void k_qem_choose(SweepArgs a, int round) {
-  __shared__ SliceIndex pi;
-  load_slice_index(a.ctrl->pend_count[round], &pi);
-  const int32_t n = pi.start[kSlices];
-  const SweepView v = make_view(a);
-  const int slice = blockIdx.x % kSlices;
-  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-    const int32_t t = base + int32_t(threadIdx.x);
-    bool want = false;
-    int64_t i = 0;
-    if (t < n) {
-      i = slice_entry(a.pend, a.pend_cap, &pi, t);
-      const int m = mesh_of_item(a, i);
-      const DevMesh& dm = a.meshes[m];
-      MeshState* st = &a.state[m];
-      const int32_t lab = a.labels[i];
-      QemItem s;
-      qem_item_terms(v, i, lab, 1, &s);
-      int32_t target = kNull;
-      const double* rj = a.res_join;
-      const bool ok =
-          qem_choose(v, lab, s, 1, a.res_leave[i], [rj](int32_t k, int32_t) { return rj[k]; }, &target);
-      if (round == 0) {
-        if (ok) {
-          register_round0(a, st, dm, i, lab, target);
-          want = true;
-        }
-      } else if (ok) {
-        const unsigned long long key = accept_key(dm.seed, st->loops, round, uint32_t(i - dm.item_off));
-        atomicMin(&a.prio[lab], key);
-        atomicMin(&a.prio[target], key);
-        a.cand_target[i] = target;
-      } else {
-        a.cand_target[i] = -3;
-      }
-    }
-    if (round == 0) cand_append(a, slice, want, int32_t(i));
-  }
-}
-
-// Re-evaluate (round r > 0): each live candidate (cand_target >= 0) is
-// evaluated again against the state left by round r-1; it either registers
-// its round-r key with its (possibly new) target or is dropped (-3).  A
-// candidate none of whose clusters (its own and its neighbours' labels) was
-// modified since its last evaluation would get the same answer, so it only
-// registers its key again.  Constrained QEM candidates that pass the screen
-// go to the round's pending list (solves and choice in the next kernels).
+// Re-evaluate (round r > 0): a live candidate (cand_target >= 0) whose own
+// cluster or target moved since its last evaluation is evaluated again
+// against the state left by round r-1: it registers its round-r key with its
+// (possibly new) target or is dropped (-3).  The others keep their move, whose
+// dE depends only on the two unchanged clusters, and register it again.
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ SliceIndex ci;
   load_slice_index(a.ctrl->cand_count, &ci);
   const int32_t n = ci.start[kSlices];
   const SweepView v = make_view(a);
-  const int slice = blockIdx.x % kSlices;
-  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-    const int32_t t = base + int32_t(threadIdx.x);
-    bool pend = false;
-    int64_t i = 0;
-    if (t < n) {
-      i = slice_entry(a.cand, a.cand_cap, &ci, t);
-      const int m = mesh_of_item(a, i);
-      const DevMesh& dm = a.meshes[m];
-      if (a.cand_target[i] >= 0 && dm.metric == METRIC) {
-        const MeshState& st = a.state[m];
-        const int32_t lab = a.labels[i];
-        const int32_t since = a.cand_round[i];
-        bool stale = lab != kNull && a.cl_round[lab] >= since;
-        for (int32_t k = a.row_ptr[i]; !stale && k < a.row_ptr[i + 1]; ++k) {
-          const int32_t b = a.labels[a.col[k]];
-          stale = b != kNull && a.cl_round[b] >= since;
-        }
-        const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
-        if (!stale) {
-          if (lab != kNull) atomicMin(&a.prio[lab], key);
-          atomicMin(&a.prio[a.cand_target[i]], key);
-        } else {
-          a.cand_round[i] = st.loops * kMatchRounds + round;
-          int32_t target = kNull;
-          if (METRIC == 1 && st.constrained && lab != kNull) {
-            QemItem s;
-            pend = qem_screen(v, i, lab, st.connexity, 1, &s);
-            if (!pend) a.cand_target[i] = -3;
-          } else if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
-            if (lab != kNull) atomicMin(&a.prio[lab], key);
-            atomicMin(&a.prio[target], key);
-            a.cand_target[i] = target;
-          } else {
-            a.cand_target[i] = -3;
-          }
-        }
-      }
+  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const int64_t i = slice_entry(a.cand, a.cand_cap, &ci, t);
+    if (a.cand_target[i] < 0) continue;
+    const int m = mesh_of_item(a, i);
+    const DevMesh& dm = a.meshes[m];
+    if (dm.metric != METRIC) continue;
+    const MeshState& st = a.state[m];
+    const int32_t lab = a.labels[i];
+    const int32_t since = a.cand_round[i];
+    const bool stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[a.cand_target[i]] >= since;
+    const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
+    if (!stale) {
+      if (lab != kNull) atomicMin(&a.prio[lab], key);
+      atomicMin(&a.prio[a.cand_target[i]], key);
+      continue;
     }
-    if (METRIC == 1) pend_append(a, round, slice, pend, int32_t(i));
+    int32_t target = kNull;
+    a.cand_round[i] = st.loops * kMatchRounds + round;
+    if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
+      if (lab != kNull) atomicMin(&a.prio[lab], key);
+      atomicMin(&a.prio[target], key);
+      a.cand_target[i] = target;
+    } else {
+      a.cand_target[i] = -3;
+    }
   }
 }
 
@@ -652,7 +543,6 @@ __global__ void k_controller(SweepArgs a) {
     a.ctrl->work_count[threadIdx.x] = 0;
     a.ctrl->cand_count[threadIdx.x] = 0;
     a.ctrl->mods[threadIdx.x] = 0;
-    for (int r = 0; r < kMatchRounds; ++r) a.ctrl->pend_count[r][threadIdx.x] = 0;
   }
   if (threadIdx.x == 0) {
     a.ctrl->evaluated += (unsigned long long)s_work;
@@ -896,12 +786,6 @@ hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any
 
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round) {
   hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, round);
-  return hipGetLastError();
-}
-
-hipError_t launch_qem_solve_choose(hipStream_t s, const SweepArgs& a, int grid, int round) {
-  hipLaunchKernelGGL(k_qem_solve, dim3(grid), dim3(kBlock), 0, s, a, round);
-  hipLaunchKernelGGL(k_qem_choose, dim3(grid), dim3(kBlock), 0, s, a, round);
   return hipGetLastError();
 }
 

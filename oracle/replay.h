@@ -37,7 +37,7 @@ struct Replay {
   const float* xw = nullptr;
   const float* q = nullptr;
 
-  std::vector<int32_t> labels, sizes, stamp, cand_pass, pending;
+  std::vector<int32_t> labels, sizes, stamp, cand_pass, pending, cl_round;
   std::vector<unsigned long long> acc_ws, acc_q;
   std::vector<acvd::ClusterGeom> geom;
   std::vector<double> T;
@@ -123,7 +123,7 @@ struct Replay {
     derive(c);
   }
 
-  void apply_move(int64_t i, int32_t lab, int32_t b, int32_t L) {
+  void apply_move(int64_t i, int32_t lab, int32_t b, int32_t L, int32_t rtag) {
     labels[i] = b;
     const float w = xw[4 * i + 3];
     const int64_t iw = acvd::fix_w(w, sc);
@@ -136,10 +136,12 @@ struct Replay {
       update(lab, -1, iw, is, iq);
       sizes[lab]--;
       stamp[lab] = L;
+      cl_round[lab] = rtag;
     }
     update(b, +1, iw, is, iq);
     sizes[b]++;
     stamp[b] = L;
+    cl_round[b] = rtag;
   }
 
   bool propose(int64_t i, int32_t* target) {
@@ -167,11 +169,18 @@ struct Replay {
     for (int64_t i : cand) cand_pass[i] = L;
     int mods = 0;
     std::vector<int32_t> from(cand.size());
+    std::vector<int32_t> since(cand.size(), L * acvd::kMatchRounds);
     for (int round = 0; round < acvd::kMatchRounds; ++round) {
       if (round > 0) {
-        // live proposals are re-evaluated against the state left by the last round
+        // a live proposal whose own cluster or target moved since its last
+        // evaluation is evaluated again against the state left by the last
+        // round; the others keep their (still exact) move
         for (size_t t = 0; t < cand.size(); ++t) {
           if (target[t] < 0) continue;
+          const int32_t lab = labels[cand[t]];
+          const bool stale = (lab != acvd::kNull && cl_round[lab] >= since[t]) || cl_round[target[t]] >= since[t];
+          if (!stale) continue;
+          since[t] = L * acvd::kMatchRounds + round;
           int32_t nt = acvd::kNull;
           target[t] = propose(cand[t], &nt) ? nt : -3;
         }
@@ -190,7 +199,7 @@ struct Replay {
         const uint64_t key = acvd::accept_key(seed, L, round, uint32_t(cand[t]));
         if (from[t] != acvd::kNull && prio[from[t]] != key) continue;
         if (prio[target[t]] != key) continue;
-        apply_move(cand[t], from[t], target[t], L);
+        apply_move(cand[t], from[t], target[t], L, L * acvd::kMatchRounds + round);
         target[t] = -3;
         mods++;
       }
@@ -254,6 +263,7 @@ struct Replay {
     sizes.assign(K, 0);
     stamp.assign(K, 0);
     cand_pass.assign(N, -2);
+    cl_round.assign(K, -1);
     pending.assign(N, acvd::kNull);
     acc_ws.assign(4 * size_t(K), 0ull);
     acc_q.assign(metric == 1 ? 10 * size_t(K) : 0, 0ull);
