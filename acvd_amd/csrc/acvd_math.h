@@ -29,9 +29,11 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define ACVD_HD __host__ __device__ inline
+#define ACVD_UNROLL _Pragma("unroll")
 #else
 #include <cmath>
 #define ACVD_HD inline
+#define ACVD_UNROLL _Pragma("GCC unroll 16")
 #endif
 
 namespace acvd {
@@ -311,7 +313,54 @@ ACVD_HD double qem_T_trial(const int64_t* ws, const int64_t* q10, int64_t iw, co
 // neighbours are connected only through mesh edges between themselves?
 // rp/col is the CSR adjacency with each row sorted by neighbour id.  Rings
 // with more than 64 neighbours are refused (treated as a problem).
+ACVD_HD bool ring_problem_rows(const int32_t* rp, const int32_t* col, const int32_t* lab, int32_t i, int32_t a);
+
+// Rings of at most kRingCache neighbours whose neighbour rows are also that
+// short (the usual case) are checked with all row entries loaded at once into
+// registers and an all-pairs intersection (the rows hold distinct ids, so it
+// finds exactly what the merge-walk of sorted rows finds), which replaces the
+// chain of dependent loads of the walk; longer rows take the walk.
+constexpr int kRingCache = 12;
 ACVD_HD bool ring_problem(const int32_t* rp, const int32_t* col, const int32_t* lab, int32_t i, int32_t a) {
+  const int32_t r0 = rp[i], r1 = rp[i + 1];
+  const int32_t deg = r1 - r0;
+  if (deg > kRingCache) return ring_problem_rows(rp, col, lab, i, a);
+  int32_t ci[kRingCache];
+ACVD_UNROLL
+  for (int p = 0; p < kRingCache; ++p) ci[p] = p < deg ? col[r0 + p] : -1;
+  uint64_t M = 0;
+ACVD_UNROLL
+  for (int p = 0; p < kRingCache; ++p)
+    if (p < deg && lab[ci[p]] == a) M |= uint64_t(1) << p;
+  if (M == 0) return false;
+  uint64_t reach = M & (~M + 1);
+  uint64_t frontier = reach;
+  while (frontier) {
+    const int t = __builtin_ctzll(frontier);
+    frontier &= frontier - 1;
+    const int32_t j = col[r0 + t];
+    const int32_t s0 = rp[j], len = rp[j + 1] - s0;
+    if (len > kRingCache) return ring_problem_rows(rp, col, lab, i, a);
+    int32_t cj[kRingCache];
+ACVD_UNROLL
+    for (int u = 0; u < kRingCache; ++u) cj[u] = u < len ? col[s0 + u] : -2;
+    uint64_t hit = 0;
+ACVD_UNROLL
+    for (int p = 0; p < kRingCache; ++p) {
+      bool h = false;
+ACVD_UNROLL
+      for (int u = 0; u < kRingCache; ++u) h |= (ci[p] == cj[u]);
+      if (p < deg && h) hit |= uint64_t(1) << p;
+    }
+    const uint64_t add = hit & M & ~reach;
+    reach |= add;
+    frontier |= add;
+  }
+  return reach != M;
+}
+
+// The same check with merge-walks of the sorted rows (any length up to 64).
+ACVD_HD bool ring_problem_rows(const int32_t* rp, const int32_t* col, const int32_t* lab, int32_t i, int32_t a) {
   const int32_t r0 = rp[i], r1 = rp[i + 1];
   const int32_t deg = r1 - r0;
   if (deg > 64) return true;
@@ -502,6 +551,51 @@ ACVD_HD bool qem_choose(const SweepView& v, int32_t lab, const QemItem& s, int c
   return target != kNull && qem_accept(best_d, best_m);
 }
 
+// The QEM rule for rows of at most kRingCache neighbours with the neighbour
+// labels, join terms and cluster T read once into registers (independent
+// loads, no chains through the row): the same tests in the same CSR order as
+// qem_screen / qem_choose, so it returns the same move.
+ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained,
+                                   int level, const Scales& sc, int32_t* target_out) {
+  const int32_t r0 = v.row_ptr[i], deg = v.row_ptr[i + 1] - r0;
+  if (v.sizes[lab] <= 1) return false;
+  if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
+  if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
+  QemItem s;
+  qem_item_terms(v, i, lab, constrained, &s);
+  int32_t nb[kRingCache];
+  ACVD_UNROLL
+  for (int p = 0; p < kRingCache; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
+  uint32_t open = 0;
+  ACVD_UNROLL
+  for (int p = 0; p < kRingCache; ++p) {
+    bool first = nb[p] != kNull && nb[p] != lab;
+    ACVD_UNROLL
+    for (int t = 0; t < p; ++t) first = first && nb[t] != nb[p];
+    if (first && qem_open(v, s, nb[p], constrained)) open |= 1u << p;
+  }
+  if (!open) return false;
+  const double Ta_new = constrained ? qem_leave_T(v, i, lab, sc, level) : 0.0;
+  int32_t target = kNull;
+  double best_d = 0.0, best_m = 0.0;
+  while (open) {
+    const int p = __builtin_ctz(open);
+    open &= open - 1;
+    const int32_t b = v.labels[v.col[r0 + p]];
+    const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
+    const double Tb = constrained ? v.T[b] : 0.0;
+    const double Tb_new = constrained ? qem_join_T(v, i, b, sc, level) : 0.0;
+    const double d = qem_delta(s.pw, s.ta, tb, Ta_new, s.Ta, Tb_new, Tb);
+    if (target == kNull || d < best_d || (d == best_d && b < target)) {
+      best_d = d;
+      best_m = qem_mag(s.pw, s.ta, tb, Ta_new, s.Ta, Tb_new, Tb);
+      target = b;
+    }
+  }
+  *target_out = target;
+  return target != kNull && qem_accept(best_d, best_m);
+}
+
 // Best move of an evaluated item (label lab, loop L).  Returns true and the
 // target cluster if the move is proposed.
 //  - null item: joins the neighbouring cluster with the smallest join term
@@ -531,6 +625,8 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
     return target != kNull;
   }
   if (METRIC == 1) {
+    if (r1 - r0 <= kRingCache)
+      return qem_propose_short_row(v, i, lab, connexity, constrained, level, sc, target_out);
     // QEM: screen, solves, choice
     QemItem s;
     if (!qem_screen(v, i, lab, connexity, constrained, &s)) return false;
