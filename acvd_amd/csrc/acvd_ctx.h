@@ -51,6 +51,8 @@ struct Ctrl {
   int32_t work_count[kSlices];  // items to evaluate in this pass, per slice
   int32_t cand_count[kSlices];  // proposals of this pass, per slice
   int32_t mods[kSlices];        // applied moves of this pass (single-mesh contexts)
+  int32_t moved_count[kSlices]; // items moved in this pass
+  int32_t dirty_count[kMatchRounds][kSlices];  // clusters modified, per round
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
@@ -88,6 +90,10 @@ struct SweepArgs {
   int32_t cand_cap;            // capacity of one candidate-list slice
   Ctrl* ctrl;
   int32_t* cand;               // proposals of this pass (kSlices slices)
+  int32_t* dirty;              // clusters modified by the current round (kSlices slices)
+  int32_t* moved;              // items moved in this pass (kSlices slices)
+  int32_t dirty_cap;
+  int32_t moved_cap;
   // scan tiles
   const int32_t* tile_items;   // n_tiles * kTileItems item ids, -1 padding
   const int32_t* item_tile;    // tile of each item
@@ -126,6 +132,7 @@ hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbo
                              int32_t* item_tile, void** tmp, size_t* tmp_bytes);
 hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid);
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round);
+hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a, int grid);
 hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid, int round);
 hipError_t launch_controller(hipStream_t s, const SweepArgs& a);
 hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* pending);

@@ -103,8 +103,8 @@ struct acvd_ctx {
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T;
   DBuf<int32_t> d_cand_target, d_cand_round, d_cl_round;
-  DBuf<int32_t> d_work, d_cand;
-  int32_t slice_cap = 0, cand_cap = 0;
+  DBuf<int32_t> d_work, d_cand, d_dirty, d_moved;
+  int32_t slice_cap = 0, cand_cap = 0, dirty_cap = 0, moved_cap = 0;
   // scan tiles
   int32_t n_tiles = 0;
   DBuf<int32_t> d_tile_items, d_item_tile, d_tile_mesh, d_tile_cnt, d_tile_set, d_tile_prop;
@@ -167,6 +167,10 @@ struct acvd_ctx {
     a.slice_cap = slice_cap;
     a.cand_cap = cand_cap;
     a.cand = d_cand.p;
+    a.dirty = d_dirty.p;
+    a.moved = d_moved.p;
+    a.dirty_cap = dirty_cap;
+    a.moved_cap = moved_cap;
     a.ctrl = d_ctrl.p;
     a.tile_items = d_tile_items.p;
     a.item_tile = d_item_tile.p;
@@ -449,6 +453,13 @@ int build(acvd_ctx* c) {
     const int64_t grid = sweep_grid_for(N);
     c->cand_cap = int32_t(slice_bound(grid));
     HIP_TRY(c->d_cand.alloc(size_t(c->cand_cap) * kSlices));
+    // apply blocks (round grid, grid-stride over at most N candidates) append
+    // at most one moved item and two dirty clusters per candidate they visit
+    const int64_t rgrid = std::min<int64_t>(grid, 1024);
+    c->moved_cap = int32_t(slice_bound(rgrid));
+    c->dirty_cap = int32_t(2 * slice_bound(rgrid));
+    HIP_TRY(c->d_moved.alloc(size_t(c->moved_cap) * kSlices));
+    HIP_TRY(c->d_dirty.alloc(size_t(c->dirty_cap) * kSlices));
   }
   HIP_TRY(c->d_parent.alloc(N));
   HIP_TRY(c->d_comp_size.alloc(N));
@@ -520,6 +531,7 @@ int enqueue_chunk(acvd_ctx* c) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, rgrid, r));
       HIP_TRY(launch_apply(s, a, rgrid, r));
     }
+    HIP_TRY(launch_tile_update(s, a, rgrid));
     HIP_TRY(launch_controller(s, a));
   }
   return ACVD_OK;
@@ -543,6 +555,7 @@ int run_chunk_timed(acvd_ctx* c) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, round_grid(c), r));
       HIP_TRY(launch_apply(s, a, round_grid(c), r));
     }
+    HIP_TRY(launch_tile_update(s, a, round_grid(c)));
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
     HIP_TRY(launch_controller(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 4], s));

@@ -412,23 +412,6 @@ __device__ inline void tile_insert(const SweepArgs& a, int32_t t, int32_t c) {
   if (slot < kTileSet) a.tile_set[int64_t(t) * kTileSet + slot] = c;
 }
 
-__device__ void update_cluster(const SweepArgs& a, int64_t c, int sign, int64_t iw, const int64_t* is,
-                               const int64_t* iq, const DevMesh& dm, int constrained) {
-  const unsigned long long s = (unsigned long long)int64_t(sign);
-  int64_t W = int64_t(atomicAdd(&a.acc_ws[4 * c], (unsigned long long)iw * s)) + sign * iw;
-  int64_t Sx = int64_t(atomicAdd(&a.acc_ws[4 * c + 1], (unsigned long long)is[0] * s)) + sign * is[0];
-  int64_t Sy = int64_t(atomicAdd(&a.acc_ws[4 * c + 2], (unsigned long long)is[1] * s)) + sign * is[1];
-  int64_t Sz = int64_t(atomicAdd(&a.acc_ws[4 * c + 3], (unsigned long long)is[2] * s)) + sign * is[2];
-  const ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, dm.scales);
-  a.geom[c] = g;
-  if (dm.metric == 1) {
-    int64_t q10[10];
-    for (int k = 0; k < 10; ++k)
-      q10[k] = int64_t(atomicAdd(&a.acc_q[10 * c + k], (unsigned long long)iq[k] * s)) + sign * iq[k];
-    a.T[c] = constrained ? qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr) : 0.0;
-  }
-}
-
 __device__ inline int block_sum(int v) {
   __shared__ int s_red[kBlock / 64];
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -441,63 +424,120 @@ __device__ inline int block_sum(int v) {
   return tot;  // valid in thread 0
 }
 
+// Wave-aggregated append of v (when want) to slice `slice` of a sliced list.
+__device__ inline void list_append(int32_t* list, int32_t cap, int32_t* counts, int slice, bool want, int32_t v) {
+  const unsigned long long bal = __ballot(want);
+  if (bal == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)bal) - 1;
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(&counts[slice], __popcll(bal));
+  base = __shfl(base, leader);
+  if (want) list[int64_t(slice) * cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = v;
+}
+
 // Apply round `round`: a live candidate holding the minimum round key at both
-// of its clusters moves.  Counts go to the sliced counters (single mesh) or
-// per mesh (batches).
+// of its clusters moves: label, fixed-point accumulator deltas (no returned
+// values needed: the round owns both clusters), sizes and stamps.  The two
+// clusters join the round's dirty list (their centroid and T are derived by
+// k_derive_dirty) and the item joins the pass's moved list (tile sets are
+// extended by k_tile_update).  Counts go to the sliced counters (single mesh)
+// or per mesh (batches).
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ SliceIndex ci;
   load_slice_index(a.ctrl->cand_count, &ci);
   const int32_t n = ci.start[kSlices];
+  const int slice = blockIdx.x % kSlices;
   int mods = 0;
   for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
     const int32_t t = base + int32_t(threadIdx.x);
-    if (t >= n) continue;
-    const int64_t i = slice_entry(a.cand, a.cand_cap, &ci, t);
-    const int32_t b = a.cand_target[i];
-    if (b < 0) continue;
-    const int m = mesh_of_item(a, i);
-    MeshState* st = &a.state[m];
-    const int32_t L = st->loops;
-    const int32_t lab = a.labels[i];
-    const DevMesh dm = a.meshes[m];
-    const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
-    if (lab != kNull && a.prio[lab] != key) continue;
-    if (a.prio[b] != key) continue;
-    // exclusive owner of clusters lab and b for this round
-    a.labels[i] = b;
-    a.cand_target[i] = -3;
-    const float4 p = a.xw[i];
-    const Scales sc = dm.scales;
-    const int64_t iw = fix_w(p.w, sc);
-    const int64_t is[3] = {fix_s(p.w, p.x, sc), fix_s(p.w, p.y, sc), fix_s(p.w, p.z, sc)};
-    int64_t iq[10];
-    if (dm.metric == 1)
-      for (int k = 0; k < 10; ++k) iq[k] = fix_q(a.q[10 * i + k], sc);
-    const int constrained = st->constrained;
-    const int32_t rtag = L * kMatchRounds + round;
-    if (lab != kNull) {
-      update_cluster(a, lab, -1, iw, is, iq, dm, constrained);
-      a.sizes[lab] -= 1;
-      a.stamp[lab] = L;
-      a.cl_round[lab] = rtag;
+    bool win = false;
+    int64_t i = 0;
+    int32_t lab = kNull, b = kNull;
+    if (t < n) {
+      i = slice_entry(a.cand, a.cand_cap, &ci, t);
+      b = a.cand_target[i];
+      if (b >= 0) {
+        const int m = mesh_of_item(a, i);
+        MeshState* st = &a.state[m];
+        const int32_t L = st->loops;
+        lab = a.labels[i];
+        const DevMesh& dm = a.meshes[m];
+        const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
+        win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
+        if (win) {
+          // exclusive owner of clusters lab and b for this round
+          a.labels[i] = b;
+          a.cand_target[i] = -3;
+          const float4 p = a.xw[i];
+          const Scales sc = dm.scales;
+          const unsigned long long iw = (unsigned long long)fix_w(p.w, sc);
+          const unsigned long long is[3] = {(unsigned long long)fix_s(p.w, p.x, sc),
+                                            (unsigned long long)fix_s(p.w, p.y, sc),
+                                            (unsigned long long)fix_s(p.w, p.z, sc)};
+          const int32_t rtag = L * kMatchRounds + round;
+          if (lab != kNull) {
+            atomicSub(&a.acc_ws[4 * int64_t(lab)], iw);
+            for (int k = 0; k < 3; ++k) atomicSub(&a.acc_ws[4 * int64_t(lab) + 1 + k], is[k]);
+            a.sizes[lab] -= 1;
+            a.stamp[lab] = L;
+            a.cl_round[lab] = rtag;
+          }
+          atomicAdd(&a.acc_ws[4 * int64_t(b)], iw);
+          for (int k = 0; k < 3; ++k) atomicAdd(&a.acc_ws[4 * int64_t(b) + 1 + k], is[k]);
+          if (dm.metric == 1) {
+            for (int k = 0; k < 10; ++k) {
+              const unsigned long long iq = (unsigned long long)fix_q(a.q[10 * i + k], sc);
+              if (lab != kNull) atomicSub(&a.acc_q[10 * int64_t(lab) + k], iq);
+              atomicAdd(&a.acc_q[10 * int64_t(b) + k], iq);
+            }
+          }
+          a.sizes[b] += 1;
+          a.stamp[b] = L;
+          a.cl_round[b] = rtag;
+          ++mods;
+          if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
+        }
+      }
     }
-    update_cluster(a, b, +1, iw, is, iq, dm, constrained);
-    a.sizes[b] += 1;
-    a.stamp[b] = L;
-    a.cl_round[b] = rtag;
-    // item i now shows cluster b to its own tile and to its neighbours' tiles
+    list_append(a.dirty, a.dirty_cap, a.ctrl->dirty_count[round], slice, win && lab != kNull, lab);
+    list_append(a.dirty, a.dirty_cap, a.ctrl->dirty_count[round], slice, win, b);
+    list_append(a.moved, a.moved_cap, a.ctrl->moved_count, slice, win, int32_t(i));
+  }
+  if (!a.item_mesh) {
+    const int tm = block_sum(mods);
+    if (threadIdx.x == 0 && tm) atomicAdd(&a.ctrl->mods[blockIdx.x % kSlices], tm);
+  }
+}
+
+// Centroid, W and (QEM) T of the clusters a round modified, from their final
+// accumulators.
+__global__ __launch_bounds__(kBlock) void k_derive_dirty(SweepArgs a, int round) {
+  __shared__ SliceIndex di;
+  load_slice_index(a.ctrl->dirty_count[round], &di);
+  const int32_t n = di.start[kSlices];
+  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const int64_t c = slice_entry(a.dirty, a.dirty_cap, &di, t);
+    const int m = mesh_of_cluster(a, c);
+    derive_cluster(a, c, a.meshes[m], a.state[m].constrained);
+  }
+}
+
+// Tile sets after a pass: an item that moved to cluster b shows b to its own
+// tile and to the tiles of its neighbours (k_scan of the next pass reads them).
+__global__ __launch_bounds__(kBlock) void k_tile_update(SweepArgs a) {
+  __shared__ SliceIndex mi;
+  load_slice_index(a.ctrl->moved_count, &mi);
+  const int32_t n = mi.start[kSlices];
+  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const int64_t i = slice_entry(a.moved, a.moved_cap, &mi, t);
+    const int32_t b = a.labels[i];
     const int32_t ti = a.item_tile[i];
     tile_insert(a, ti, b);
     for (int32_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
       const int32_t tj = a.item_tile[a.col[k]];
       if (tj != ti) tile_insert(a, tj, b);
     }
-    ++mods;
-    if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
-  }
-  if (!a.item_mesh) {
-    const int tm = block_sum(mods);
-    if (threadIdx.x == 0 && tm) atomicAdd(&a.ctrl->mods[blockIdx.x % kSlices], tm);
   }
 }
 
@@ -543,6 +583,8 @@ __global__ void k_controller(SweepArgs a) {
     a.ctrl->work_count[threadIdx.x] = 0;
     a.ctrl->cand_count[threadIdx.x] = 0;
     a.ctrl->mods[threadIdx.x] = 0;
+    a.ctrl->moved_count[threadIdx.x] = 0;
+    for (int r = 0; r < kMatchRounds; ++r) a.ctrl->dirty_count[r][threadIdx.x] = 0;
   }
   if (threadIdx.x == 0) {
     a.ctrl->evaluated += (unsigned long long)s_work;
@@ -786,6 +828,12 @@ hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any
 
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round) {
   hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, round);
+  hipLaunchKernelGGL(k_derive_dirty, dim3(grid), dim3(kBlock), 0, s, a, round);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a, int grid) {
+  hipLaunchKernelGGL(k_tile_update, dim3(grid), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
