@@ -236,10 +236,32 @@ def run_batch_leg(args):
     n = len(meshes)
     log(f"batch: packed {batched_s * 1e3:.0f} ms wall ({batched_dev:.0f} ms device), one-by-one "
         f"{serial_s * 1e3:.0f} ms wall ({serial_dev:.0f} ms device); identical labels {same}/{n}")
-    return {"meshes": n, "vertices": tot, "meshes_per_s": round(n / batched_s, 2),
-            "serial_meshes_per_s": round(n / serial_s, 2), "ratio": round(serial_s / batched_s, 2),
-            "batched_device_ms": round(batched_dev, 1), "serial_device_ms": round(serial_dev, 1),
-            "identical_labels": same}
+    out = {"meshes": n, "vertices": tot, "meshes_per_s": round(n / batched_s, 2),
+           "serial_meshes_per_s": round(n / serial_s, 2), "ratio": round(serial_s / batched_s, 2),
+           "batched_device_ms": round(batched_dev, 1), "serial_device_ms": round(serial_dev, 1),
+           "identical_labels": same}
+    if args.batch_gap:
+        # energy gap against the serial reference restatement on a seeded
+        # subset of the meshes up to level 6 (the CPU runs stay short)
+        from oracle import pyoracle as po
+        rng = np.random.default_rng(44)
+        small = [j for j in range(n) if len(meshes[j][0]) <= 40962]
+        pick = sorted(rng.choice(small, min(args.batch_gap, len(small)), replace=False).tolist())
+        gaps = []
+        for j in pick:
+            xyz, tri, K, metric = meshes[j]
+            m = po.RefMesh(xyz, tri)
+            slab, _ = m.serial_run(metric, K, 0.0, None, 3, metric, inits[j])
+            es = m.energy(metric, K, slab)[0]
+            eg = m.energy(metric, K, ctx.labels(j))[0]
+            gaps.append((eg - es) / es * 100)
+        gaps = np.asarray(gaps)
+        log(f"batch gap vs serial on {len(pick)} meshes (<= 40,962 vertices): median {np.median(gaps):+.3f}% "
+            f"mean {gaps.mean():+.3f}% min {gaps.min():+.3f}% max {gaps.max():+.3f}%")
+        out["gap_vs_serial_pct"] = {"meshes": pick, "median": round(float(np.median(gaps)), 4),
+                                    "mean": round(float(gaps.mean()), 4), "min": round(float(gaps.min()), 4),
+                                    "max": round(float(gaps.max()), 4)}
+    return out
 
 
 def run_energy_leg(args):
@@ -347,6 +369,7 @@ def main():
     ap.add_argument("--gap", action="store_true", help="also run the serial ACVDQ restatement for the energy gap")
     ap.add_argument("--batch", action="store_true", help="also run the 384-mesh batched leg")
     ap.add_argument("--batch-count", type=int, default=384)
+    ap.add_argument("--batch-gap", type=int, default=0, help="serial-reference energy gap on this many batch meshes")
     ap.add_argument("--energy", action="store_true", help="also run the differentiable-energy leg (16 tori)")
     ap.add_argument("--energy-meshes", type=int, default=16)
     args = ap.parse_args()
