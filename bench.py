@@ -320,7 +320,12 @@ def run_energy_leg(args):
     torch.cuda.synchronize()
     fwd_ms = ev[0].elapsed_time(ev[1]) / reps
     fb_ms = ev[1].elapsed_time(ev[2]) / reps
-    # FD check on mesh 0 (1,000 seeded coordinates)
+    # checks on mesh 0 at 1,000 seeded coordinates: FP64 central differences
+    # (step 1e-6) and the oracle's dual-number derivative (exact at the point).
+    # The QEM energy is not differentiable where a cluster's kept eigen-set
+    # changes (threshold 1e-3): a coordinate whose difference quotients at
+    # steps 1e-6 and 1e-7 disagree by more than 1e-3 lies at such a point and is
+    # left out of the FD statistic (counted).
     x0 = xs[0].astype(np.float64)
     w0 = ws[0].astype(np.float64)
     ref = po.EnergyRef(len(x0), meshes[0][1], ls[0], K, 1, 3)
@@ -328,17 +333,28 @@ def run_energy_leg(args):
     n0 = len(x0)
     coords = rng.choice(4 * n0, 1000, replace=False)
     fd = ref.central_differences(x0, w0, coords, 1e-6)
+    fd7 = ref.central_differences(x0, w0, coords, 1e-7)
+    dual = ref.derivatives(x0, w0, coords)
     gg = np.concatenate([dx[:n0].cpu().numpy().reshape(-1), dw[:n0].cpu().numpy()]).astype(np.float64)
-    scale = np.abs(fd).max()
+    scale = np.abs(dual).max()
+    smooth = np.abs(fd - fd7) <= 1e-3 * np.maximum(np.abs(fd), 1e-3 * scale)
     err = np.abs(gg[coords] - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
+    err_dual = np.abs(gg[coords] - dual) / np.maximum(np.abs(dual), 1e-3 * scale)
     e_cpu0 = ref.total(x0, w0)
+    for k in np.argsort(-err_dual)[:3]:
+        c = int(coords[k])
+        what = f"x[{c // 3}][{c % 3}]" if c < 3 * n0 else f"w[{c - 3 * n0}]"
+        log(f"  coordinate {what}: gpu {gg[c]:.6e} dual {dual[k]:.6e} fd(1e-6) {fd[k]:.6e} fd(1e-7) {fd7[k]:.6e}")
     log(f"energy: forward {fwd_ms:.3f} ms, forward+backward {fb_ms:.3f} ms (ratio {fb_ms / fwd_ms:.2f}), "
-        f"{n} items; FD rel err median {np.median(err):.2e} max {err.max():.2e}; "
+        f"{n} items; FD rel err (smooth coords {int(smooth.sum())}/1000) median {np.median(err[smooth]):.2e} "
+        f"max {err[smooth].max():.2e}; vs dual median {np.median(err_dual):.2e} max {err_dual.max():.2e}; "
         f"E0 gpu {e[0].item():.9e} cpu {e_cpu0:.9e}")
     fwd_bytes = 24.0 * n
     return {"meshes": count, "items": n, "forward_ms": round(fwd_ms, 3), "forward_backward_ms": round(fb_ms, 3),
             "ratio": round(fb_ms / fwd_ms, 3), "forward_GBps_algorithmic": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9, 1),
-            "fd_rel_err_median": float(np.median(err)), "fd_rel_err_max": float(err.max()),
+            "fd_coords_smooth": int(smooth.sum()), "fd_rel_err_median": float(np.median(err[smooth])),
+            "fd_rel_err_max": float(err[smooth].max()), "dual_rel_err_median": float(np.median(err_dual)),
+            "dual_rel_err_max": float(err_dual.max()),
             "energy_rel_diff_vs_cpu": float(abs(e[0].item() - e_cpu0) / e_cpu0)}
 
 

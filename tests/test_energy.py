@@ -131,10 +131,20 @@ def test_gpu_adjoint_against_central_differences_torus(gpu_ready):
     rng = np.random.default_rng(5)
     coords = rng.choice(4 * len(x), 1000, replace=False)
     fd = ref.central_differences(x, w, coords, 1e-6)
-    scale = np.abs(fd).max()
+    fd7 = ref.central_differences(x, w, coords, 1e-7)
+    dual = ref.derivatives(x, w, coords)
+    scale = np.abs(dual).max()
+    # the QEM energy is not differentiable where a cluster's kept eigen-set
+    # changes; such coordinates show difference quotients that depend on the step
+    smooth = np.abs(fd - fd7) <= 1e-3 * np.maximum(np.abs(fd), 1e-3 * scale)
+    assert smooth.sum() >= 990
     err = np.abs(g[coords] - fd) / np.maximum(np.abs(fd), 1e-3 * scale)
-    print(f"FD check: median rel err {np.median(err):.2e}, 99% {np.quantile(err, 0.99):.2e}, max {err.max():.2e}")
-    assert np.quantile(err, 0.99) <= 1e-4 and err.max() <= 1e-3
+    print(f"FD check: {int(smooth.sum())} smooth coordinates, median rel err {np.median(err[smooth]):.2e}, "
+          f"max {err[smooth].max():.2e}")
+    assert np.quantile(err[smooth], 0.99) <= 1e-4 and err[smooth].max() <= 1e-3
+    # the dual-number derivative is exact at the point, smooth or not
+    err_dual = np.abs(g[coords] - dual) / np.maximum(np.abs(dual), 1e-3 * scale)
+    assert err_dual.max() <= 1e-4, err_dual.max()
 
 
 @pytest.mark.gpu
