@@ -226,6 +226,11 @@ int add_mesh(acvd_ctx* c, int64_t nv, const P* xyz, int64_t nf, const int32_t* t
   return ACVD_OK;
 }
 
+bool morton_tiles() {
+  const char* e = std::getenv("ACVD_TILE_ORDER");
+  return e && std::strcmp(e, "morton") == 0;
+}
+
 // eval grid (blocks of kBlock threads, grid-stride over the work list)
 int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 2048)); }
 
@@ -302,9 +307,14 @@ int build(acvd_ctx* c) {
           lo[k] = std::min(lo[k], h.xyz[3 * v + k]);
           hi[k] = std::max(hi[k], h.xyz[3 * v + k]);
         }
+      // Tiles follow the vertex order (contiguous items: coalesced reads, and
+      // spatially coherent when the input order is, as for grid-generated and
+      // most scanned meshes) unless ACVD_TILE_ORDER=morton asks for spatial
+      // (Morton) tiles, whose cluster sets are tighter for arbitrary orders.
+      const bool morton = morton_tiles();
       for (int k = 0; k < 3; ++k) {
         bbox[6 * m + k] = float(lo[k]);
-        bbox[6 * m + 3 + k] = hi[k] > lo[k] ? float(65535.0 / (hi[k] - lo[k])) : 0.0f;
+        bbox[6 * m + 3 + k] = (morton && hi[k] > lo[k]) ? float(65535.0 / (hi[k] - lo[k])) : 0.0f;
       }
     }
     std::copy(h.xyz.begin(), h.xyz.end(), pos.begin() + 3 * h.item_off);
