@@ -604,7 +604,9 @@ ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, i
 //  - otherwise: no move out of a size-1 cluster or against the ring check
 //    (vtkUniformClustering.h:929, 945), then the neighbouring cluster of
 //    lowest dE (ties: lowest cluster id) if dE passes the acceptance test.
-template <int METRIC>
+// kShortRows = false forces the generic QEM path for every row (the oracle
+// checks both paths give the same moves).
+template <int METRIC, bool kShortRows = true>
 ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained, int level,
                           const Scales& sc, int32_t* target_out) {
   const float px = v.xw[4 * i], py = v.xw[4 * i + 1], pz = v.xw[4 * i + 2], pw = v.xw[4 * i + 3];
@@ -625,7 +627,7 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
     return target != kNull;
   }
   if (METRIC == 1) {
-    if (r1 - r0 <= kRingCache)
+    if (kShortRows && r1 - r0 <= kRingCache)
       return qem_propose_short_row(v, i, lab, connexity, constrained, level, sc, target_out);
     // QEM: screen, solves, choice
     QemItem s;

@@ -405,9 +405,18 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
 __device__ inline void tile_insert(const SweepArgs& a, int32_t t, int32_t c) {
   const int32_t cnt = a.tile_cnt[t];
   if (cnt > kTileSet) return;
-  const int32_t* set = a.tile_set + int64_t(t) * kTileSet;
-  for (int32_t k = 0; k < cnt; ++k)
-    if (set[k] == c) return;
+  // all set entries read at once (16-byte loads), entries past cnt ignored
+  const int4* set4 = reinterpret_cast<const int4*>(a.tile_set + int64_t(t) * kTileSet);
+  bool found = false;
+#pragma unroll
+  for (int q = 0; q < kTileSet / 4; ++q) {
+    if (4 * q < cnt) {
+      const int4 v = set4[q];
+      found |= (v.x == c) || (4 * q + 1 < cnt && v.y == c) || (4 * q + 2 < cnt && v.z == c) ||
+               (4 * q + 3 < cnt && v.w == c);
+    }
+  }
+  if (found) return;
   const int32_t slot = atomicAdd(&a.tile_cnt[t], 1);
   if (slot < kTileSet) a.tile_set[int64_t(t) * kTileSet + slot] = c;
 }

@@ -111,7 +111,7 @@ def run_gpu_leg(args):
     ctx.process()
     tst = ctx.stats(0)
     ctx.set_timing(False)
-    names = ["k_scan", "k_eval<QEM>", "k_apply", "k_controller"]
+    names = ["k_scan", "k_eval<QEM>", "accept rounds (k_apply, k_reeval, k_derive_dirty, k_tile_update)", "k_controller"]
     launches = [max(1, v) for v in tst["kernel_launches"]]
     avg_ms = [tst["kernel_ms"][k] / launches[k] for k in range(4)]
     work_per = tst["ctx_evaluated"] / launches[0]
@@ -127,11 +127,21 @@ def run_gpu_leg(args):
     log(f"work items/pass {work_per:.0f}, proposals/pass {prop_per:.0f}, timed-step total {sum(tst['kernel_ms']):.1f} ms")
     dom = max((0, 1), key=lambda k: tst["kernel_ms"][k])
     achieved = byts[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    # HBM traffic per launch of that kernel from the committed rocprofv3 --pmc
+    # summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    traffic, traffic_src = None, None
+    pmc = os.path.join(ROOT, "profiles", "c3_pmc_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            tr = json.load(fh)
+        if names[dom] in tr.get("kernels", {}):
+            traffic = tr["kernels"][names[dom]]["bytes"]
+            traffic_src = "profiles/c3_pmc_traffic.json"
     return {
         "N": N, "K": K, "init": init, "labels": labels, "dev_ms": dev_ms, "wall_ms": wall_ms, "stats": st,
         "energy_gpu": energy, "xyz": xyz, "tri": tri, "H": H,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": names[dom], "avg_launch_us": round(avg_ms[dom] * 1e3, 2),
                      "bytes_per_launch": round(byts[dom]), "launches": launches[dom],
                      "kernel_ms_per_step": {names[k]: round(tst["kernel_ms"][k], 2) for k in range(4)}},

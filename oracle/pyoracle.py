@@ -65,6 +65,7 @@ def lib():
         L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int, ctypes.c_uint32]
         L.orc_replay_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
+        L.orc_replay_run_path.argtypes = [ctypes.c_int] + L.orc_replay_run.argtypes
         L.orc_devinputs_create.restype = P
         L.orc_devinputs_create.argtypes = [ctypes.c_int64, P, ctypes.c_int64, P, ctypes.c_int, P, ctypes.c_double]
         L.orc_devinputs_ncol.restype = ctypes.c_int64
@@ -214,7 +215,7 @@ def hdr_accept_key(seed, loop, item, round=0):
 
 
 def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, unconstrained=0, max_loops=5000000,
-               max_conv=1000000000):
+               max_conv=1000000000, generic_qem=False):
     n = len(row_ptr) - 1
     out = np.empty(n, np.int32)
     st = np.zeros(6, np.int64)
@@ -224,8 +225,8 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
     qq = None if q is None else np.ascontiguousarray(q, np.float32)
     sc = np.ascontiguousarray(scales, np.float64)
     ini = np.ascontiguousarray(init, np.int32)
-    lib().orc_replay_run(n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed, unconstrained, max_loops,
-                         max_conv, _p(ini), _p(out), _p(st))
+    lib().orc_replay_run_path(int(generic_qem), n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
+                              unconstrained, max_loops, max_conv, _p(ini), _p(out), _p(st))
     keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
     return out, dict(zip(keys, (int(v) for v in st)))
 

@@ -32,6 +32,7 @@ struct Replay {
   uint32_t seed = 0;
   acvd::Scales sc{};
   int max_loops = 5000000, max_conv = 1000000000, unconstrained_init = 0;
+  bool generic_qem = false;  // force the generic QEM proposal path (equivalence checks)
   const int32_t* rp = nullptr;
   const int32_t* col = nullptr;
   const float* xw = nullptr;
@@ -147,7 +148,8 @@ struct Replay {
   bool propose(int64_t i, int32_t* target) {
     const acvd::SweepView v = view();
     const int32_t lab = labels[i];
-    return metric == 0 ? acvd::propose_item<0>(v, i, lab, st.connexity, st.constrained, level, sc, target)
+    if (metric == 0) return acvd::propose_item<0>(v, i, lab, st.connexity, st.constrained, level, sc, target);
+    return generic_qem ? acvd::propose_item<1, false>(v, i, lab, st.connexity, st.constrained, level, sc, target)
                        : acvd::propose_item<1>(v, i, lab, st.connexity, st.constrained, level, sc, target);
   }
 

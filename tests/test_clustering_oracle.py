@@ -162,3 +162,24 @@ def test_accept_keys_order_newer_passes_first():
     assert k_new < po.hdr_accept_key(5, 10, 7, round=5)
     keys = {po.hdr_accept_key(5, 7, i) for i in range(5000)}
     assert len(keys) == 5000  # unique per item within a pass
+
+
+@pytest.mark.parametrize("case", ["torus-graded", "cube", "ico"])
+def test_short_row_qem_path_equals_generic_path(case):
+    """The register-resident QEM rule for rows of at most 12 neighbours makes
+    the same moves as the generic screen / solve / choice path: full replays
+    (constrained phases, ring checks) end with identical labels."""
+    if case == "torus-graded":
+        xyz, tri = meshgen.torus(128, 64, 3)
+        H, g, K = meshgen.mean_curvature(xyz, tri), 1.5, 600
+    elif case == "cube":
+        (xyz, tri), H, g, K = meshgen.cube(40, 2, 1e-4), None, 0.0, 400
+    else:
+        (xyz, tri), H, g, K = meshgen.icosphere(5, 1, 0.02), None, 0.0, 500
+    m = po.RefMesh(xyz, tri)
+    init = m.initial_sampling(po.QEM, K, g, H)
+    rp, col, xw, q, sc = po.device_inputs(np.asarray(xyz, np.float64), tri, po.QEM, H, g)
+    a, sa = po.replay_run(rp, col, xw, q, sc, K, po.QEM, init, seed=7, unconstrained=1)
+    b, sb = po.replay_run(rp, col, xw, q, sc, K, po.QEM, init, seed=7, unconstrained=1, generic_qem=True)
+    np.testing.assert_array_equal(a, b)
+    assert sa == sb
