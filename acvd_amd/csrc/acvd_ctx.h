@@ -76,6 +76,7 @@ struct SweepArgs {
   int32_t* labels;
   unsigned long long* acc_ws;  // 4 per cluster: W, Sx, Sy, Sz (int64 fixed)
   unsigned long long* acc_q;   // 10 per cluster (QEM), may be null
+  double* evec;                // 9 per cluster (QEM): eigenvectors of the cluster quadric, may be null
   ClusterGeom* geom;           // centroid + W per cluster (FP64)
   double* T;                   // W |q - c|^2 per cluster (QEM), may be null
   int32_t* sizes;

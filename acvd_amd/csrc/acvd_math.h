@@ -86,11 +86,38 @@ ACVD_HD int64_t fix_q(float q, const Scales& s) { return to_fixed(double(q) * s.
 // Eigen-decomposition: cyclic Jacobi, at most 8 sweeps, stopping as soon as
 // the off-diagonal part is negligible (same decisions on host and device since
 // the inputs are bit-identical).
-ACVD_HD int rep_point_delta(const double* Q, const double* c, int level, double* D) {
+// V0 (optional, 3x3 row-major, orthonormal columns): a warm start, e.g. the
+// eigenvectors of a cluster whose quadric this one perturbs by one item; the
+// solve then runs on V0^T A V0, nearly diagonal, in fewer sweeps.  Vout
+// (optional) receives the eigenvectors (columns, row-major).
+ACVD_HD int rep_point_delta(const double* Q, const double* c, int level, double* D, const double* V0 = nullptr,
+                            double* Vout = nullptr) {
   double a00 = Q[0], a01 = Q[1], a02 = Q[2], a11 = Q[4], a12 = Q[5], a22 = Q[7];
   double r0 = -Q[3] - (Q[0] * c[0] + Q[1] * c[1] + Q[2] * c[2]);
   double r1 = -Q[6] - (Q[1] * c[0] + Q[4] * c[1] + Q[5] * c[2]);
   double r2 = -Q[8] - (Q[2] * c[0] + Q[5] * c[1] + Q[7] * c[2]);
+  if (V0) {
+    // A' = V0^T A V0, r' = V0^T r
+    const double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+    double M[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int q = 0; q < 3; ++q) M[i][q] = A[i][0] * V0[q] + A[i][1] * V0[3 + q] + A[i][2] * V0[6 + q];
+    double B[3][3];
+    for (int p = 0; p < 3; ++p)
+      for (int q = p; q < 3; ++q) B[p][q] = V0[p] * M[0][q] + V0[3 + p] * M[1][q] + V0[6 + p] * M[2][q];
+    a00 = B[0][0];
+    a01 = B[0][1];
+    a02 = B[0][2];
+    a11 = B[1][1];
+    a12 = B[1][2];
+    a22 = B[2][2];
+    const double s0 = V0[0] * r0 + V0[3] * r1 + V0[6] * r2;
+    const double s1 = V0[1] * r0 + V0[4] * r1 + V0[7] * r2;
+    const double s2 = V0[2] * r0 + V0[5] * r1 + V0[8] * r2;
+    r0 = s0;
+    r1 = s1;
+    r2 = s2;
+  }
   // V = identity; columns are eigenvectors
   double v00 = 1, v01 = 0, v02 = 0, v10 = 0, v11 = 1, v12 = 0, v20 = 0, v21 = 0, v22 = 1;
   for (int sweep = 0; sweep < 8; ++sweep) {
@@ -172,9 +199,24 @@ ACVD_HD int rep_point_delta(const double* Q, const double* c, int level, double*
     double p = (v02 * r0 + v12 * r1 + v22 * r2) / lam[2];
     d0 += v02 * p; d1 += v12 * p; d2 += v22 * p;
   }
-  D[0] = d0;
-  D[1] = d1;
-  D[2] = d2;
+  if (V0) {
+    D[0] = V0[0] * d0 + V0[1] * d1 + V0[2] * d2;
+    D[1] = V0[3] * d0 + V0[4] * d1 + V0[5] * d2;
+    D[2] = V0[6] * d0 + V0[7] * d1 + V0[8] * d2;
+  } else {
+    D[0] = d0;
+    D[1] = d1;
+    D[2] = d2;
+  }
+  if (Vout) {
+    const double W[9] = {v00, v01, v02, v10, v11, v12, v20, v21, v22};
+    if (V0) {
+      for (int i = 0; i < 3; ++i)
+        for (int q = 0; q < 3; ++q) Vout[3 * i + q] = V0[3 * i] * W[q] + V0[3 * i + 1] * W[3 + q] + V0[3 * i + 2] * W[6 + q];
+    } else {
+      for (int k = 0; k < 9; ++k) Vout[k] = W[k];
+    }
+  }
   return rank_def;
 }
 
@@ -203,7 +245,7 @@ ACVD_HD ClusterGeom geom_from_fixed(int64_t W, int64_t Sx, int64_t Sy, int64_t S
 // T = W |q - c|^2 of a cluster given its fixed accumulators (QEM metric with
 // active constraints; 0 otherwise).  Optionally returns q.
 ACVD_HD double qem_T_from_fixed(const int64_t* q10, const ClusterGeom& g, const Scales& s, int level,
-                                double* q_out) {
+                                double* q_out, const double* V0 = nullptr, double* Vout = nullptr) {
   if (!(g.W > 0.0) || level <= 0) {
     if (q_out) { q_out[0] = g.cx; q_out[1] = g.cy; q_out[2] = g.cz; }
     return 0.0;
@@ -212,7 +254,7 @@ ACVD_HD double qem_T_from_fixed(const int64_t* q10, const ClusterGeom& g, const 
   for (int k = 0; k < 9; ++k) Q[k] = double(q10[k]) * s.isq;
   double c[3] = {g.cx, g.cy, g.cz};
   double D[3];
-  rep_point_delta(Q, c, level, D);
+  rep_point_delta(Q, c, level, D, V0, Vout);
   if (q_out) { q_out[0] = c[0] + D[0]; q_out[1] = c[1] + D[1]; q_out[2] = c[2] + D[2]; }
   return g.W * (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
 }
@@ -296,15 +338,16 @@ ACVD_HD double qem_mag(float w, float ta, float tb, double Ta_new, double Ta, do
 }
 ACVD_HD bool qem_accept(double delta, double mag) { return delta < -double(kAcceptEps) * mag; }
 
-// T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item.
+// T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item,
+// the solve warm-started from the cluster's own eigenvectors V0 when given.
 ACVD_HD double qem_T_trial(const int64_t* ws, const int64_t* q10, int64_t iw, const int64_t* is,
-                           const int64_t* iq, int sign, const Scales& s, int level) {
+                           const int64_t* iq, int sign, const Scales& s, int level, const double* V0 = nullptr) {
   int64_t W = ws[0] + sign * iw;
   int64_t Sx = ws[1] + sign * is[0], Sy = ws[2] + sign * is[1], Sz = ws[3] + sign * is[2];
   int64_t Q[10];
   for (int k = 0; k < 10; ++k) Q[k] = q10[k] + sign * iq[k];
   ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, s);
-  return qem_T_from_fixed(Q, g, s, level, nullptr);
+  return qem_T_from_fixed(Q, g, s, level, nullptr, V0);
 }
 
 // Ring check of the vertex path (ConnexityConstraintProblemLocal,
@@ -408,6 +451,7 @@ struct SweepView {
   const float* q;                    // 10 floats per item (QEM)
   const unsigned long long* acc_ws;  // 4 per cluster (int64 fixed bits)
   const unsigned long long* acc_q;   // 10 per cluster
+  const double* evec;                // 9 per cluster: eigenvectors of its quadric (QEM, constrained), or null
   const ClusterGeom* geom;
   const double* T;
   const int32_t* sizes;
@@ -517,13 +561,13 @@ ACVD_HD double qem_leave_T(const SweepView& v, int64_t i, int32_t lab, const Sca
   int64_t iw, is[3], iq[10], ws[4], q10[10];
   item_fixed(v, i, sc, &iw, is, iq);
   load_acc(v, lab, ws, q10);
-  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level);
+  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level, v.evec ? v.evec + 9 * int64_t(lab) : nullptr);
 }
 ACVD_HD double qem_join_T(const SweepView& v, int64_t i, int32_t b, const Scales& sc, int level) {
   int64_t iw, is[3], iq[10], ws[4], q10[10];
   item_fixed(v, i, sc, &iw, is, iq);
   load_acc(v, b, ws, q10);
-  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level);
+  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level, v.evec ? v.evec + 9 * int64_t(b) : nullptr);
 }
 
 // Choice among the open targets (CSR order; ties: lowest cluster id) and the

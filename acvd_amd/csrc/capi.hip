@@ -101,7 +101,7 @@ struct acvd_ctx {
   DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
   DBuf<ClusterGeom> d_geom;
-  DBuf<double> d_T;
+  DBuf<double> d_T, d_evec;
   DBuf<int32_t> d_cand_target, d_cand_round, d_cl_round;
   DBuf<int32_t> d_work, d_cand, d_dirty, d_moved;
   int32_t slice_cap = 0, cand_cap = 0, dirty_cap = 0, moved_cap = 0;
@@ -156,6 +156,7 @@ struct acvd_ctx {
     a.acc_q = any_qem ? d_acc_q.p : nullptr;
     a.geom = d_geom.p;
     a.T = any_qem ? d_T.p : nullptr;
+    a.evec = any_qem ? d_evec.p : nullptr;
     a.sizes = d_sizes.p;
     a.stamp = d_stamp.p;
     a.prio = d_prio.p;
@@ -443,6 +444,7 @@ int build(acvd_ctx* c) {
   if (c->any_qem) {
     HIP_TRY(c->d_acc_q.alloc(10 * K));
     HIP_TRY(c->d_T.alloc(K));
+    HIP_TRY(c->d_evec.alloc(9 * K));
   }
   HIP_TRY(c->d_prio.alloc(K));
   HIP_TRY(c->d_geom.alloc(K));

@@ -109,7 +109,8 @@ __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm,
     if (constrained) {
       int64_t q10[10];
       for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
-      T = qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr);
+      // cold solve; its eigenvectors warm-start the trial solves on this cluster
+      T = qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr, nullptr, a.evec + 9 * c);
     }
     a.T[c] = T;
   }
@@ -135,6 +136,7 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   v.q = a.q;
   v.acc_ws = a.acc_ws;
   v.acc_q = a.acc_q;
+  v.evec = a.evec;
   v.geom = a.geom;
   v.T = a.T;
   v.sizes = a.sizes;

@@ -288,8 +288,11 @@ int orc_ring_problem(void* mh, const int32_t* labels, int64_t item, int32_t clus
 }
 
 // Shared-header functions evaluated on the host (compared with the oracle).
-int orc_hdr_rep_point_delta(const double* Q9, const double* c, int level, double* D) {
-  return acvd::rep_point_delta(Q9, c, level, D);
+// V0 (nullable): eigenvector columns to start the Jacobi solve from; Vout
+// (nullable) receives the eigenvector columns of the solve.
+int orc_hdr_rep_point_delta(const double* Q9, const double* c, int level, double* D, const double* V0,
+                            double* Vout) {
+  return acvd::rep_point_delta(Q9, c, level, D, V0, Vout);
 }
 
 int orc_hdr_ring_problem(const int32_t* rp, const int32_t* col, const int32_t* labels, int32_t item,

@@ -59,7 +59,7 @@ def lib():
         L.orc_sym_eigen.argtypes = [P, P, P]
         L.orc_triangle_quadric.argtypes = [P, P, P, P]
         L.orc_ring_problem.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
-        L.orc_hdr_rep_point_delta.argtypes = [P, P, ctypes.c_int, P]
+        L.orc_hdr_rep_point_delta.argtypes = [P, P, ctypes.c_int, P, P, P]
         L.orc_hdr_ring_problem.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32]
         L.orc_hdr_accept_key.restype = ctypes.c_uint64
         L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int, ctypes.c_uint32]
@@ -196,12 +196,16 @@ def triangle_quadric(x1, x2, x3):
     return out
 
 
-def hdr_rep_point_delta(Q9, c, level=3):
+def hdr_rep_point_delta(Q9, c, level=3, V0=None, want_V=False):
+    """shared-header solve: (D, rank deficiency) or, with want_V, (D, rd, V)
+    where V's columns are the eigenvectors; V0 warm-starts the solve"""
     Q = np.ascontiguousarray(Q9, np.float64)
     cc = np.ascontiguousarray(c, np.float64)
     D = np.empty(3)
-    rd = lib().orc_hdr_rep_point_delta(_p(Q), _p(cc), level, _p(D))
-    return D, rd
+    V0a = None if V0 is None else np.ascontiguousarray(V0, np.float64).reshape(9)
+    V = np.empty(9) if want_V else None
+    rd = lib().orc_hdr_rep_point_delta(_p(Q), _p(cc), level, _p(D), _p(V0a), _p(V))
+    return (D, rd, V.reshape(3, 3)) if want_V else (D, rd)
 
 
 def hdr_ring_problem(row_ptr, col, labels, item, cluster):

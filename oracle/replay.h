@@ -41,7 +41,7 @@ struct Replay {
   std::vector<int32_t> labels, sizes, stamp, cand_pass, pending, cl_round;
   std::vector<unsigned long long> acc_ws, acc_q;
   std::vector<acvd::ClusterGeom> geom;
-  std::vector<double> T;
+  std::vector<double> T, evec;
   std::vector<unsigned long long> prio;
   acvd::MeshState st{};
   ReplayStats stats;
@@ -55,6 +55,7 @@ struct Replay {
     v.q = q;
     v.acc_ws = acc_ws.data();
     v.acc_q = acc_q.data();
+    v.evec = metric == 1 ? evec.data() : nullptr;
     v.geom = geom.data();
     v.T = T.data();
     v.sizes = sizes.data();
@@ -88,7 +89,9 @@ struct Replay {
     geom[c] = acvd::geom_from_fixed(ws[0], ws[1], ws[2], ws[3], sc);
     if (metric == 1) {
       const int64_t* q10 = reinterpret_cast<const int64_t*>(&acc_q[10 * int64_t(c)]);
-      T[c] = st.constrained ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr) : 0.0;
+      T[c] = st.constrained
+                 ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr, nullptr, &evec[9 * int64_t(c)])
+                 : 0.0;
     }
   }
 
@@ -271,6 +274,7 @@ struct Replay {
     acc_q.assign(metric == 1 ? 10 * size_t(K) : 0, 0ull);
     geom.assign(K, acvd::ClusterGeom{});
     T.assign(metric == 1 ? K : 0, 0.0);
+    evec.assign(metric == 1 ? 9 * size_t(K) : 0, 0.0);
     prio.assign(K, ~0ull);
     fill_until_stable();
     st = acvd::MeshState{};

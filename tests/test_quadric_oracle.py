@@ -181,3 +181,55 @@ def test_header_solve_agrees_with_oracle_on_random_quadrics():
             D, rd2 = po.hdr_rep_point_delta(Q, c, level)
             assert rd == rd2
             np.testing.assert_allclose(c + D, p, rtol=1e-7, atol=1e-9 * (1 + np.abs(p).max()))
+
+
+def _random_rotation(rng):
+    Qm, R = np.linalg.qr(rng.standard_normal((3, 3)))
+    return Qm * np.sign(np.diag(R))
+
+
+def _q9(A, b):
+    return np.array([A[0, 0], A[0, 1], A[0, 2], -b[0], A[1, 1], A[1, 2], -b[1], A[2, 2], -b[2]])
+
+
+def test_warm_started_solve_equals_the_cold_solve():
+    # the sweep starts a trial solve from the eigenvectors of the cluster it
+    # perturbs: the result must not depend on the start basis
+    rng = np.random.default_rng(5)
+    for it in range(300):
+        # eigenvalue ratios stay away from the 1e-3 truncation threshold
+        w = rng.choice([1.0, 1e-1, 1e-5], 3) if it % 3 else np.array([1.0, 0.3, 0.05])
+        U = _random_rotation(rng)
+        A = U @ np.diag(w) @ U.T
+        b = rng.standard_normal(3)
+        c = rng.standard_normal(3)
+        Q = _q9(A, b)
+        D0, rd0, V = po.hdr_rep_point_delta(Q, c, 3, want_V=True)
+        # the returned basis diagonalises A and is orthonormal
+        np.testing.assert_allclose(V.T @ V, np.eye(3), atol=1e-13)
+        off = V.T @ A @ V - np.diag(np.diag(V.T @ A @ V))
+        assert np.abs(off).max() <= 1e-13 * np.abs(A).max()
+        E = rng.standard_normal((3, 3)) * 1e-3 * w.max()
+        _, V1 = np.linalg.eigh(A + E + E.T)
+        for V0 in (V, V1, _random_rotation(rng)):
+            D, rd = po.hdr_rep_point_delta(Q, c, 3, V0=V0)
+            assert rd == rd0
+            np.testing.assert_allclose(D, D0, rtol=1e-8, atol=1e-9 * (np.abs(D0).max() + np.abs(c).max()))
+        # the identity start is the cold solve, bit for bit
+        D, rd = po.hdr_rep_point_delta(Q, c, 3, V0=np.eye(3))
+        assert rd == rd0 and np.array_equal(D, D0)
+
+
+def test_warm_started_solve_kat_edge():
+    tris = grid_tris((1.0, 0.8, 0.3), (0, 1, 0), (0, 0, 1), 4, 0.05)
+    tris += grid_tris((0.8, 1.0, 0.3), (1, 0, 0), (0, 0, 1), 4, 0.05)
+    Q = face_quadric(tris)
+    c = np.array([0.93, 0.91, 0.37])
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        D, rd, V = po.hdr_rep_point_delta(Q[:9], c, 3, V0=_random_rotation(rng), want_V=True)
+        assert rd == 1
+        np.testing.assert_allclose(c + D, [1.0, 1.0, 0.37], atol=1e-12)
+        # the dropped direction is the edge line
+        lam = np.diag(V.T @ np.array([[Q[0], Q[1], Q[2]], [Q[1], Q[4], Q[5]], [Q[2], Q[5], Q[7]]]) @ V)
+        assert abs(abs(V[:, np.argmin(np.abs(lam))] @ [0, 0, 1]) - 1) < 1e-12
