@@ -42,6 +42,12 @@ struct DevMesh {
   Scales scales;
 };
 
+// Pass log entry: pass number, items evaluated, proposals, moves, moves of
+// each accept round, and the phase flags of the first mesh (bit 0
+// constrained, bit 1 connexity).
+constexpr int kPassLogFields = 5 + kMatchRounds;
+constexpr int kPassLogCap = 64;  // > passes per chunk
+
 // Small control block (device), read back by the host between pass chunks.
 struct Ctrl {
   int32_t any_sweep;
@@ -50,13 +56,16 @@ struct Ctrl {
   int32_t pad;
   int32_t work_count[kSlices];  // items to evaluate in this pass, per slice
   int32_t cand_count[kSlices];  // proposals of this pass, per slice
-  int32_t mods[kSlices];        // applied moves of this pass (single-mesh contexts)
+  int32_t mods[kMatchRounds][kSlices];  // applied moves of this pass, per accept round
   int32_t moved_count[kSlices]; // items moved in this pass
   int32_t dirty_count[kMatchRounds][kSlices];  // clusters modified, per round
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
   unsigned long long passes;     // passes run (any mesh sweeping)
+  // Per-pass counts of the last kPassLogCap passes (ring indexed by pass
+  // number): the per-loop console output reads them between pass chunks.
+  int32_t pass_log[kPassLogCap][kPassLogFields];
 };
 
 // Pointers handed to the sweep kernels (kernel argument block).
@@ -83,8 +92,10 @@ struct SweepArgs {
   int32_t* stamp;              // mesh-local loop of the last modification
   unsigned long long* prio;    // accept keys per cluster
   int32_t* cand_pass;          // per item: loop of its last proposal
-  int32_t* cand_target;        // per item: target of its live proposal (< 0: none)
-  int32_t* cand_round;         // per item: loop * kMatchRounds + round of its last evaluation
+  // candidate records, indexed like the candidate list storage (cand):
+  int32_t* cand_tgt;           // target of the live proposal (< 0: none)
+  int32_t* cand_lab;           // label of the candidate item (fixed while it is live)
+  int32_t* cand_since;         // loop * kMatchRounds + round of its last evaluation
   int32_t* cl_round;           // per cluster: loop * kMatchRounds + round of its last move
   int32_t* work;               // items to evaluate in this pass (kSlices slices)
   int32_t slice_cap;           // capacity of one work-list slice

@@ -79,6 +79,7 @@ struct HostMesh {
 };
 
 constexpr int kChunk = 16;        // passes per graph launch
+static_assert(kPassLogCap > kChunk, "the pass log must hold a chunk of passes");
 constexpr int kReduceItems = 16384;  // items per block of the deterministic reductions
 
 }  // namespace
@@ -102,7 +103,7 @@ struct acvd_ctx {
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T, d_evec;
-  DBuf<int32_t> d_cand_target, d_cand_round, d_cl_round;
+  DBuf<int32_t> d_cand_tgt, d_cand_lab, d_cand_since, d_cl_round;
   DBuf<int32_t> d_work, d_cand, d_dirty, d_moved;
   int32_t slice_cap = 0, cand_cap = 0, dirty_cap = 0, moved_cap = 0;
   // scan tiles
@@ -161,8 +162,9 @@ struct acvd_ctx {
     a.stamp = d_stamp.p;
     a.prio = d_prio.p;
     a.cand_pass = d_cand_pass.p;
-    a.cand_target = d_cand_target.p;
-    a.cand_round = d_cand_round.p;
+    a.cand_tgt = d_cand_tgt.p;
+    a.cand_lab = d_cand_lab.p;
+    a.cand_since = d_cand_since.p;
     a.cl_round = d_cl_round.p;
     a.work = d_work.p;
     a.slice_cap = slice_cap;
@@ -448,8 +450,6 @@ int build(acvd_ctx* c) {
   }
   HIP_TRY(c->d_prio.alloc(K));
   HIP_TRY(c->d_geom.alloc(K));
-  HIP_TRY(c->d_cand_target.alloc(N));
-  HIP_TRY(c->d_cand_round.alloc(N));
   HIP_TRY(c->d_cl_round.alloc(K));
   {
     // scan tiles are dealt round-robin to the slices
@@ -465,9 +465,12 @@ int build(acvd_ctx* c) {
     const int64_t grid = sweep_grid_for(N);
     c->cand_cap = int32_t(slice_bound(grid));
     HIP_TRY(c->d_cand.alloc(size_t(c->cand_cap) * kSlices));
+    HIP_TRY(c->d_cand_tgt.alloc(size_t(c->cand_cap) * kSlices));
+    HIP_TRY(c->d_cand_lab.alloc(size_t(c->cand_cap) * kSlices));
+    HIP_TRY(c->d_cand_since.alloc(size_t(c->cand_cap) * kSlices));
     // apply blocks (round grid, grid-stride over at most N candidates) append
     // at most one moved item and two dirty clusters per candidate they visit
-    const int64_t rgrid = std::min<int64_t>(grid, 1024);
+    const int64_t rgrid = grid;
     c->moved_cap = int32_t(slice_bound(rgrid));
     c->dirty_cap = int32_t(2 * slice_bound(rgrid));
     HIP_TRY(c->d_moved.alloc(size_t(c->moved_cap) * kSlices));
@@ -505,7 +508,7 @@ int build(acvd_ctx* c) {
 int sweep_grid(const acvd_ctx* c) { return int(sweep_grid_for(c->N)); }
 
 // grid of the accept rounds (grid-stride over the candidate list)
-int round_grid(const acvd_ctx* c) { return std::min(sweep_grid(c), 1024); }
+int round_grid(const acvd_ctx* c) { return sweep_grid(c); }
 
 int fill_until_stable(acvd_ctx* c, const int32_t* mask, int max_rounds) {
   hipStream_t s = c->stream;
@@ -604,6 +607,21 @@ int run_chunk(acvd_ctx* c, bool use_graph) {
   return enqueue_chunk(c);
 }
 
+// Prints the logged passes in [*printed, ctrl.passes) and advances *printed.
+void print_pass_log(const Ctrl& ctrl, unsigned long long* printed) {
+  unsigned long long p = *printed;
+  if (ctrl.passes > kPassLogCap && p < ctrl.passes - kPassLogCap) p = ctrl.passes - kPassLogCap;
+  for (; p < ctrl.passes; ++p) {
+    const int32_t* e = ctrl.pass_log[p % kPassLogCap];
+    std::printf("%sLoop %d: %d items evaluated, %d proposals, %d Modifications (per round:", (e[4 + kMatchRounds] & 2) ? "*" : "",
+                e[0], e[1], e[2], e[3]);
+    for (int r = 0; r < kMatchRounds; ++r) std::printf(" %d", e[4 + r]);
+    std::printf(")\n");
+  }
+  std::fflush(stdout);
+  *printed = ctrl.passes;
+}
+
 int minimize_energy(acvd_ctx* c) {
   hipStream_t s = c->stream;
   const int M = int(c->meshes.size());
@@ -636,7 +654,6 @@ int minimize_energy(acvd_ctx* c) {
   HIP_TRY(launch_fill_i32(s, c->d_cand_pass.p, c->N, -2));
   HIP_TRY(launch_fill_i32(s, c->d_tile_prop.p, c->n_tiles, -2));
   HIP_TRY(launch_tile_sets(s, a, nullptr));
-  HIP_TRY(launch_fill_i32(s, c->d_cand_target.p, c->N, -3));
   HIP_TRY(launch_fill_i32(s, c->d_cl_round.p, c->K, -1));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
@@ -646,11 +663,16 @@ int minimize_energy(acvd_ctx* c) {
   }
   std::vector<int32_t> mask(M), ndisc(M);
   int64_t guard = 0;
+  // per-loop console output (SetConsoleOutput(2), vtkUniformClustering.h:751-758)
+  bool loop_output = false;
+  for (int m = 0; m < M; ++m) loop_output |= c->meshes[m].params.console_output > 1;
+  unsigned long long printed = 0;
   while (true) {
     int rc2 = run_chunk(c, use_graph);
     if (rc2) return rc2;
     HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (loop_output) print_pass_log(*c->h_ctrl, &printed);
     if (c->h_ctrl->any_need_clean) {
       HIP_TRY(hipMemcpyAsync(c->h_state, c->d_state.p, M * sizeof(MeshState), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));

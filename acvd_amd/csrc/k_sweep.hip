@@ -1,17 +1,21 @@
 // Boundary sweep of the vertex clustering, one pass =
-//   propose  : every boundary item whose clusters changed recently evaluates
-//              its best move to a neighbouring cluster (isotropic or QEM
-//              energy, vtkUniformClustering.h:922-991) and, if it lowers the
-//              energy, registers it with an atomicMin of its accept key at the
-//              source and target clusters;
-//   apply    : a candidate whose key is the minimum at both clusters is
-//              applied: label write plus int64 fixed-point atomics on the two
-//              cluster accumulators, then the two cluster summaries (centroid,
-//              W, and the FP64 rank-truncated QEM point) are refreshed;
+//   scan     : the boundary items whose clusters changed recently (or whose
+//              last proposal lost) form the work list;
+//   evaluate : each evaluates its best move to a neighbouring cluster
+//              (isotropic or QEM energy, vtkUniformClustering.h:922-991) and,
+//              if it lowers the energy, registers its accept key at the source
+//              and target clusters (minimum per cluster) and joins the
+//              candidate list;
+//   rounds   : kMatchRounds accept rounds.  In each, a candidate whose key is
+//              the minimum at both of its clusters is applied (label, int64
+//              fixed-point accumulators of the two clusters, then their
+//              centroid, W and FP64 rank-truncated QEM point); the remaining
+//              candidates are re-evaluated against the new state when one of
+//              their clusters moved, and register their next-round key;
 //   controller: per-mesh phase logic of MinimizeEnergy
 //              (vtkUniformClustering.h:724-830) after the pass.
-// Since a cluster takes part in at most one applied move per pass, every
-// applied dE is exact with respect to the state at the start of the pass and
+// A cluster takes part in at most one applied move per round, so every
+// applied dE is exact with respect to the state at the start of its round and
 // the energy decreases monotonically.  Between convergences the host runs the
 // clean (largest connected component per cluster, vtkUniformClustering.h:405-549)
 // and fill (regrowth of null items, :551-633) kernels below.
@@ -292,19 +296,26 @@ __device__ inline void load_slice_index(const int32_t* counts, SliceIndex* wi) {
   __syncthreads();
 }
 
-__device__ inline int32_t slice_entry(const int32_t* list, int32_t cap, const SliceIndex* wi, int32_t v) {
+// Storage index of position v of the concatenation.
+__device__ inline int64_t slice_pos(int32_t cap, const SliceIndex* wi, int32_t v) {
   int lo = 0, hi = kSlices;  // find the slice k with start[k] <= v < start[k+1]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (wi->start[mid] <= v) lo = mid;
     else hi = mid;
   }
-  return list[int64_t(lo) * cap + (v - wi->start[lo])];
+  return int64_t(lo) * cap + (v - wi->start[lo]);
 }
 
-// Wave-aggregated append of item i (when want) to slice `slice` of the
-// candidate list; every lane of the wave must call it.
-__device__ inline void cand_append(const SweepArgs& a, int slice, bool want, int32_t i) {
+__device__ inline int32_t slice_entry(const int32_t* list, int32_t cap, const SliceIndex* wi, int32_t v) {
+  return list[slice_pos(cap, wi, v)];
+}
+
+// Wave-aggregated append of the proposal (item i: lab -> target, evaluated at
+// round tag `since`) to slice `slice` of the candidate list, with its record;
+// every lane of the wave must call it.
+__device__ inline void cand_append(const SweepArgs& a, int slice, bool want, int32_t i, int32_t lab, int32_t target,
+                                   int32_t since) {
   const unsigned long long bal = __ballot(want);
   if (bal == 0ull) return;
   const int lane = __lane_id();
@@ -312,20 +323,74 @@ __device__ inline void cand_append(const SweepArgs& a, int slice, bool want, int
   int32_t base = 0;
   if (lane == leader) base = atomicAdd(&a.ctrl->cand_count[slice], __popcll(bal));
   base = __shfl(base, leader);
-  if (want) a.cand[int64_t(slice) * a.cand_cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = i;
+  if (want) {
+    const int64_t e = int64_t(slice) * a.cand_cap + base + __popcll(bal & ((1ull << lane) - 1ull));
+    a.cand[e] = i;
+    a.cand_lab[e] = lab;
+    a.cand_tgt[e] = target;
+    a.cand_since[e] = since;
+  }
 }
 
-// Round-0 bookkeeping of a proposal: round-0 key at both clusters, candidate
-// state, tile activity, per-mesh proposal count (batches).
-__device__ inline void register_round0(const SweepArgs& a, MeshState* st, const DevMesh& dm, int64_t i, int32_t lab,
-                                       int32_t target) {
+// Block-local minimum of the accept keys registered per cluster: an
+// open-addressing table in LDS, flushed with one global atomicMin per distinct
+// cluster.  Candidates of a block lie close together on the mesh and share
+// few clusters, so this replaces most same-address global atomics.  The
+// minimum does not depend on the order, so the keys are those of direct
+// atomics.  A block registers at most 2 * kBlock keys between flushes, which
+// keeps the table at most half full (every probe sequence ends).
+constexpr int kKeyAggBits = 10;
+constexpr int kKeyAggSlots = 1 << kKeyAggBits;
+static_assert(2 * kBlock <= kKeyAggSlots / 2, "key table must stay at most half full");
+struct KeyAgg {
+  int32_t cl[kKeyAggSlots];
+  unsigned long long key[kKeyAggSlots];
+};
+
+__device__ inline void keyagg_clear(KeyAgg* g) {
+  for (int k = threadIdx.x; k < kKeyAggSlots; k += blockDim.x) {
+    g->cl[k] = -1;
+    g->key[k] = ~0ull;
+  }
+}
+
+__device__ inline void keyagg_min(KeyAgg* g, int32_t c, unsigned long long key) {
+  uint32_t h = (uint32_t(c) * 2654435761u) >> (32 - kKeyAggBits);
+  while (true) {
+    const int32_t prev = atomicCAS(&g->cl[h], -1, c);
+    if (prev == -1 || prev == c) {
+      atomicMin(&g->key[h], key);
+      return;
+    }
+    h = (h + 1) & (kKeyAggSlots - 1);
+  }
+}
+
+// Flushes the table to the global keys and clears it; all threads of the
+// block call it (it synchronises the block before and after).
+__device__ inline void keyagg_flush(KeyAgg* g, unsigned long long* prio) {
+  __syncthreads();
+  for (int k = threadIdx.x; k < kKeyAggSlots; k += blockDim.x) {
+    const int32_t c = g->cl[k];
+    if (c >= 0) {
+      atomicMin(&prio[c], g->key[k]);
+      g->cl[k] = -1;
+      g->key[k] = ~0ull;
+    }
+  }
+  __syncthreads();
+}
+
+// Round-0 bookkeeping of a proposal: round-0 key at both clusters (through
+// the block's key table), item stamp, tile activity, per-mesh proposal count
+// (batches).
+__device__ inline void register_round0(const SweepArgs& a, KeyAgg* agg, MeshState* st, const DevMesh& dm, int64_t i,
+                                       int32_t lab, int32_t target) {
   const int32_t L = st->loops;
   const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
-  if (lab != kNull) atomicMin(&a.prio[lab], key);
-  atomicMin(&a.prio[target], key);
+  if (lab != kNull) keyagg_min(agg, lab, key);
+  keyagg_min(agg, target, key);
   a.cand_pass[i] = L;
-  a.cand_target[i] = target;
-  a.cand_round[i] = L * kMatchRounds;
   a.tile_prop[a.item_tile[i]] = L;
   if (a.item_mesh) atomicAdd(&st->pass_props, 1);
 }
@@ -336,6 +401,8 @@ __device__ inline void register_round0(const SweepArgs& a, MeshState* st, const 
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
   __shared__ SliceIndex wi;
+  __shared__ KeyAgg agg;
+  keyagg_clear(&agg);
   load_slice_index(a.ctrl->work_count, &wi);
   const int32_t n = wi.start[kSlices];
   const SweepView v = make_view(a);
@@ -344,25 +411,27 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
     const int32_t t = base + int32_t(threadIdx.x);
     bool want = false;
     int64_t i = 0;
+    int32_t lab = kNull, target = kNull, since = 0;
     if (t < n) {
       i = slice_entry(a.work, a.slice_cap, &wi, t);
       const int m = mesh_of_item(a, i);
       const DevMesh& dm = a.meshes[m];
       if (dm.metric == METRIC) {
         MeshState* st = &a.state[m];
-        const int32_t lab = a.labels[i];
-        int32_t target = kNull;
+        lab = a.labels[i];
         if (propose_item<METRIC>(v, i, lab, st->connexity, st->constrained, dm.level, dm.scales, &target)) {
-          register_round0(a, st, dm, i, lab, target);
+          register_round0(a, &agg, st, dm, i, lab, target);
+          since = st->loops * kMatchRounds;
           want = true;
         }
       }
     }
-    cand_append(a, slice, want, int32_t(i));
+    cand_append(a, slice, want, int32_t(i), lab, target, since);
+    keyagg_flush(&agg, a.prio);
   }
 }
 
-// Re-evaluate (round r > 0): a live candidate (cand_target >= 0) whose own
+// Re-evaluate (round r > 0): a live candidate (cand_tgt >= 0) whose own
 // cluster or target moved since its last evaluation is evaluated again
 // against the state left by round r-1: it registers its round-r key with its
 // (possibly new) target or is dropped (-3).  The others keep their move, whose
@@ -370,34 +439,40 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ SliceIndex ci;
+  __shared__ KeyAgg agg;
+  keyagg_clear(&agg);
   load_slice_index(a.ctrl->cand_count, &ci);
   const int32_t n = ci.start[kSlices];
   const SweepView v = make_view(a);
-  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    const int64_t i = slice_entry(a.cand, a.cand_cap, &ci, t);
-    if (a.cand_target[i] < 0) continue;
-    const int m = mesh_of_item(a, i);
-    const DevMesh& dm = a.meshes[m];
-    if (dm.metric != METRIC) continue;
-    const MeshState& st = a.state[m];
-    const int32_t lab = a.labels[i];
-    const int32_t since = a.cand_round[i];
-    const bool stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[a.cand_target[i]] >= since;
-    const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
-    if (!stale) {
-      if (lab != kNull) atomicMin(&a.prio[lab], key);
-      atomicMin(&a.prio[a.cand_target[i]], key);
-      continue;
+  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    const int32_t t = base + int32_t(threadIdx.x);
+    const int64_t e = t < n ? slice_pos(a.cand_cap, &ci, t) : 0;
+    const int32_t b = t < n ? a.cand_tgt[e] : -1;
+    if (b >= 0) {
+      const int64_t i = a.cand[e];
+      const int32_t lab = a.cand_lab[e];  // a live candidate has not moved
+      const int32_t since = a.cand_since[e];
+      const int m = mesh_of_item(a, i);
+      const DevMesh& dm = a.meshes[m];
+      if (dm.metric == METRIC) {
+        const MeshState& st = a.state[m];
+        const bool stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
+        const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
+        int32_t target = b;
+        bool live = true;
+        if (stale) {
+          a.cand_since[e] = st.loops * kMatchRounds + round;
+          target = kNull;
+          live = propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target);
+          a.cand_tgt[e] = live ? target : -3;
+        }
+        if (live) {
+          if (lab != kNull) keyagg_min(&agg, lab, key);
+          keyagg_min(&agg, target, key);
+        }
+      }
     }
-    int32_t target = kNull;
-    a.cand_round[i] = st.loops * kMatchRounds + round;
-    if (propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target)) {
-      if (lab != kNull) atomicMin(&a.prio[lab], key);
-      atomicMin(&a.prio[target], key);
-      a.cand_target[i] = target;
-    } else {
-      a.cand_target[i] = -3;
-    }
+    keyagg_flush(&agg, a.prio);
   }
 }
 
@@ -448,8 +523,8 @@ __device__ inline void list_append(int32_t* list, int32_t cap, int32_t* counts, 
 }
 
 // Apply round `round`: a live candidate holding the minimum round key at both
-// of its clusters moves: label, fixed-point accumulator deltas (no returned
-// values needed: the round owns both clusters), sizes and stamps.  The two
+// of its clusters moves: label, fixed-point accumulator deltas (plain
+// read-modify-write: the round owns both clusters), sizes and stamps.  The two
 // clusters join the round's dirty list (their centroid and T are derived by
 // k_derive_dirty) and the item joins the pass's moved list (tile sets are
 // extended by k_tile_update).  Counts go to the sliced counters (single mesh)
@@ -466,20 +541,21 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
     int64_t i = 0;
     int32_t lab = kNull, b = kNull;
     if (t < n) {
-      i = slice_entry(a.cand, a.cand_cap, &ci, t);
-      b = a.cand_target[i];
+      const int64_t e = slice_pos(a.cand_cap, &ci, t);
+      b = a.cand_tgt[e];
       if (b >= 0) {
+        i = a.cand[e];
+        lab = a.cand_lab[e];
         const int m = mesh_of_item(a, i);
         MeshState* st = &a.state[m];
         const int32_t L = st->loops;
-        lab = a.labels[i];
         const DevMesh& dm = a.meshes[m];
         const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
         win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
         if (win) {
           // exclusive owner of clusters lab and b for this round
           a.labels[i] = b;
-          a.cand_target[i] = -3;
+          a.cand_tgt[e] = -3;
           const float4 p = a.xw[i];
           const Scales sc = dm.scales;
           const unsigned long long iw = (unsigned long long)fix_w(p.w, sc);
@@ -487,20 +563,26 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
                                             (unsigned long long)fix_s(p.w, p.y, sc),
                                             (unsigned long long)fix_s(p.w, p.z, sc)};
           const int32_t rtag = L * kMatchRounds + round;
+          // plain read-modify-write: no other thread of this round touches
+          // the accumulators of lab or b
           if (lab != kNull) {
-            atomicSub(&a.acc_ws[4 * int64_t(lab)], iw);
-            for (int k = 0; k < 3; ++k) atomicSub(&a.acc_ws[4 * int64_t(lab) + 1 + k], is[k]);
+            unsigned long long* wa = a.acc_ws + 4 * int64_t(lab);
+            wa[0] -= iw;
+            for (int k = 0; k < 3; ++k) wa[1 + k] -= is[k];
             a.sizes[lab] -= 1;
             a.stamp[lab] = L;
             a.cl_round[lab] = rtag;
           }
-          atomicAdd(&a.acc_ws[4 * int64_t(b)], iw);
-          for (int k = 0; k < 3; ++k) atomicAdd(&a.acc_ws[4 * int64_t(b) + 1 + k], is[k]);
+          unsigned long long* wb = a.acc_ws + 4 * int64_t(b);
+          wb[0] += iw;
+          for (int k = 0; k < 3; ++k) wb[1 + k] += is[k];
           if (dm.metric == 1) {
+            unsigned long long* qa = lab != kNull ? a.acc_q + 10 * int64_t(lab) : nullptr;
+            unsigned long long* qb = a.acc_q + 10 * int64_t(b);
             for (int k = 0; k < 10; ++k) {
               const unsigned long long iq = (unsigned long long)fix_q(a.q[10 * i + k], sc);
-              if (lab != kNull) atomicSub(&a.acc_q[10 * int64_t(lab) + k], iq);
-              atomicAdd(&a.acc_q[10 * int64_t(b) + k], iq);
+              if (qa) qa[k] -= iq;
+              qb[k] += iq;
             }
           }
           a.sizes[b] += 1;
@@ -515,10 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
     list_append(a.dirty, a.dirty_cap, a.ctrl->dirty_count[round], slice, win, b);
     list_append(a.moved, a.moved_cap, a.ctrl->moved_count, slice, win, int32_t(i));
   }
-  if (!a.item_mesh) {
-    const int tm = block_sum(mods);
-    if (threadIdx.x == 0 && tm) atomicAdd(&a.ctrl->mods[blockIdx.x % kSlices], tm);
-  }
+  const int tm = block_sum(mods);
+  if (threadIdx.x == 0 && tm) atomicAdd(&a.ctrl->mods[round][blockIdx.x % kSlices], tm);
 }
 
 // Centroid, W and (QEM) T of the clusters a round modified, from their final
@@ -560,22 +640,34 @@ __global__ void k_controller(SweepArgs a) {
     any_clean = 0;
   }
   __syncthreads();
-  __shared__ int s_work, s_props, s_mods;
-  if (threadIdx.x == 0) {
-    int w = 0, p = 0, d = 0;
-    for (int k = 0; k < kSlices; ++k) {
-      w += a.ctrl->work_count[k];
-      p += a.ctrl->cand_count[k];
-      d += a.ctrl->mods[k];
+  __shared__ int s_work, s_props, s_mods, s_round_mods[kMatchRounds];
+  static_assert(kSlices == 64, "one wave sums the slice counters");
+  if (threadIdx.x < 64) {  // lane k reads slice k, wave sums
+    int w = a.ctrl->work_count[threadIdx.x], p = a.ctrl->cand_count[threadIdx.x];
+    int dr[kMatchRounds];
+    for (int r = 0; r < kMatchRounds; ++r) dr[r] = a.ctrl->mods[r][threadIdx.x];
+    for (int off = 32; off > 0; off >>= 1) {
+      w += __shfl_xor(w, off);
+      p += __shfl_xor(p, off);
+      for (int r = 0; r < kMatchRounds; ++r) dr[r] += __shfl_xor(dr[r], off);
     }
-    s_work = w;
-    s_props = p;
-    s_mods = d;
+    if (threadIdx.x == 0) {
+      int d = 0;
+      for (int r = 0; r < kMatchRounds; ++r) {
+        s_round_mods[r] = dr[r];
+        d += dr[r];
+      }
+      s_work = w;
+      s_props = p;
+      s_mods = d;
+    }
   }
   __syncthreads();
   bool ran = false;
+  __shared__ int s_flags;  // phase flags of mesh 0 during this pass
   for (int m = threadIdx.x; m < a.n_meshes; m += blockDim.x) {
     MeshState s = a.state[m];
+    if (m == 0) s_flags = (s.constrained ? 1 : 0) | (s.connexity ? 2 : 0);
     if (s.phase == kPhaseSweep) {
       ran = true;
       if (!a.item_mesh) {  // one mesh: the sliced counters are its own
@@ -593,9 +685,21 @@ __global__ void k_controller(SweepArgs a) {
   if (threadIdx.x < kSlices) {
     a.ctrl->work_count[threadIdx.x] = 0;
     a.ctrl->cand_count[threadIdx.x] = 0;
-    a.ctrl->mods[threadIdx.x] = 0;
     a.ctrl->moved_count[threadIdx.x] = 0;
-    for (int r = 0; r < kMatchRounds; ++r) a.ctrl->dirty_count[r][threadIdx.x] = 0;
+    for (int r = 0; r < kMatchRounds; ++r) {
+      a.ctrl->mods[r][threadIdx.x] = 0;
+      a.ctrl->dirty_count[r][threadIdx.x] = 0;
+    }
+  }
+  if (threadIdx.x == 0 && (any_sweep & 2)) {
+    const unsigned long long pass = a.ctrl->passes;
+    int32_t* e = a.ctrl->pass_log[pass % kPassLogCap];
+    e[0] = int32_t(pass);
+    e[1] = s_work;
+    e[2] = s_props;
+    e[3] = s_mods;
+    for (int r = 0; r < kMatchRounds; ++r) e[4 + r] = s_round_mods[r];
+    e[4 + kMatchRounds] = s_flags;
   }
   if (threadIdx.x == 0) {
     a.ctrl->evaluated += (unsigned long long)s_work;
