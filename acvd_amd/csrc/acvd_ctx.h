@@ -16,9 +16,17 @@
 namespace acvd {
 
 constexpr int kBlock = 256;
-// Work-list slices: blocks append to counter blockIdx % kSlices, which keeps
-// same-address atomics (slow at device scope) to a few per slice per pass.
+// Move-counter slices: apply blocks add their count (no returned value) to
+// counter blockIdx % kSlices, which spreads same-address atomics.
 constexpr int kSlices = 64;
+// Work-list slices: scan tile t appends to slice t % kWorkSlices, so that a
+// slice counter sees few same-address atomics per pass.
+constexpr int kWorkSlices = 4096;
+// Upper bound of the block-owned segments (= evaluate / apply grid).
+constexpr int kMaxSeg = 2048;
+// Candidate lists of at most this many proposals (last pass) are walked one
+// segment per block, longer ones evenly through an index of the segments.
+constexpr int kSegWalkMax = 16384;
 // Scan tiles: groups of kTileItems items of one mesh in spatial (Morton)
 // order.  Each tile keeps a superset of the clusters its items and their
 // neighbours belong to; a tile none of whose clusters changed in the last
@@ -53,12 +61,8 @@ struct Ctrl {
   int32_t any_sweep;
   int32_t any_need_clean;
   int32_t fill_changed;
-  int32_t pad;
-  int32_t work_count[kSlices];  // items to evaluate in this pass, per slice
-  int32_t cand_count[kSlices];  // proposals of this pass, per slice
+  int32_t seg_walk;  // candidate walk of the next pass: 1 segment per block, 0 dense
   int32_t mods[kMatchRounds][kSlices];  // applied moves of this pass, per accept round
-  int32_t moved_count[kSlices]; // items moved in this pass
-  int32_t dirty_count[kMatchRounds][kSlices];  // clusters modified, per round
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
@@ -92,20 +96,29 @@ struct SweepArgs {
   int32_t* stamp;              // mesh-local loop of the last modification
   unsigned long long* prio;    // accept keys per cluster
   int32_t* cand_pass;          // per item: loop of its last proposal
-  // candidate records, indexed like the candidate list storage (cand):
+  // candidate records, indexed like cand:
   int32_t* cand_tgt;           // target of the live proposal (< 0: none)
   int32_t* cand_lab;           // label of the candidate item (fixed while it is live)
   int32_t* cand_since;         // loop * kMatchRounds + round of its last evaluation
   int32_t* cl_round;           // per cluster: loop * kMatchRounds + round of its last move
-  int32_t* work;               // items to evaluate in this pass (kSlices slices)
-  int32_t slice_cap;           // capacity of one work-list slice
-  int32_t cand_cap;            // capacity of one candidate-list slice
+  int32_t* work;               // items to evaluate in this pass (kWorkSlices slices)
+  int32_t* work_count;         // fill count of each work slice
+  int32_t work_cap;            // capacity of one work slice
   Ctrl* ctrl;
-  int32_t* cand;               // proposals of this pass (kSlices slices)
-  int32_t* dirty;              // clusters modified by the current round (kSlices slices)
-  int32_t* moved;              // items moved in this pass (kSlices slices)
+  // Block-owned segments: evaluate block b appends the proposals of its
+  // items to candidate segment b (entries [b * cand_cap, + seg_cand[b]));
+  // apply block b walks segment b and appends the clusters it modified to
+  // dirty segment b (capacity dirty_cap, per round) and the items it moved to
+  // moved segment b (capacity cand_cap, per pass).
+  int32_t n_seg;               // segments = evaluate / apply blocks
+  int32_t cand_cap;
   int32_t dirty_cap;
-  int32_t moved_cap;
+  int32_t* cand;               // item of each candidate
+  int32_t* seg_cand;
+  int32_t* dirty;
+  int32_t* seg_dirty;
+  int32_t* moved;
+  int32_t* seg_moved;
   // scan tiles
   const int32_t* tile_items;   // n_tiles * kTileItems item ids, -1 padding
   const int32_t* item_tile;    // tile of each item
@@ -142,10 +155,10 @@ hipError_t launch_scan(hipStream_t s, const SweepArgs& a);
 hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask);
 hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbox, int32_t* tile_items,
                              int32_t* item_tile, void** tmp, size_t* tmp_bytes);
-hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid);
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round);
-hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a, int grid);
-hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid, int round);
+hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem);
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round);
+hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a);
+hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int round);
 hipError_t launch_controller(hipStream_t s, const SweepArgs& a);
 hipError_t launch_fill_round(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* pending);
 hipError_t launch_cc(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int32_t* parent,

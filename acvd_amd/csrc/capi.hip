@@ -104,8 +104,8 @@ struct acvd_ctx {
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T, d_evec;
   DBuf<int32_t> d_cand_tgt, d_cand_lab, d_cand_since, d_cl_round;
-  DBuf<int32_t> d_work, d_cand, d_dirty, d_moved;
-  int32_t slice_cap = 0, cand_cap = 0, dirty_cap = 0, moved_cap = 0;
+  DBuf<int32_t> d_work, d_work_count, d_cand, d_seg_cand, d_dirty, d_seg_dirty, d_moved, d_seg_moved;
+  int32_t work_cap = 0, n_seg = 0, cand_cap = 0, dirty_cap = 0;
   // scan tiles
   int32_t n_tiles = 0;
   DBuf<int32_t> d_tile_items, d_item_tile, d_tile_mesh, d_tile_cnt, d_tile_set, d_tile_prop;
@@ -167,13 +167,17 @@ struct acvd_ctx {
     a.cand_since = d_cand_since.p;
     a.cl_round = d_cl_round.p;
     a.work = d_work.p;
-    a.slice_cap = slice_cap;
+    a.work_count = d_work_count.p;
+    a.work_cap = work_cap;
+    a.n_seg = n_seg;
     a.cand_cap = cand_cap;
-    a.cand = d_cand.p;
-    a.dirty = d_dirty.p;
-    a.moved = d_moved.p;
     a.dirty_cap = dirty_cap;
-    a.moved_cap = moved_cap;
+    a.seg_cand = d_seg_cand.p;
+    a.dirty = d_dirty.p;
+    a.seg_dirty = d_seg_dirty.p;
+    a.moved = d_moved.p;
+    a.seg_moved = d_seg_moved.p;
+    a.cand = d_cand.p;
     a.ctrl = d_ctrl.p;
     a.tile_items = d_tile_items.p;
     a.item_tile = d_item_tile.p;
@@ -235,7 +239,7 @@ bool morton_tiles() {
 }
 
 // eval grid (blocks of kBlock threads, grid-stride over the work list)
-int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 2048)); }
+int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxSeg)); }
 
 // ------------------------------------------------------------------- build
 int build(acvd_ctx* c) {
@@ -452,29 +456,31 @@ int build(acvd_ctx* c) {
   HIP_TRY(c->d_geom.alloc(K));
   HIP_TRY(c->d_cl_round.alloc(K));
   {
-    // scan tiles are dealt round-robin to the slices
-    const int64_t blocks = c->n_tiles;
-    c->slice_cap = int32_t(((blocks + kSlices - 1) / kSlices) * kTileItems);
-    HIP_TRY(c->d_work.alloc(size_t(c->slice_cap) * kSlices));
-    // evaluate blocks are dealt round-robin to the candidate slices; a block
-    // appends at most kBlock proposals per grid-stride step
-    auto slice_bound = [&](int64_t grid) {
-      const int64_t steps = (N + grid * kBlock - 1) / (grid * kBlock);
-      return ((grid + kSlices - 1) / kSlices) * steps * kBlock;
-    };
-    const int64_t grid = sweep_grid_for(N);
-    c->cand_cap = int32_t(slice_bound(grid));
-    HIP_TRY(c->d_cand.alloc(size_t(c->cand_cap) * kSlices));
-    HIP_TRY(c->d_cand_tgt.alloc(size_t(c->cand_cap) * kSlices));
-    HIP_TRY(c->d_cand_lab.alloc(size_t(c->cand_cap) * kSlices));
-    HIP_TRY(c->d_cand_since.alloc(size_t(c->cand_cap) * kSlices));
-    // apply blocks (round grid, grid-stride over at most N candidates) append
-    // at most one moved item and two dirty clusters per candidate they visit
-    const int64_t rgrid = grid;
-    c->moved_cap = int32_t(slice_bound(rgrid));
-    c->dirty_cap = int32_t(2 * slice_bound(rgrid));
-    HIP_TRY(c->d_moved.alloc(size_t(c->moved_cap) * kSlices));
-    HIP_TRY(c->d_dirty.alloc(size_t(c->dirty_cap) * kSlices));
+    // work slices: scan tiles are dealt round-robin to the slices
+    c->work_cap = int32_t(((int64_t(c->n_tiles) + kWorkSlices - 1) / kWorkSlices) * kTileItems);
+    HIP_TRY(c->d_work.alloc(size_t(c->work_cap) * kWorkSlices));
+    HIP_TRY(c->d_work_count.alloc(kWorkSlices));
+    HIP_TRY(hipMemsetAsync(c->d_work_count.p, 0, kWorkSlices * sizeof(int32_t), s));
+    // candidate segments: evaluate block b takes work positions
+    // b * kBlock + j * n_seg * kBlock (at most N of them) and proposes at most
+    // once per item; apply block b appends two dirty clusters per move and
+    // one moved item per candidate (once per pass)
+    c->n_seg = int32_t(sweep_grid_for(N));
+    const int64_t steps = (N + int64_t(c->n_seg) * kBlock - 1) / (int64_t(c->n_seg) * kBlock);
+    const int64_t cap = std::max<int64_t>(1, steps) * kBlock;
+    if (2 * cap * c->n_seg > (int64_t(1) << 31) - 1) return fail(ACVD_ERR_ARG, "too many items for one context");
+    c->cand_cap = int32_t(cap);
+    c->dirty_cap = int32_t(2 * cap);
+    const size_t nc = size_t(c->cand_cap) * size_t(c->n_seg);
+    HIP_TRY(c->d_cand.alloc(nc));
+    HIP_TRY(c->d_cand_tgt.alloc(nc));
+    HIP_TRY(c->d_cand_lab.alloc(nc));
+    HIP_TRY(c->d_cand_since.alloc(nc));
+    HIP_TRY(c->d_moved.alloc(nc));
+    HIP_TRY(c->d_dirty.alloc(2 * nc));
+    HIP_TRY(c->d_seg_cand.alloc(c->n_seg));
+    HIP_TRY(c->d_seg_dirty.alloc(c->n_seg));
+    HIP_TRY(c->d_seg_moved.alloc(c->n_seg));
   }
   HIP_TRY(c->d_parent.alloc(N));
   HIP_TRY(c->d_comp_size.alloc(N));
@@ -505,10 +511,6 @@ int build(acvd_ctx* c) {
 }
 
 // ------------------------------------------------------------ minimisation
-int sweep_grid(const acvd_ctx* c) { return int(sweep_grid_for(c->N)); }
-
-// grid of the accept rounds (grid-stride over the candidate list)
-int round_grid(const acvd_ctx* c) { return sweep_grid(c); }
 
 int fill_until_stable(acvd_ctx* c, const int32_t* mask, int max_rounds) {
   hipStream_t s = c->stream;
@@ -536,17 +538,15 @@ int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsign
 int enqueue_chunk(acvd_ctx* c) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
-  const int grid = sweep_grid(c);
-  const int rgrid = round_grid(c);
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(launch_scan(s, a));
-    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
-    HIP_TRY(launch_apply(s, a, rgrid, 0));
+    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem));
+    HIP_TRY(launch_apply(s, a, 0));
     for (int r = 1; r < kMatchRounds; ++r) {
-      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, rgrid, r));
-      HIP_TRY(launch_apply(s, a, rgrid, r));
+      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, r));
+      HIP_TRY(launch_apply(s, a, r));
     }
-    HIP_TRY(launch_tile_update(s, a, rgrid));
+    HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(launch_controller(s, a));
   }
   return ACVD_OK;
@@ -556,21 +556,20 @@ int enqueue_chunk(acvd_ctx* c) {
 int run_chunk_timed(acvd_ctx* c) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
-  const int grid = sweep_grid(c);
   std::vector<hipEvent_t> ev(5 * kChunk);
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(hipEventRecord(ev[5 * p], s));
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
-    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, grid));
+    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
-    HIP_TRY(launch_apply(s, a, round_grid(c), 0));
+    HIP_TRY(launch_apply(s, a, 0));
     for (int r = 1; r < kMatchRounds; ++r) {
-      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, round_grid(c), r));
-      HIP_TRY(launch_apply(s, a, round_grid(c), r));
+      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, r));
+      HIP_TRY(launch_apply(s, a, r));
     }
-    HIP_TRY(launch_tile_update(s, a, round_grid(c)));
+    HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
     HIP_TRY(launch_controller(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 4], s));
@@ -657,6 +656,7 @@ int minimize_energy(acvd_ctx* c) {
   HIP_TRY(launch_fill_i32(s, c->d_cl_round.p, c->K, -1));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
+  HIP_TRY(hipMemsetAsync(c->d_work_count.p, 0, kWorkSlices * sizeof(int32_t), s));
   for (int k = 0; k < 4; ++k) {
     c->kernel_ms[k] = 0.0;
     c->kernel_launches[k] = 0;

@@ -180,10 +180,11 @@ __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t la
 }
 
 // Scan: items of sweeping meshes that must be evaluated this pass are
-// compacted into the work list.  One wavefront per tile: the lanes first test
-// the tile's cluster set against the cluster stamps (and the tile's last
-// proposal loop); only an active tile reads its items and CSR rows.  The test
-// is conservative: every item needs_eval would select lies in an active tile.
+// compacted into the work list (slice t % kWorkSlices of tile t).  One
+// wavefront per tile: the lanes first test the tile's cluster set against the
+// cluster stamps (and the tile's last proposal loop); only an active tile
+// reads its items and CSR rows.  The test is conservative: every item
+// needs_eval would select lies in an active tile.
 __global__ __launch_bounds__(64) void k_scan(SweepArgs a) {
   const int32_t t = blockIdx.x;
   const int lane = threadIdx.x;
@@ -218,15 +219,15 @@ __global__ __launch_bounds__(64) void k_scan(SweepArgs a) {
     tot += __popcll(bal[r]);
   }
   if (tot == 0) return;
-  const int slice = t % kSlices;
+  const int slice = t % kWorkSlices;
   int32_t base = 0;
-  if (lane == 0) base = atomicAdd(&a.ctrl->work_count[slice], tot);
+  if (lane == 0) base = atomicAdd(&a.work_count[slice], tot);
   base = __shfl(base, 0);
   const unsigned long long below = (1ull << lane) - 1ull;
   int32_t off = 0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    if (want[r]) a.work[int64_t(slice) * a.slice_cap + base + off + __popcll(bal[r] & below)] = it[r];
+    if (want[r]) a.work[int64_t(slice) * a.work_cap + base + off + __popcll(bal[r] & below)] = it[r];
     off += __popcll(bal[r]);
   }
 }
@@ -278,57 +279,122 @@ __global__ __launch_bounds__(64) void k_tile_sets(SweepArgs a, const int32_t* __
   }
   if (threadIdx.x == 0) a.tile_cnt[t] = big ? kTileSet + 1 : n_uniq;
 }
-// Maps a position v of the concatenation of a sliced list (kSlices slices of
-// capacity cap, fill counts in counts[]) to its entry.
-struct SliceIndex {
-  int32_t start[kSlices + 1];
+// ------------------------------------------------------------------ propose
+// Work slices: a scan tile appends its work items to slice t % kWorkSlices;
+// many slices keep the same-address counter atomics few.  A consumer block
+// maps a position of the concatenation to its entry with an LDS index of the
+// slice starts.
+struct WorkIndex {
+  int32_t start[kWorkSlices + 1];
 };
 
-__device__ inline void load_slice_index(const int32_t* counts, SliceIndex* wi) {
-  if (threadIdx.x == 0) {
-    int32_t s = 0;
-    for (int k = 0; k < kSlices; ++k) {
-      wi->start[k] = s;
-      s += counts[k];
-    }
-    wi->start[kSlices] = s;
+__device__ inline void load_work_index(const int32_t* counts, WorkIndex* wi) {
+  constexpr int kPer = kWorkSlices / kBlock;
+  static_assert(kWorkSlices % kBlock == 0, "work slices per thread");
+  __shared__ int32_t s_wave[kBlock / 64];
+  int32_t v[kPer];
+  int32_t tot = 0;
+  for (int k = 0; k < kPer; ++k) {
+    v[k] = counts[threadIdx.x * kPer + k];
+    tot += v[k];
   }
+  // inclusive scan of the thread totals over the wave, then over the waves
+  const int lane = __lane_id(), wv = threadIdx.x >> 6;
+  int32_t inc = tot;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t u = __shfl_up(inc, off);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) s_wave[wv] = inc;
+  __syncthreads();
+  int32_t before = inc - tot;
+  for (int w = 0; w < wv; ++w) before += s_wave[w];
+  for (int k = 0; k < kPer; ++k) {
+    wi->start[threadIdx.x * kPer + k] = before;
+    before += v[k];
+  }
+  if (threadIdx.x == kBlock - 1) wi->start[kWorkSlices] = before;
   __syncthreads();
 }
 
-// Storage index of position v of the concatenation.
-__device__ inline int64_t slice_pos(int32_t cap, const SliceIndex* wi, int32_t v) {
-  int lo = 0, hi = kSlices;  // find the slice k with start[k] <= v < start[k+1]
+__device__ inline int32_t work_entry(const SweepArgs& a, const WorkIndex* wi, int32_t p) {
+  int lo = 0, hi = kWorkSlices;  // slice k with start[k] <= p < start[k+1]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (wi->start[mid] <= v) lo = mid;
+    if (wi->start[mid] <= p) lo = mid;
     else hi = mid;
   }
-  return int64_t(lo) * cap + (v - wi->start[lo]);
+  return a.work[int64_t(lo) * a.work_cap + (p - wi->start[lo])];
 }
 
-__device__ inline int32_t slice_entry(const int32_t* list, int32_t cap, const SliceIndex* wi, int32_t v) {
-  return list[slice_pos(cap, wi, v)];
+// Candidate segments seen as one list: an LDS index of the segment starts
+// (prefix of the n_seg <= kMaxSeg fill counts), so that consumer blocks walk
+// all candidates evenly whatever their segments.
+struct SegIndex {
+  int32_t start[kMaxSeg + 1];
+};
+
+__device__ inline void load_seg_index(const int32_t* counts, int32_t n_seg, SegIndex* si) {
+  constexpr int kPer = kMaxSeg / kBlock;
+  __shared__ int32_t s_wave[kBlock / 64];
+  int32_t v[kPer];
+  int32_t tot = 0;
+  for (int k = 0; k < kPer; ++k) {
+    const int32_t j = threadIdx.x * kPer + k;
+    v[k] = j < n_seg ? counts[j] : 0;
+    tot += v[k];
+  }
+  const int lane = __lane_id(), wv = threadIdx.x >> 6;
+  int32_t inc = tot;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t u = __shfl_up(inc, off);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) s_wave[wv] = inc;
+  __syncthreads();
+  int32_t before = inc - tot;
+  for (int w = 0; w < wv; ++w) before += s_wave[w];
+  for (int k = 0; k < kPer; ++k) {
+    si->start[threadIdx.x * kPer + k] = before;
+    before += v[k];
+  }
+  if (threadIdx.x == kBlock - 1) si->start[kMaxSeg] = before;
+  __syncthreads();
 }
 
-// Wave-aggregated append of the proposal (item i: lab -> target, evaluated at
-// round tag `since`) to slice `slice` of the candidate list, with its record;
-// every lane of the wave must call it.
-__device__ inline void cand_append(const SweepArgs& a, int slice, bool want, int32_t i, int32_t lab, int32_t target,
-                                   int32_t since) {
-  const unsigned long long bal = __ballot(want);
-  if (bal == 0ull) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)bal) - 1;
-  int32_t base = 0;
-  if (lane == leader) base = atomicAdd(&a.ctrl->cand_count[slice], __popcll(bal));
-  base = __shfl(base, leader);
-  if (want) {
-    const int64_t e = int64_t(slice) * a.cand_cap + base + __popcll(bal & ((1ull << lane) - 1ull));
-    a.cand[e] = i;
-    a.cand_lab[e] = lab;
-    a.cand_tgt[e] = target;
-    a.cand_since[e] = since;
+// Storage index (segment * cap + offset) of position p of the list.
+__device__ inline int64_t seg_pos(const SegIndex* si, int32_t cap, int32_t p) {
+  int lo = 0, hi = kMaxSeg;  // segment k with start[k] <= p < start[k+1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (si->start[mid] <= p) lo = mid;
+    else hi = mid;
+  }
+  return int64_t(lo) * cap + (p - si->start[lo]);
+}
+
+// Walks the candidate list, calling f(valid, entry) for one position per
+// thread per step (every thread of the block takes part in every step).  A
+// long list (last pass's proposals above kSegWalkMax) is walked evenly by all
+// blocks through the LDS index; a short one by block b over segment b alone,
+// which spares the index (grid = n_seg).  The results do not depend on the
+// walk.
+template <class F>
+__device__ inline void walk_candidates(const SweepArgs& a, SegIndex* si, F&& f) {
+  if (a.ctrl->seg_walk) {
+    const int32_t n = a.seg_cand[blockIdx.x];
+    const int64_t seg = int64_t(blockIdx.x) * a.cand_cap;
+    for (int32_t base = 0; base < n; base += kBlock) {
+      const int32_t k = base + int32_t(threadIdx.x);
+      f(k < n, seg + k);
+    }
+  } else {
+    load_seg_index(a.seg_cand, a.n_seg, si);
+    const int32_t n = si->start[kMaxSeg];
+    for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+      const int32_t k = base + int32_t(threadIdx.x);
+      f(k < n, k < n ? seg_pos(si, a.cand_cap, k) : int64_t(0));
+    }
   }
 }
 
@@ -381,54 +447,70 @@ __device__ inline void keyagg_flush(KeyAgg* g, unsigned long long* prio) {
   __syncthreads();
 }
 
-// Round-0 bookkeeping of a proposal: round-0 key at both clusters (through
-// the block's key table), item stamp, tile activity, per-mesh proposal count
-// (batches).
-__device__ inline void register_round0(const SweepArgs& a, KeyAgg* agg, MeshState* st, const DevMesh& dm, int64_t i,
-                                       int32_t lab, int32_t target) {
-  const int32_t L = st->loops;
-  const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
-  if (lab != kNull) keyagg_min(agg, lab, key);
-  keyagg_min(agg, target, key);
-  a.cand_pass[i] = L;
-  a.tile_prop[a.item_tile[i]] = L;
-  if (a.item_mesh) atomicAdd(&st->pass_props, 1);
+// Wave-aggregated append of v to a block-owned list segment (fill count in
+// LDS); every lane of the wave calls it.  Returns the entry index (when want).
+__device__ inline int64_t seg_append(int64_t seg, int32_t* s_n, bool want) {
+  const unsigned long long bal = __ballot(want);
+  if (bal == 0ull) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)bal) - 1;
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(s_n, __popcll(bal));
+  base = __shfl(base, leader);
+  return want ? seg + base + __popcll(bal & ((1ull << lane) - 1ull)) : -1;
 }
 
 // Evaluate (round 0): best move of each work item (shared rule of
 // acvd_math.h) against the state at the pass start; a proposal registers its
-// round-0 accept key at both clusters and joins the candidate list.
+// round-0 accept key at both clusters and joins the block's candidate
+// segment.  Block b owns segment b (capacity cand_cap, count seg_cand[b]);
+// the second launch of a mixed batch (append = 1) continues the segments of
+// the first.
 template <int METRIC>
-__global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
-  __shared__ SliceIndex wi;
+__global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
+  __shared__ WorkIndex wi;
   __shared__ KeyAgg agg;
+  __shared__ int32_t s_n;
+  if (threadIdx.x == 0) s_n = append ? a.seg_cand[blockIdx.x] : 0;
   keyagg_clear(&agg);
-  load_slice_index(a.ctrl->work_count, &wi);
-  const int32_t n = wi.start[kSlices];
+  load_work_index(a.work_count, &wi);
+  const int32_t n = wi.start[kWorkSlices];
   const SweepView v = make_view(a);
-  const int slice = blockIdx.x % kSlices;
+  const int64_t seg = int64_t(blockIdx.x) * a.cand_cap;
   for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
     const int32_t t = base + int32_t(threadIdx.x);
     bool want = false;
-    int64_t i = 0;
-    int32_t lab = kNull, target = kNull, since = 0;
+    int32_t i = 0, lab = kNull, target = kNull, L = 0;
     if (t < n) {
-      i = slice_entry(a.work, a.slice_cap, &wi, t);
+      i = work_entry(a, &wi, t);
       const int m = mesh_of_item(a, i);
       const DevMesh& dm = a.meshes[m];
       if (dm.metric == METRIC) {
         MeshState* st = &a.state[m];
+        L = st->loops;
         lab = a.labels[i];
         if (propose_item<METRIC>(v, i, lab, st->connexity, st->constrained, dm.level, dm.scales, &target)) {
-          register_round0(a, &agg, st, dm, i, lab, target);
-          since = st->loops * kMatchRounds;
+          const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
+          if (lab != kNull) keyagg_min(&agg, lab, key);
+          keyagg_min(&agg, target, key);
+          a.cand_pass[i] = L;
+          a.tile_prop[a.item_tile[i]] = L;
+          if (a.item_mesh) atomicAdd(&st->pass_props, 1);
           want = true;
         }
       }
     }
-    cand_append(a, slice, want, int32_t(i), lab, target, since);
+    const int64_t e = seg_append(seg, &s_n, want);
+    if (want) {
+      a.cand[e] = i;
+      a.cand_lab[e] = lab;
+      a.cand_tgt[e] = target;
+      a.cand_since[e] = L * kMatchRounds;
+    }
     keyagg_flush(&agg, a.prio);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) a.seg_cand[blockIdx.x] = s_n;
 }
 
 // Re-evaluate (round r > 0): a live candidate (cand_tgt >= 0) whose own
@@ -438,16 +520,13 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a) {
 // dE depends only on the two unchanged clusters, and register it again.
 template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
-  __shared__ SliceIndex ci;
   __shared__ KeyAgg agg;
+  __shared__ SegIndex si;
   keyagg_clear(&agg);
-  load_slice_index(a.ctrl->cand_count, &ci);
-  const int32_t n = ci.start[kSlices];
+  __syncthreads();
   const SweepView v = make_view(a);
-  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-    const int32_t t = base + int32_t(threadIdx.x);
-    const int64_t e = t < n ? slice_pos(a.cand_cap, &ci, t) : 0;
-    const int32_t b = t < n ? a.cand_tgt[e] : -1;
+  walk_candidates(a, &si, [&](bool valid, int64_t e) {
+    const int32_t b = valid ? a.cand_tgt[e] : -1;
     if (b >= 0) {
       const int64_t i = a.cand[e];
       const int32_t lab = a.cand_lab[e];  // a live candidate has not moved
@@ -473,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
       }
     }
     keyagg_flush(&agg, a.prio);
-  }
+  });
 }
 
 // --------------------------------------------------------------------- apply
@@ -510,105 +589,96 @@ __device__ inline int block_sum(int v) {
   return tot;  // valid in thread 0
 }
 
-// Wave-aggregated append of v (when want) to slice `slice` of a sliced list.
-__device__ inline void list_append(int32_t* list, int32_t cap, int32_t* counts, int slice, bool want, int32_t v) {
-  const unsigned long long bal = __ballot(want);
-  if (bal == 0ull) return;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)bal) - 1;
-  int32_t base = 0;
-  if (lane == leader) base = atomicAdd(&counts[slice], __popcll(bal));
-  base = __shfl(base, leader);
-  if (want) list[int64_t(slice) * cap + base + __popcll(bal & ((1ull << lane) - 1ull))] = v;
-}
-
 // Apply round `round`: a live candidate holding the minimum round key at both
 // of its clusters moves: label, fixed-point accumulator deltas (plain
-// read-modify-write: the round owns both clusters), sizes and stamps.  The two
-// clusters join the round's dirty list (their centroid and T are derived by
-// k_derive_dirty) and the item joins the pass's moved list (tile sets are
-// extended by k_tile_update).  Counts go to the sliced counters (single mesh)
-// or per mesh (batches).
+// read-modify-write: the round owns both clusters), sizes and stamps.  The
+// blocks walk all candidates evenly; the two clusters join the block's dirty
+// segment (their centroid and T are derived by k_derive_dirty) and the item
+// its moved segment (tile sets are extended by k_tile_update after the pass).
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
-  __shared__ SliceIndex ci;
-  load_slice_index(a.ctrl->cand_count, &ci);
-  const int32_t n = ci.start[kSlices];
-  const int slice = blockIdx.x % kSlices;
+  __shared__ int32_t s_dirty, s_moved;
+  __shared__ SegIndex si;
+  const int64_t dseg = int64_t(blockIdx.x) * a.dirty_cap, mseg = int64_t(blockIdx.x) * a.cand_cap;
+  if (threadIdx.x == 0) {
+    s_dirty = 0;
+    s_moved = round == 0 ? 0 : a.seg_moved[blockIdx.x];
+  }
+  __syncthreads();
   int mods = 0;
-  for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-    const int32_t t = base + int32_t(threadIdx.x);
+  walk_candidates(a, &si, [&](bool valid, int64_t e) {
     bool win = false;
     int64_t i = 0;
-    int32_t lab = kNull, b = kNull;
-    if (t < n) {
-      const int64_t e = slice_pos(a.cand_cap, &ci, t);
-      b = a.cand_tgt[e];
-      if (b >= 0) {
-        i = a.cand[e];
-        lab = a.cand_lab[e];
-        const int m = mesh_of_item(a, i);
-        MeshState* st = &a.state[m];
-        const int32_t L = st->loops;
-        const DevMesh& dm = a.meshes[m];
-        const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
-        win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
-        if (win) {
-          // exclusive owner of clusters lab and b for this round
-          a.labels[i] = b;
-          a.cand_tgt[e] = -3;
-          const float4 p = a.xw[i];
-          const Scales sc = dm.scales;
-          const unsigned long long iw = (unsigned long long)fix_w(p.w, sc);
-          const unsigned long long is[3] = {(unsigned long long)fix_s(p.w, p.x, sc),
-                                            (unsigned long long)fix_s(p.w, p.y, sc),
-                                            (unsigned long long)fix_s(p.w, p.z, sc)};
-          const int32_t rtag = L * kMatchRounds + round;
-          // plain read-modify-write: no other thread of this round touches
-          // the accumulators of lab or b
-          if (lab != kNull) {
-            unsigned long long* wa = a.acc_ws + 4 * int64_t(lab);
-            wa[0] -= iw;
-            for (int k = 0; k < 3; ++k) wa[1 + k] -= is[k];
-            a.sizes[lab] -= 1;
-            a.stamp[lab] = L;
-            a.cl_round[lab] = rtag;
-          }
-          unsigned long long* wb = a.acc_ws + 4 * int64_t(b);
-          wb[0] += iw;
-          for (int k = 0; k < 3; ++k) wb[1 + k] += is[k];
-          if (dm.metric == 1) {
-            unsigned long long* qa = lab != kNull ? a.acc_q + 10 * int64_t(lab) : nullptr;
-            unsigned long long* qb = a.acc_q + 10 * int64_t(b);
-            for (int k = 0; k < 10; ++k) {
-              const unsigned long long iq = (unsigned long long)fix_q(a.q[10 * i + k], sc);
-              if (qa) qa[k] -= iq;
-              qb[k] += iq;
-            }
-          }
-          a.sizes[b] += 1;
-          a.stamp[b] = L;
-          a.cl_round[b] = rtag;
-          ++mods;
-          if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
+    int32_t lab = kNull;
+    const int32_t b = valid ? a.cand_tgt[e] : -1;
+    if (b >= 0) {
+      i = a.cand[e];
+      lab = a.cand_lab[e];
+      const int m = mesh_of_item(a, i);
+      MeshState* st = &a.state[m];
+      const int32_t L = st->loops;
+      const DevMesh& dm = a.meshes[m];
+      const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
+      win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
+      if (win) {
+        // exclusive owner of clusters lab and b for this round
+        a.labels[i] = b;
+        a.cand_tgt[e] = -3;
+        const float4 p = a.xw[i];
+        const Scales sc = dm.scales;
+        const unsigned long long iw = (unsigned long long)fix_w(p.w, sc);
+        const unsigned long long is[3] = {(unsigned long long)fix_s(p.w, p.x, sc),
+                                          (unsigned long long)fix_s(p.w, p.y, sc),
+                                          (unsigned long long)fix_s(p.w, p.z, sc)};
+        const int32_t rtag = L * kMatchRounds + round;
+        if (lab != kNull) {
+          unsigned long long* wa = a.acc_ws + 4 * int64_t(lab);
+          wa[0] -= iw;
+          for (int q = 0; q < 3; ++q) wa[1 + q] -= is[q];
+          a.sizes[lab] -= 1;
+          a.stamp[lab] = L;
+          a.cl_round[lab] = rtag;
         }
+        unsigned long long* wb = a.acc_ws + 4 * int64_t(b);
+        wb[0] += iw;
+        for (int q = 0; q < 3; ++q) wb[1 + q] += is[q];
+        if (dm.metric == 1) {
+          unsigned long long* qa = lab != kNull ? a.acc_q + 10 * int64_t(lab) : nullptr;
+          unsigned long long* qb = a.acc_q + 10 * int64_t(b);
+          for (int q = 0; q < 10; ++q) {
+            const unsigned long long iq = (unsigned long long)fix_q(a.q[10 * i + q], sc);
+            if (qa) qa[q] -= iq;
+            qb[q] += iq;
+          }
+        }
+        a.sizes[b] += 1;
+        a.stamp[b] = L;
+        a.cl_round[b] = rtag;
+        ++mods;
+        if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
       }
     }
-    list_append(a.dirty, a.dirty_cap, a.ctrl->dirty_count[round], slice, win && lab != kNull, lab);
-    list_append(a.dirty, a.dirty_cap, a.ctrl->dirty_count[round], slice, win, b);
-    list_append(a.moved, a.moved_cap, a.ctrl->moved_count, slice, win, int32_t(i));
+    const int64_t ea = seg_append(dseg, &s_dirty, win && lab != kNull);
+    if (ea >= 0) a.dirty[ea] = lab;
+    const int64_t eb = seg_append(dseg, &s_dirty, win);
+    if (eb >= 0) a.dirty[eb] = b;
+    const int64_t em = seg_append(mseg, &s_moved, win);
+    if (em >= 0) a.moved[em] = int32_t(i);
+  });
+  const int tm = block_sum(mods);  // synchronises the block
+  if (threadIdx.x == 0) {
+    a.seg_dirty[blockIdx.x] = s_dirty;
+    a.seg_moved[blockIdx.x] = s_moved;
+    if (tm) atomicAdd(&a.ctrl->mods[round][blockIdx.x % kSlices], tm);
   }
-  const int tm = block_sum(mods);
-  if (threadIdx.x == 0 && tm) atomicAdd(&a.ctrl->mods[round][blockIdx.x % kSlices], tm);
 }
 
-// Centroid, W and (QEM) T of the clusters a round modified, from their final
-// accumulators.
-__global__ __launch_bounds__(kBlock) void k_derive_dirty(SweepArgs a, int round) {
-  __shared__ SliceIndex di;
-  load_slice_index(a.ctrl->dirty_count[round], &di);
-  const int32_t n = di.start[kSlices];
-  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    const int64_t c = slice_entry(a.dirty, a.dirty_cap, &di, t);
+// Centroid, W and (QEM) T of the clusters the apply block of the same index
+// modified in this round, from their final accumulators.
+__global__ __launch_bounds__(kBlock) void k_derive_dirty(SweepArgs a) {
+  const int32_t n = a.seg_dirty[blockIdx.x];
+  const int64_t seg = int64_t(blockIdx.x) * a.dirty_cap;
+  for (int32_t k = threadIdx.x; k < n; k += kBlock) {
+    const int64_t c = a.dirty[seg + k];
     const int m = mesh_of_cluster(a, c);
     derive_cluster(a, c, a.meshes[m], a.state[m].constrained);
   }
@@ -617,16 +687,15 @@ __global__ __launch_bounds__(kBlock) void k_derive_dirty(SweepArgs a, int round)
 // Tile sets after a pass: an item that moved to cluster b shows b to its own
 // tile and to the tiles of its neighbours (k_scan of the next pass reads them).
 __global__ __launch_bounds__(kBlock) void k_tile_update(SweepArgs a) {
-  __shared__ SliceIndex mi;
-  load_slice_index(a.ctrl->moved_count, &mi);
-  const int32_t n = mi.start[kSlices];
-  for (int32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    const int64_t i = slice_entry(a.moved, a.moved_cap, &mi, t);
+  const int32_t n = a.seg_moved[blockIdx.x];
+  const int64_t seg = int64_t(blockIdx.x) * a.cand_cap;
+  for (int32_t k = threadIdx.x; k < n; k += kBlock) {
+    const int64_t i = a.moved[seg + k];
     const int32_t b = a.labels[i];
     const int32_t ti = a.item_tile[i];
     tile_insert(a, ti, b);
-    for (int32_t k = a.row_ptr[i]; k < a.row_ptr[i + 1]; ++k) {
-      const int32_t tj = a.item_tile[a.col[k]];
+    for (int32_t r = a.row_ptr[i]; r < a.row_ptr[i + 1]; ++r) {
+      const int32_t tj = a.item_tile[a.col[r]];
       if (tj != ti) tile_insert(a, tj, b);
     }
   }
@@ -641,26 +710,48 @@ __global__ void k_controller(SweepArgs a) {
   }
   __syncthreads();
   __shared__ int s_work, s_props, s_mods, s_round_mods[kMatchRounds];
+  __shared__ int s_part[2][kBlock / 64];
   static_assert(kSlices == 64, "one wave sums the slice counters");
-  if (threadIdx.x < 64) {  // lane k reads slice k, wave sums
-    int w = a.ctrl->work_count[threadIdx.x], p = a.ctrl->cand_count[threadIdx.x];
-    int dr[kMatchRounds];
-    for (int r = 0; r < kMatchRounds; ++r) dr[r] = a.ctrl->mods[r][threadIdx.x];
+  {  // items evaluated and proposals of the pass: sums over the work slices
+     // (cleared for the next scan) and the candidate segments
+    int w = 0, p = 0;
+    for (int k = threadIdx.x; k < kWorkSlices; k += blockDim.x) {
+      w += a.work_count[k];
+      a.work_count[k] = 0;
+    }
+    for (int k = threadIdx.x; k < a.n_seg; k += blockDim.x) p += a.seg_cand[k];
     for (int off = 32; off > 0; off >>= 1) {
       w += __shfl_xor(w, off);
       p += __shfl_xor(p, off);
-      for (int r = 0; r < kMatchRounds; ++r) dr[r] += __shfl_xor(dr[r], off);
     }
+    if (__lane_id() == 0) {
+      s_part[0][threadIdx.x >> 6] = w;
+      s_part[1][threadIdx.x >> 6] = p;
+    }
+  }
+  if (threadIdx.x < 64) {  // lane k reads slice k of the move counters, wave sums
+    int dr[kMatchRounds];
+    for (int r = 0; r < kMatchRounds; ++r) dr[r] = a.ctrl->mods[r][threadIdx.x];
+    for (int off = 32; off > 0; off >>= 1)
+      for (int r = 0; r < kMatchRounds; ++r) dr[r] += __shfl_xor(dr[r], off);
     if (threadIdx.x == 0) {
       int d = 0;
       for (int r = 0; r < kMatchRounds; ++r) {
         s_round_mods[r] = dr[r];
         d += dr[r];
       }
-      s_work = w;
-      s_props = p;
       s_mods = d;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int w = 0, p = 0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      w += s_part[0][k];
+      p += s_part[1][k];
+    }
+    s_work = w;
+    s_props = p;
   }
   __syncthreads();
   bool ran = false;
@@ -682,15 +773,8 @@ __global__ void k_controller(SweepArgs a) {
   }
   if (ran) atomicOr(&any_sweep, 2);
   __syncthreads();
-  if (threadIdx.x < kSlices) {
-    a.ctrl->work_count[threadIdx.x] = 0;
-    a.ctrl->cand_count[threadIdx.x] = 0;
-    a.ctrl->moved_count[threadIdx.x] = 0;
-    for (int r = 0; r < kMatchRounds; ++r) {
-      a.ctrl->mods[r][threadIdx.x] = 0;
-      a.ctrl->dirty_count[r][threadIdx.x] = 0;
-    }
-  }
+  if (threadIdx.x < kSlices)
+    for (int r = 0; r < kMatchRounds; ++r) a.ctrl->mods[r][threadIdx.x] = 0;
   if (threadIdx.x == 0 && (any_sweep & 2)) {
     const unsigned long long pass = a.ctrl->passes;
     int32_t* e = a.ctrl->pass_log[pass % kPassLogCap];
@@ -706,6 +790,7 @@ __global__ void k_controller(SweepArgs a) {
     a.ctrl->proposed += (unsigned long long)s_props;
     a.ctrl->accepted += (unsigned long long)s_mods;
     a.ctrl->any_sweep = any_sweep & 1;
+    a.ctrl->seg_walk = s_props <= kSegWalkMax ? 1 : 0;
     a.ctrl->any_need_clean = any_clean;
     if (any_sweep & 2) a.ctrl->passes += 1;
   }
@@ -935,26 +1020,29 @@ hipError_t launch_scan(hipStream_t s, const SweepArgs& a) {
   return hipGetLastError();
 }
 
-hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid) {
-  if (any_iso) hipLaunchKernelGGL(k_eval<0>, dim3(grid), dim3(kBlock), 0, s, a);
-  if (any_qem) hipLaunchKernelGGL(k_eval<1>, dim3(grid), dim3(kBlock), 0, s, a);
+// One evaluate block per candidate segment: block b writes segment b's count.
+hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem) {
+  if (any_iso) hipLaunchKernelGGL(k_eval<0>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, 0);
+  if (any_qem) hipLaunchKernelGGL(k_eval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
   return hipGetLastError();
 }
 
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int grid, int round) {
-  hipLaunchKernelGGL(k_apply, dim3(grid), dim3(kBlock), 0, s, a, round);
-  hipLaunchKernelGGL(k_derive_dirty, dim3(grid), dim3(kBlock), 0, s, a, round);
+// One apply block per candidate segment (its dirty and moved segments have
+// the same index), then the derive of the round's dirty clusters.
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round) {
+  hipLaunchKernelGGL(k_apply, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  hipLaunchKernelGGL(k_derive_dirty, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a, int grid) {
-  hipLaunchKernelGGL(k_tile_update, dim3(grid), dim3(kBlock), 0, s, a);
+hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a) {
+  hipLaunchKernelGGL(k_tile_update, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int grid, int round) {
-  if (any_iso) hipLaunchKernelGGL(k_reeval<0>, dim3(grid), dim3(kBlock), 0, s, a, round);
-  if (any_qem) hipLaunchKernelGGL(k_reeval<1>, dim3(grid), dim3(kBlock), 0, s, a, round);
+hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int round) {
+  if (any_iso) hipLaunchKernelGGL(k_reeval<0>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  if (any_qem) hipLaunchKernelGGL(k_reeval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
   return hipGetLastError();
 }
 
