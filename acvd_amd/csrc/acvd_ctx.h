@@ -84,6 +84,7 @@ struct SweepArgs {
   const int32_t* clus_mesh;  // null when n_meshes == 1
   const int32_t* row_ptr;    // n_items + 1
   const int32_t* col;        // global item ids
+  const unsigned long long* ring_adj;  // ring adjacency per item (ring_adjacency of acvd_math.h)
   const float4* xw;          // position + weight
   const float* q;            // 10 floats per item (QEM meshes), may be null
   int32_t* labels;
@@ -139,6 +140,8 @@ hipError_t launch_csr_compact(hipStream_t s, int64_t n, const int32_t* tptr, con
                               const int32_t* row_ptr, int32_t* col);
 hipError_t exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, void** tmp,
                               size_t* tmp_bytes);
+hipError_t launch_ring_adj(hipStream_t s, int64_t n, const int32_t* row_ptr, const int32_t* col,
+                            unsigned long long* ring_adj);
 hipError_t launch_item_build(hipStream_t s, const SweepArgs& a, const double* pos, const int32_t* tri_global,
                              const int32_t* vfptr, const int32_t* vfcol, const double* indicator,
                              double* raw_w, float4* xw, float* q);

@@ -362,8 +362,10 @@ ACVD_HD bool ring_problem_rows(const int32_t* rp, const int32_t* col, const int3
 // short (the usual case) are checked with all row entries loaded at once into
 // registers and an all-pairs intersection (the rows hold distinct ids, so it
 // finds exactly what the merge-walk of sorted rows finds), which replaces the
-// chain of dependent loads of the walk; longer rows take the walk.
-constexpr int kRingCache = 12;
+// chain of dependent loads of the walk; longer rows take the walk.  The same
+// bound selects the register-resident QEM rule (valence <= 8 covers the
+// usual meshes and keeps the evaluate kernels at 3 waves per SIMD).
+constexpr int kRingCache = 8;
 ACVD_HD bool ring_problem(const int32_t* rp, const int32_t* col, const int32_t* lab, int32_t i, int32_t a) {
   const int32_t r0 = rp[i], r1 = rp[i + 1];
   const int32_t deg = r1 - r0;
@@ -440,6 +442,46 @@ ACVD_HD bool ring_problem_rows(const int32_t* rp, const int32_t* col, const int3
   return reach != M;
 }
 
+// Ring adjacency of item i (rows of at most kRingAdjMax neighbours): bit
+// 8 * p + q is set when ring neighbours p and q (positions in the CSR row of
+// i) share a mesh edge.  A property of the mesh alone, built once; with it
+// the ring check needs only the labels of the ring neighbours.
+constexpr int kRingAdjMax = 8;
+ACVD_HD uint64_t ring_adjacency(const int32_t* rp, const int32_t* col, int32_t i) {
+  const int32_t r0 = rp[i], deg = rp[i + 1] - r0;
+  if (deg > kRingAdjMax) return 0;
+  uint64_t adj = 0;
+  for (int p = 0; p < deg; ++p) {
+    const int32_t j = col[r0 + p];
+    for (int32_t k = rp[j]; k < rp[j + 1]; ++k) {
+      const int32_t u = col[k];
+      for (int q = 0; q < deg; ++q)
+        if (col[r0 + q] == u) adj |= uint64_t(1) << (8 * p + q);
+    }
+  }
+  return adj;
+}
+
+// The ring check from the ring adjacency and the ring labels nb[0..deg)
+// (deg <= kRingAdjMax): do the neighbours labelled a form one connected set
+// through edges between themselves?  Same answer as ring_problem.
+ACVD_HD bool ring_problem_mask(uint64_t adj, const int32_t* nb, int deg, int32_t a) {
+  uint32_t M = 0;
+  for (int p = 0; p < kRingAdjMax; ++p)
+    if (p < deg && nb[p] == a) M |= 1u << p;
+  if (M == 0) return false;
+  uint32_t reach = M & (~M + 1u);
+  uint32_t frontier = reach;
+  while (frontier) {
+    const int t = __builtin_ctz(frontier);
+    frontier &= frontier - 1;
+    const uint32_t add = uint32_t(adj >> (8 * t)) & 0xffu & M & ~reach;
+    reach |= add;
+    frontier |= add;
+  }
+  return reach != M;
+}
+
 // ---------------------------------------------------------------------------
 // Read-only view of the sweep state used by the propose rule.  On the device
 // the pointers are global memory; in the replay they are host arrays.
@@ -457,7 +499,21 @@ struct SweepView {
   const int32_t* sizes;
   const int32_t* stamp;
   const int32_t* cand_pass;
+  const unsigned long long* ring_adj;  // ring adjacency per item (rows <= kRingAdjMax), or null
 };
+
+// Ring check of item i leaving cluster a: with the ring adjacency for short
+// rows, else by the row walk.
+ACVD_HD bool ring_check(const SweepView& v, int64_t i, int32_t a) {
+  const int32_t r0 = v.row_ptr[i], deg = v.row_ptr[i + 1] - r0;
+  if (v.ring_adj && deg <= kRingAdjMax) {
+    int32_t nb[kRingAdjMax];
+    ACVD_UNROLL
+    for (int p = 0; p < kRingAdjMax; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
+    return ring_problem_mask(v.ring_adj[i], nb, deg, a);
+  }
+  return ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), a);
+}
 
 // Item i must be evaluated in the pass of loop L: it is on a cluster
 // boundary, and its own cluster or a neighbouring one changed in this or the
@@ -537,7 +593,7 @@ ACVD_HD int32_t qem_next_open(const SweepView& v, int32_t lab, const QemItem& s,
 // Returns true when at least one target is open (solves and choice follow).
 ACVD_HD bool qem_screen(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained, QemItem* s) {
   if (v.sizes[lab] <= 1) return false;
-  if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
+  if (connexity && ring_check(v, i, lab)) return false;
   if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
   qem_item_terms(v, i, lab, constrained, s);
   for (int32_t k = s->r0; k < s->r1; ++k) {
@@ -602,14 +658,19 @@ ACVD_HD bool qem_choose(const SweepView& v, int32_t lab, const QemItem& s, int c
 ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained,
                                    int level, const Scales& sc, int32_t* target_out) {
   const int32_t r0 = v.row_ptr[i], deg = v.row_ptr[i + 1] - r0;
-  if (v.sizes[lab] <= 1) return false;
-  if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
-  if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
-  QemItem s;
-  qem_item_terms(v, i, lab, constrained, &s);
   int32_t nb[kRingCache];
   ACVD_UNROLL
   for (int p = 0; p < kRingCache; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
+  if (v.sizes[lab] <= 1) return false;
+  if (connexity) {
+    const bool split = (v.ring_adj && deg <= kRingAdjMax)
+                           ? ring_problem_mask(v.ring_adj[i], nb, deg, lab)
+                           : ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab);
+    if (split) return false;
+  }
+  if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
+  QemItem s;
+  qem_item_terms(v, i, lab, constrained, &s);
   uint32_t open = 0;
   ACVD_UNROLL
   for (int p = 0; p < kRingCache; ++p) {
@@ -625,7 +686,10 @@ ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, i
   while (open) {
     const int p = __builtin_ctz(open);
     open &= open - 1;
-    const int32_t b = v.labels[v.col[r0 + p]];
+    int32_t b = kNull;  // nb[p], selected in registers
+    ACVD_UNROLL
+    for (int q = 0; q < kRingCache; ++q)
+      if (q == p) b = nb[q];
     const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
     const double Tb = constrained ? v.T[b] : 0.0;
     const double Tb_new = constrained ? qem_join_T(v, i, b, sc, level) : 0.0;
@@ -682,7 +746,7 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
         [&](int32_t, int32_t b) { return constrained ? qem_join_T(v, i, b, sc, level) : 0.0; }, target_out);
   }
   if (v.sizes[lab] <= 1) return false;
-  if (connexity && ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab)) return false;
+  if (connexity && ring_check(v, i, lab)) return false;
   const ClusterGeom ga = v.geom[lab];
   if (!can_leave(pw, ga)) return false;
   const float ta = leave_term(px, py, pz, pw, ga);

@@ -100,7 +100,7 @@ struct acvd_ctx {
   DBuf<float4> d_xw;
   DBuf<float> d_q;
   DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
-  DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio;
+  DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio, d_ring_adj;
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T, d_evec;
   DBuf<int32_t> d_cand_tgt, d_cand_lab, d_cand_since, d_cl_round;
@@ -150,6 +150,7 @@ struct acvd_ctx {
     a.clus_mesh = meshes.size() > 1 ? d_clus_mesh.p : nullptr;
     a.row_ptr = d_row_ptr.p;
     a.col = d_col.p;
+    a.ring_adj = d_ring_adj.p;
     a.xw = d_xw.p;
     a.q = d_q.p;
     a.labels = d_labels.p;
@@ -397,6 +398,8 @@ int build(acvd_ctx* c) {
     c->E2 = e2;
     HIP_TRY(c->d_col.alloc(e2));
     HIP_TRY(launch_csr_compact(s, N, tptr.p, tcol.p, c->d_row_ptr.p, c->d_col.p));
+    HIP_TRY(c->d_ring_adj.alloc(N));
+    HIP_TRY(launch_ring_adj(s, N, c->d_row_ptr.p, c->d_col.p, c->d_ring_adj.p));
     // ---- items
     HIP_TRY(c->d_xw.alloc(N));
     HIP_TRY(c->d_q.alloc(10 * N));  // vertex quadrics of every mesh

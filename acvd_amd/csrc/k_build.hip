@@ -235,6 +235,14 @@ __global__ void k_clamp(SweepArgs a, const double* __restrict__ raw_w, const dou
   xw[v].w = float(w);
 }
 
+// Ring adjacency of every item (bits of neighbour pairs sharing an edge).
+__global__ void k_ring_adj(int64_t n, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                           unsigned long long* ring_adj) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ring_adj[i] = ring_adjacency(row_ptr, col, int32_t(i));
+}
+
 }  // namespace
 
 hipError_t launch_csr_count(hipStream_t s, int64_t nf, const int32_t* tri, int32_t* deg2, int32_t* vfdeg) {
@@ -255,6 +263,13 @@ hipError_t launch_csr_sort_unique(hipStream_t s, int64_t n, const int32_t* tptr,
                                   const int32_t* vfptr, int32_t* vfcol) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_csr_sort_unique, dim3(grid_for(n)), dim3(kBlock), 0, s, n, tptr, tcol, deg, vfptr, vfcol);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_adj(hipStream_t s, int64_t n, const int32_t* row_ptr, const int32_t* col,
+                           unsigned long long* ring_adj) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ring_adj, dim3(grid_for(n)), dim3(kBlock), 0, s, n, row_ptr, col, ring_adj);
   return hipGetLastError();
 }
 

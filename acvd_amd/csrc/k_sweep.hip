@@ -146,6 +146,7 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   v.sizes = a.sizes;
   v.stamp = a.stamp;
   v.cand_pass = a.cand_pass;
+  v.ring_adj = a.ring_adj;
   return v;
 }
 
@@ -381,20 +382,24 @@ __device__ inline int64_t seg_pos(const SegIndex* si, int32_t cap, int32_t p) {
 // walk.
 template <class F>
 __device__ inline void walk_candidates(const SweepArgs& a, SegIndex* si, F&& f) {
-  if (a.ctrl->seg_walk) {
-    const int32_t n = a.seg_cand[blockIdx.x];
-    const int64_t seg = int64_t(blockIdx.x) * a.cand_cap;
-    for (int32_t base = 0; base < n; base += kBlock) {
-      const int32_t k = base + int32_t(threadIdx.x);
-      f(k < n, seg + k);
-    }
+  const bool by_seg = a.ctrl->seg_walk != 0;  // block-uniform
+  int32_t n, first, stride;
+  if (by_seg) {
+    n = a.seg_cand[blockIdx.x];
+    first = 0;
+    stride = kBlock;
   } else {
     load_seg_index(a.seg_cand, a.n_seg, si);
-    const int32_t n = si->start[kMaxSeg];
-    for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-      const int32_t k = base + int32_t(threadIdx.x);
-      f(k < n, k < n ? seg_pos(si, a.cand_cap, k) : int64_t(0));
-    }
+    n = si->start[kMaxSeg];
+    first = blockIdx.x * kBlock;
+    stride = gridDim.x * kBlock;
+  }
+  const int64_t seg = int64_t(blockIdx.x) * a.cand_cap;
+  for (int32_t base = first; base < n; base += stride) {
+    const int32_t k = base + int32_t(threadIdx.x);
+    int64_t e = 0;
+    if (k < n) e = by_seg ? seg + k : seg_pos(si, a.cand_cap, k);
+    f(k < n, e);
   }
 }
 

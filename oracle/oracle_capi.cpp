@@ -300,6 +300,17 @@ int orc_hdr_ring_problem(const int32_t* rp, const int32_t* col, const int32_t* l
   return acvd::ring_problem(rp, col, labels, item, cluster) ? 1 : 0;
 }
 
+// Ring check through the ring adjacency mask (-1 for rows longer than the
+// mask covers).
+int orc_hdr_ring_problem_mask(const int32_t* rp, const int32_t* col, const int32_t* labels, int32_t item,
+                              int32_t cluster) {
+  const int32_t r0 = rp[item], deg = rp[item + 1] - r0;
+  if (deg > acvd::kRingAdjMax) return -1;
+  int32_t nb[acvd::kRingAdjMax];
+  for (int p = 0; p < deg; ++p) nb[p] = labels[col[r0 + p]];
+  return acvd::ring_problem_mask(acvd::ring_adjacency(rp, col, item), nb, deg, cluster) ? 1 : 0;
+}
+
 uint64_t orc_hdr_accept_key(uint32_t seed, int32_t loop, int round, uint32_t item) {
   return acvd::accept_key(seed, loop, round, item);
 }

@@ -61,6 +61,7 @@ def lib():
         L.orc_ring_problem.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32]
         L.orc_hdr_rep_point_delta.argtypes = [P, P, ctypes.c_int, P, P, P]
         L.orc_hdr_ring_problem.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32]
+        L.orc_hdr_ring_problem_mask.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int32]
         L.orc_hdr_accept_key.restype = ctypes.c_uint64
         L.orc_hdr_accept_key.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int, ctypes.c_uint32]
         L.orc_replay_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -212,6 +213,13 @@ def hdr_ring_problem(row_ptr, col, labels, item, cluster):
     return lib().orc_hdr_ring_problem(_p(np.ascontiguousarray(row_ptr, np.int32)),
                                       _p(np.ascontiguousarray(col, np.int32)),
                                       _p(np.ascontiguousarray(labels, np.int32)), int(item), int(cluster))
+
+
+def hdr_ring_problem_mask(row_ptr, col, labels, item, cluster):
+    """ring check through the ring adjacency mask (-1: row too long for it)"""
+    return lib().orc_hdr_ring_problem_mask(_p(np.ascontiguousarray(row_ptr, np.int32)),
+                                           _p(np.ascontiguousarray(col, np.int32)),
+                                           _p(np.ascontiguousarray(labels, np.int32)), int(item), int(cluster))
 
 
 def hdr_accept_key(seed, loop, item, round=0):

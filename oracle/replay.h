@@ -39,7 +39,7 @@ struct Replay {
   const float* q = nullptr;
 
   std::vector<int32_t> labels, sizes, stamp, cand_pass, pending, cl_round;
-  std::vector<unsigned long long> acc_ws, acc_q;
+  std::vector<unsigned long long> acc_ws, acc_q, ring_adj;
   std::vector<acvd::ClusterGeom> geom;
   std::vector<double> T, evec;
   std::vector<unsigned long long> prio;
@@ -61,6 +61,7 @@ struct Replay {
     v.sizes = sizes.data();
     v.stamp = stamp.data();
     v.cand_pass = cand_pass.data();
+    v.ring_adj = ring_adj.empty() ? nullptr : ring_adj.data();
     return v;
   }
 
@@ -263,6 +264,8 @@ struct Replay {
 
   // init: labels in the reference convention (0..K-1, K = null)
   void run(const int32_t* init) {
+    ring_adj.resize(N);
+    for (int64_t i = 0; i < N; ++i) ring_adj[i] = acvd::ring_adjacency(rp, col, int32_t(i));
     labels.assign(N, acvd::kNull);
     for (int64_t i = 0; i < N; ++i) labels[i] = (init[i] < 0 || init[i] >= K) ? acvd::kNull : init[i];
     sizes.assign(K, 0);

@@ -166,7 +166,7 @@ def test_accept_keys_order_newer_passes_first():
 
 @pytest.mark.parametrize("case", ["torus-graded", "cube", "ico"])
 def test_short_row_qem_path_equals_generic_path(case):
-    """The register-resident QEM rule for rows of at most 12 neighbours makes
+    """The register-resident QEM rule for rows of at most 8 neighbours makes
     the same moves as the generic screen / solve / choice path: full replays
     (constrained phases, ring checks) end with identical labels."""
     if case == "torus-graded":

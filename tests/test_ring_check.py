@@ -1,8 +1,9 @@
 """Ring check (ConnexityConstraintProblemLocal, vtkVerticesProcessing.h:168-237):
-the device header's bitmask version against the oracle's restatement of the
-reference BFS and a direct graph search, on every label pattern of 1-rings of
-valence 3..8 (open fans and closed rings) and of special meshes where ring
-vertices have extra adjacency (tetrahedron, octahedron)."""
+the device header's bitmask version and its ring-adjacency-mask version
+against the oracle's restatement of the reference BFS and a direct graph
+search, on every label pattern of 1-rings of valence 3..8 (open fans and
+closed rings) and of special meshes where ring vertices have extra adjacency
+(tetrahedron, octahedron); rows of valence 9 and 10 take the row walk."""
 import itertools
 
 import numpy as np
@@ -70,9 +71,10 @@ def check_all_patterns(xyz, tri):
             want = direct(adj, labels, item, 0)
             assert m.ring_problem(labels, item, 0) == want
             assert po.hdr_ring_problem(rp, col, labels, item, 0) == want
+            assert po.hdr_ring_problem_mask(rp, col, labels, item, 0) == (want if len(ring) <= 8 else -1)
 
 
-@pytest.mark.parametrize("d", range(3, 9))
+@pytest.mark.parametrize("d", range(3, 11))
 @pytest.mark.parametrize("closed", [True, False])
 def test_fans(d, closed):
     xyz, tri = fan(d, closed)
@@ -98,3 +100,4 @@ def test_random_labelings_on_a_sphere():
         want = direct(adj, labels, item, c)
         assert m.ring_problem(labels, item, c) == want
         assert po.hdr_ring_problem(rp, col, labels, item, c) == want
+        assert po.hdr_ring_problem_mask(rp, col, labels, item, c) == want
