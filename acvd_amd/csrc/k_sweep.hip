@@ -374,6 +374,19 @@ __device__ inline int64_t seg_pos(const SegIndex* si, int32_t cap, int32_t p) {
   return int64_t(lo) * cap + (p - si->start[lo]);
 }
 
+// True (in every thread of the block) when round r applied no move: the
+// later rounds of the pass then apply nothing either and are skipped.
+__device__ inline bool round_was_empty(const SweepArgs& a, int r) {
+  __shared__ int s_empty;
+  if (threadIdx.x < 64) {
+    int m = a.ctrl->mods[r][threadIdx.x];
+    for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off);
+    if (threadIdx.x == 0) s_empty = m == 0;
+  }
+  __syncthreads();
+  return s_empty != 0;
+}
+
 // Walks the candidate list, calling f(valid, entry) for one position per
 // thread per step (every thread of the block takes part in every step).  A
 // long list (last pass's proposals above kSegWalkMax) is walked evenly by all
@@ -527,6 +540,7 @@ template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ KeyAgg agg;
   __shared__ SegIndex si;
+  if (round_was_empty(a, round - 1)) return;
   keyagg_clear(&agg);
   __syncthreads();
   const SweepView v = make_view(a);
@@ -604,6 +618,10 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ int32_t s_dirty, s_moved;
   __shared__ SegIndex si;
   const int64_t dseg = int64_t(blockIdx.x) * a.dirty_cap, mseg = int64_t(blockIdx.x) * a.cand_cap;
+  if (round > 0 && round_was_empty(a, round - 1)) {
+    if (threadIdx.x == 0) a.seg_dirty[blockIdx.x] = 0;
+    return;
+  }
   if (threadIdx.x == 0) {
     s_dirty = 0;
     s_moved = round == 0 ? 0 : a.seg_moved[blockIdx.x];

@@ -350,12 +350,14 @@ uint32_t orc_mt19937_10000(void) {
 // Replay C.  params: K, metric, level, seed, unconstrained, max_loops,
 // max_convergences.  scales3 = {sw, ss, sq}.  init/out in the reference
 // convention (K = null).
-int orc_replay_run_path(int generic_qem, int64_t n, const int32_t* rp, const int32_t* col, const float* xw,
+// path: bit 0 generic QEM rule, bit 1 every round (no end after an empty round)
+int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* col, const float* xw,
                         const float* q, const double* scales3, int K, int metric, int level, uint32_t seed,
                         int unconstrained, int max_loops, int max_conv, const int32_t* init, int32_t* out,
                         int64_t* stats6) {
   Replay r;
-  r.generic_qem = generic_qem != 0;
+  r.generic_qem = (path & 1) != 0;
+  r.all_rounds = (path & 2) != 0;
   r.N = n;
   r.K = K;
   r.metric = metric;

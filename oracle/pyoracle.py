@@ -227,7 +227,7 @@ def hdr_accept_key(seed, loop, item, round=0):
 
 
 def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, unconstrained=0, max_loops=5000000,
-               max_conv=1000000000, generic_qem=False):
+               max_conv=1000000000, generic_qem=False, all_rounds=False):
     n = len(row_ptr) - 1
     out = np.empty(n, np.int32)
     st = np.zeros(6, np.int64)
@@ -237,7 +237,7 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
     qq = None if q is None else np.ascontiguousarray(q, np.float32)
     sc = np.ascontiguousarray(scales, np.float64)
     ini = np.ascontiguousarray(init, np.int32)
-    lib().orc_replay_run_path(int(generic_qem), n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
+    lib().orc_replay_run_path(int(generic_qem) | (int(all_rounds) << 1), n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
                               unconstrained, max_loops, max_conv, _p(ini), _p(out), _p(st))
     keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
     return out, dict(zip(keys, (int(v) for v in st)))

@@ -33,6 +33,7 @@ struct Replay {
   acvd::Scales sc{};
   int max_loops = 5000000, max_conv = 1000000000, unconstrained_init = 0;
   bool generic_qem = false;  // force the generic QEM proposal path (equivalence checks)
+  bool all_rounds = false;   // run every round even after one without moves (equivalence checks)
   const int32_t* rp = nullptr;
   const int32_t* col = nullptr;
   const float* xw = nullptr;
@@ -177,6 +178,7 @@ struct Replay {
     std::vector<int32_t> from(cand.size());
     std::vector<int32_t> since(cand.size(), L * acvd::kMatchRounds);
     for (int round = 0; round < acvd::kMatchRounds; ++round) {
+      const int mods_before = mods;
       if (round > 0) {
         // a live proposal whose own cluster or target moved since its last
         // evaluation is evaluated again against the state left by the last
@@ -209,6 +211,8 @@ struct Replay {
         target[t] = -3;
         mods++;
       }
+      // a round without moves ends the pass (later rounds would apply nothing)
+      if (!all_rounds && mods == mods_before) break;
     }
     st.pass_mods = mods;
     st.pass_props = int32_t(cand.size());
