@@ -7,8 +7,9 @@ discrete mean curvature), 100,000 clusters, QEM metric with unconstrained
 initialisation (ACVDQ).  Synthetic, generated from seed 3.
 
 A step is one full clustering convergence (MinimizeEnergy: all passes, cleans
-and fills until every stopping rule is met) from the same device-seeded
-initial labelling, with the mesh, CSR and items resident in HBM.  `value` is
+and fills until every stopping rule is met) from the same initial labelling
+(the reference's region growing), with the mesh, CSR and items resident in
+HBM.  `value` is
 the median device time of a step in ms (HIP events inside the library);
 `ms_per_step` the wall time per step around the call.
 
