@@ -739,7 +739,12 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
     *target_out = target;
     return target != kNull;
   }
-  if (METRIC == 1) {
+  // QEM with active constraints: the T terms take part.  Without them (the
+  // unconstrained initialisation, vtkUniformClustering.h:683-688) the QEM
+  // move energy is w (tb - ta), the isotropic one, and the isotropic rule
+  // below decides; METRIC 2 is that case alone (QEM meshes swept while none
+  // is constrained), compiled without the FP64 solves.
+  if (METRIC == 1 && constrained) {
     if (kShortRows && r1 - r0 <= kRingCache)
       return qem_propose_short_row(v, i, lab, connexity, constrained, level, sc, target_out);
     // QEM: screen, solves, choice

@@ -120,8 +120,10 @@ struct acvd_ctx {
   std::vector<DevMesh> h_meshes;
   void* cub_tmp = nullptr;
   size_t cub_bytes = 0;
-  hipGraphExec_t chunk_exec = nullptr;
-  hipGraph_t chunk_graph = nullptr;
+  // captured pass chunks: [0] some QEM mesh constrained (QEM kernels), [1] no
+  // QEM mesh constrained (their isotropic-rule kernels)
+  hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};
+  hipGraph_t chunk_graph[2] = {nullptr, nullptr};
   double build_ms = 0.0, converge_ms = 0.0;
   Ctrl last_ctrl{};
   bool timing = false;
@@ -130,8 +132,7 @@ struct acvd_ctx {
   int64_t kernel_launches[4] = {0, 0, 0, 0};
 
   ~acvd_ctx() {
-    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
-    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
+    drop_graphs();
     if (cub_tmp) (void)hipFree(cub_tmp);
     if (h_ctrl) (void)hipHostFree(h_ctrl);
     if (h_state) (void)hipHostFree(h_state);
@@ -189,12 +190,18 @@ struct acvd_ctx {
     return a;
   }
 
+  void drop_graphs() {
+    for (int k = 0; k < 2; ++k) {
+      if (chunk_exec[k]) (void)hipGraphExecDestroy(chunk_exec[k]);
+      if (chunk_graph[k]) (void)hipGraphDestroy(chunk_graph[k]);
+      chunk_exec[k] = nullptr;
+      chunk_graph[k] = nullptr;
+    }
+  }
+
   void invalidate() {
     built = false;
-    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
-    if (chunk_graph) (void)hipGraphDestroy(chunk_graph);
-    chunk_exec = nullptr;
-    chunk_graph = nullptr;
+    drop_graphs();
   }
 };
 
@@ -538,15 +545,15 @@ int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsign
   return ACVD_OK;
 }
 
-int enqueue_chunk(acvd_ctx* c) {
+int enqueue_chunk(acvd_ctx* c, bool qem_unc) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(launch_scan(s, a));
-    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem));
+    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(launch_apply(s, a, 0));
     for (int r = 1; r < kMatchRounds; ++r) {
-      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, r));
+      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
       HIP_TRY(launch_apply(s, a, r));
     }
     HIP_TRY(launch_tile_update(s, a));
@@ -556,7 +563,7 @@ int enqueue_chunk(acvd_ctx* c) {
 }
 
 // Passes launched one by one with events around each kernel.
-int run_chunk_timed(acvd_ctx* c) {
+int run_chunk_timed(acvd_ctx* c, bool qem_unc) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   std::vector<hipEvent_t> ev(5 * kChunk);
@@ -565,11 +572,11 @@ int run_chunk_timed(acvd_ctx* c) {
     HIP_TRY(hipEventRecord(ev[5 * p], s));
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
-    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem));
+    HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
     HIP_TRY(launch_apply(s, a, 0));
     for (int r = 1; r < kMatchRounds; ++r) {
-      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, r));
+      HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
       HIP_TRY(launch_apply(s, a, r));
     }
     HIP_TRY(launch_tile_update(s, a));
@@ -589,24 +596,33 @@ int run_chunk_timed(acvd_ctx* c) {
   return ACVD_OK;
 }
 
-int run_chunk(acvd_ctx* c, bool use_graph) {
+int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc) {
   hipStream_t s = c->stream;
-  if (c->timing) return run_chunk_timed(c);
+  if (c->timing) return run_chunk_timed(c, qem_unc);
   if (use_graph) {
-    if (!c->chunk_exec) {
+    const int k = qem_unc ? 1 : 0;
+    if (!c->chunk_exec[k]) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_chunk(c);
+      int rc = enqueue_chunk(c, qem_unc);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(s, &g);
       if (rc != ACVD_OK) return rc;
       HIP_TRY(ec);
-      c->chunk_graph = g;
-      HIP_TRY(hipGraphInstantiate(&c->chunk_exec, g, nullptr, nullptr, 0));
+      c->chunk_graph[k] = g;
+      HIP_TRY(hipGraphInstantiate(&c->chunk_exec[k], g, nullptr, nullptr, 0));
     }
-    HIP_TRY(hipGraphLaunch(c->chunk_exec, s));
+    HIP_TRY(hipGraphLaunch(c->chunk_exec[k], s));
     return ACVD_OK;
   }
-  return enqueue_chunk(c);
+  return enqueue_chunk(c, qem_unc);
+}
+
+// True when no QEM mesh of the context is constrained (the state flips only
+// at a convergence, after which the host runs the clean and re-reads it).
+bool qem_unconstrained(const acvd_ctx* c) {
+  for (size_t m = 0; m < c->meshes.size(); ++m)
+    if (c->meshes[m].params.metric == ACVD_METRIC_QEM && c->h_state[m].constrained) return false;
+  return true;
 }
 
 // Prints the logged passes in [*printed, ctrl.passes) and advances *printed.
@@ -671,7 +687,7 @@ int minimize_energy(acvd_ctx* c) {
   for (int m = 0; m < M; ++m) loop_output |= c->meshes[m].params.console_output > 1;
   unsigned long long printed = 0;
   while (true) {
-    int rc2 = run_chunk(c, use_graph);
+    int rc2 = run_chunk(c, use_graph, qem_unconstrained(c));
     if (rc2) return rc2;
     HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

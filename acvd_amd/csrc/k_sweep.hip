@@ -387,6 +387,10 @@ __device__ inline bool round_was_empty(const SweepArgs& a, int r) {
   return s_empty != 0;
 }
 
+// Metric of the meshes a kernel instantiation sweeps: 0 isotropic, 1 QEM,
+// 2 QEM while unconstrained (isotropic rule, no FP64 solves compiled in).
+constexpr int mesh_metric(int kernel_metric) { return kernel_metric == 2 ? 1 : kernel_metric; }
+
 // Walks the candidate list, calling f(valid, entry) for one position per
 // thread per step (every thread of the block takes part in every step).  A
 // long list (last pass's proposals above kSegWalkMax) is walked evenly by all
@@ -503,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
       i = work_entry(a, &wi, t);
       const int m = mesh_of_item(a, i);
       const DevMesh& dm = a.meshes[m];
-      if (dm.metric == METRIC) {
+      if (dm.metric == mesh_metric(METRIC)) {
         MeshState* st = &a.state[m];
         L = st->loops;
         lab = a.labels[i];
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
       const int32_t since = a.cand_since[e];
       const int m = mesh_of_item(a, i);
       const DevMesh& dm = a.meshes[m];
-      if (dm.metric == METRIC) {
+      if (dm.metric == mesh_metric(METRIC)) {
         const MeshState& st = a.state[m];
         const bool stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
         const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
@@ -1044,9 +1048,14 @@ hipError_t launch_scan(hipStream_t s, const SweepArgs& a) {
 }
 
 // One evaluate block per candidate segment: block b writes segment b's count.
-hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem) {
+// qem_unconstrained: no QEM mesh is constrained (the QEM meshes take the
+// isotropic rule, METRIC 2).
+hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained) {
   if (any_iso) hipLaunchKernelGGL(k_eval<0>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, 0);
-  if (any_qem) hipLaunchKernelGGL(k_eval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
+  if (any_qem && qem_unconstrained)
+    hipLaunchKernelGGL(k_eval<2>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
+  else if (any_qem)
+    hipLaunchKernelGGL(k_eval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1063,9 +1072,13 @@ hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a) {
   return hipGetLastError();
 }
 
-hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, int round) {
+hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained,
+                         int round) {
   if (any_iso) hipLaunchKernelGGL(k_reeval<0>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-  if (any_qem) hipLaunchKernelGGL(k_reeval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  if (any_qem && qem_unconstrained)
+    hipLaunchKernelGGL(k_reeval<2>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  else if (any_qem)
+    hipLaunchKernelGGL(k_reeval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
   return hipGetLastError();
 }
 
