@@ -159,7 +159,7 @@ hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* me
 hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbox, int32_t* tile_items,
                              int32_t* item_tile, void** tmp, size_t* tmp_bytes);
 hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained);
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round);
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool unconstrained);
 hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a);
 hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained,
                          int round);

@@ -551,10 +551,10 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc) {
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
-    HIP_TRY(launch_apply(s, a, 0));
+    HIP_TRY(launch_apply(s, a, 0, qem_unc));
     for (int r = 1; r < kMatchRounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
-      HIP_TRY(launch_apply(s, a, r));
+      HIP_TRY(launch_apply(s, a, r, qem_unc));
     }
     HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(launch_controller(s, a));
@@ -574,10 +574,10 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc) {
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
-    HIP_TRY(launch_apply(s, a, 0));
+    HIP_TRY(launch_apply(s, a, 0, qem_unc));
     for (int r = 1; r < kMatchRounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
-      HIP_TRY(launch_apply(s, a, r));
+      HIP_TRY(launch_apply(s, a, r, qem_unc));
     }
     HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
@@ -618,7 +618,8 @@ int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc) {
 }
 
 // True when no QEM mesh of the context is constrained (the state flips only
-// at a convergence, after which the host runs the clean and re-reads it).
+// at a convergence, after which the host runs the clean and re-reads it):
+// QEM meshes then take the isotropic rule and every derive is a centroid.
 bool qem_unconstrained(const acvd_ctx* c) {
   for (size_t m = 0; m < c->meshes.size(); ++m)
     if (c->meshes[m].params.metric == ACVD_METRIC_QEM && c->h_state[m].constrained) return false;

@@ -618,6 +618,7 @@ __device__ inline int block_sum(int v) {
 // blocks walk all candidates evenly; the two clusters join the block's dirty
 // segment (their centroid and T are derived by k_derive_dirty) and the item
 // its moved segment (tile sets are extended by k_tile_update after the pass).
+template <bool kDeriveHere>
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ int32_t s_dirty, s_moved;
   __shared__ SegIndex si;
@@ -682,12 +683,18 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
         a.cl_round[b] = rtag;
         ++mods;
         if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
+        if (kDeriveHere) {  // centroid only (no mesh constrained): one division per cluster
+          if (lab != kNull) derive_cluster(a, lab, dm, 0);
+          derive_cluster(a, b, dm, 0);
+        }
       }
     }
-    const int64_t ea = seg_append(dseg, &s_dirty, win && lab != kNull);
-    if (ea >= 0) a.dirty[ea] = lab;
-    const int64_t eb = seg_append(dseg, &s_dirty, win);
-    if (eb >= 0) a.dirty[eb] = b;
+    if (!kDeriveHere) {
+      const int64_t ea = seg_append(dseg, &s_dirty, win && lab != kNull);
+      if (ea >= 0) a.dirty[ea] = lab;
+      const int64_t eb = seg_append(dseg, &s_dirty, win);
+      if (eb >= 0) a.dirty[eb] = b;
+    }
     const int64_t em = seg_append(mseg, &s_moved, win);
     if (em >= 0) a.moved[em] = int32_t(i);
   });
@@ -1061,9 +1068,17 @@ hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any
 
 // One apply block per candidate segment (its dirty and moved segments have
 // the same index), then the derive of the round's dirty clusters.
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round) {
-  hipLaunchKernelGGL(k_apply, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-  hipLaunchKernelGGL(k_derive_dirty, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a);
+// One apply block per candidate segment (its dirty and moved segments have
+// the same index), then the derive of the round's dirty clusters.  When no
+// mesh is constrained (unconstrained = true) a derive is the centroid alone
+// and the movers compute it themselves.
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool unconstrained) {
+  if (unconstrained) {
+    hipLaunchKernelGGL(k_apply<true>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  } else {
+    hipLaunchKernelGGL(k_apply<false>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+    hipLaunchKernelGGL(k_derive_dirty, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a);
+  }
   return hipGetLastError();
 }
 
