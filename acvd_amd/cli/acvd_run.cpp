@@ -1,0 +1,300 @@
+// acvd_run: command-line caller of the library with the argument grammar of
+// the reference examples (DiscreteRemeshing/Examples/ACVD.cxx:60-131, and
+// ACVDQ.cxx:319-338 with -qem 1):
+//
+//   acvd_run file nvertices gradation [options]
+//
+//   -q level      ACVD: eigenvalues of the quadric post-placement (default 1,
+//                 0 = none, ACVD.cxx:66, 217-291); ACVDQ: quadrics
+//                 optimisation level of the metric (default 3)
+//   -s threshold  subsampling threshold (default 10); inputs with fewer than
+//                 threshold * nvertices vertices are refused (no subdivision)
+//   -o directory  output directory;  -of file  output file (simplification.ply)
+//   -cd file      per-vertex density indicator, one value per line (the
+//                 reference reads a VTK image instead); without it and with
+//                 gradation > 0 the indicator is |H|, the cotangent-formula mean
+//                 curvature (the reference estimates curvature with VTK)
+//   -qem 0/1      QEM metric with unconstrained initialisation (ACVDQ)
+//   -m -d -b -l -w -cmin -cmax -cf -np -p   accepted; not supported (warned)
+//
+// Meshes are read from PLY (ascii or binary little-endian) or OBJ and written
+// as ASCII PLY.  Polygons are fanned into triangles, as vtkSurface::SetInput
+// triangulates its input.  Exit status 0 on success, 1 on error.
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "acvd/remeshing.hpp"
+
+extern "C" int acvd_mean_curvature(int64_t V, const float* xyz32, const double* xyz64, int64_t F,
+                                   const int32_t* tri, double* absH);
+
+namespace {
+
+bool ends_with(const std::string& s, const char* suf) {
+  const size_t n = std::strlen(suf);
+  return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
+}
+
+void add_polygon(acvd::Mesh& m, const std::vector<int64_t>& idx) {
+  for (size_t k = 2; k < idx.size(); ++k) {
+    m.triangles.push_back(int32_t(idx[0]));
+    m.triangles.push_back(int32_t(idx[k - 1]));
+    m.triangles.push_back(int32_t(idx[k]));
+  }
+}
+
+bool read_obj(const std::string& path, acvd::Mesh& m, std::string& err) {
+  std::ifstream in(path);
+  if (!in) return err = "cannot open " + path, false;
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string tag;
+    ls >> tag;
+    if (tag == "v") {
+      double x, y, z;
+      if (!(ls >> x >> y >> z)) return err = "bad vertex line: " + line, false;
+      m.points.insert(m.points.end(), {x, y, z});
+    } else if (tag == "f") {
+      std::vector<int64_t> idx;
+      std::string tok;
+      const int64_t nv = m.GetNumberOfPoints();
+      while (ls >> tok) {
+        int64_t v = std::strtoll(tok.c_str(), nullptr, 10);  // "a", "a/b", "a//c", "a/b/c"
+        v = v < 0 ? nv + v : v - 1;
+        if (v < 0 || v >= nv) return err = "face index out of range: " + line, false;
+        idx.push_back(v);
+      }
+      add_polygon(m, idx);
+    }
+  }
+  return true;
+}
+
+// Reads one scalar of a PLY type from ascii or little-endian binary input.
+bool ply_value(std::istream& in, bool ascii, const std::string& type, double* out) {
+  if (ascii) return bool(in >> *out);
+  auto rd = [&](auto v) {
+    in.read(reinterpret_cast<char*>(&v), sizeof(v));
+    *out = double(v);
+    return bool(in);
+  };
+  if (type == "char" || type == "int8") return rd(int8_t());
+  if (type == "uchar" || type == "uint8") return rd(uint8_t());
+  if (type == "short" || type == "int16") return rd(int16_t());
+  if (type == "ushort" || type == "uint16") return rd(uint16_t());
+  if (type == "int" || type == "int32") return rd(int32_t());
+  if (type == "uint" || type == "uint32") return rd(uint32_t());
+  if (type == "float" || type == "float32") return rd(float());
+  if (type == "double" || type == "float64") return rd(double());
+  return false;
+}
+
+bool read_ply(const std::string& path, acvd::Mesh& m, std::string& err) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return err = "cannot open " + path, false;
+  struct Prop {
+    std::string name, type, count_type;
+    bool list = false;
+  };
+  struct Elem {
+    std::string name;
+    int64_t count = 0;
+    std::vector<Prop> props;
+  };
+  std::vector<Elem> elems;
+  std::string line, format;
+  std::getline(in, line);
+  if (line.rfind("ply", 0) != 0) return err = path + " is not a PLY file", false;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    std::istringstream ls(line);
+    std::string tag;
+    ls >> tag;
+    if (tag == "format") ls >> format;
+    else if (tag == "element") {
+      Elem e;
+      ls >> e.name >> e.count;
+      elems.push_back(e);
+    } else if (tag == "property" && !elems.empty()) {
+      Prop p;
+      std::string t;
+      ls >> t;
+      if (t == "list") {
+        p.list = true;
+        ls >> p.count_type >> p.type >> p.name;
+      } else {
+        p.type = t;
+        ls >> p.name;
+      }
+      elems.back().props.push_back(p);
+    } else if (tag == "end_header") break;
+  }
+  const bool ascii = format == "ascii";
+  if (!ascii && format != "binary_little_endian") return err = "unsupported PLY format " + format, false;
+  for (const Elem& e : elems) {
+    for (int64_t r = 0; r < e.count; ++r) {
+      double xyz[3] = {0, 0, 0};
+      std::vector<int64_t> idx;
+      for (const Prop& p : e.props) {
+        if (p.list) {
+          double n = 0;
+          if (!ply_value(in, ascii, p.count_type, &n)) return err = "truncated PLY list", false;
+          idx.clear();
+          for (int k = 0; k < int(n); ++k) {
+            double v = 0;
+            if (!ply_value(in, ascii, p.type, &v)) return err = "truncated PLY list", false;
+            idx.push_back(int64_t(v));
+          }
+        } else {
+          double v = 0;
+          if (!ply_value(in, ascii, p.type, &v)) return err = "truncated PLY element " + e.name, false;
+          if (p.name == "x") xyz[0] = v;
+          if (p.name == "y") xyz[1] = v;
+          if (p.name == "z") xyz[2] = v;
+        }
+      }
+      if (e.name == "vertex") m.points.insert(m.points.end(), {xyz[0], xyz[1], xyz[2]});
+      if (e.name == "face") {
+        for (int64_t v : idx)
+          if (v < 0 || v >= m.GetNumberOfPoints()) return err = "face index out of range", false;
+        add_polygon(m, idx);
+      }
+    }
+  }
+  return true;
+}
+
+bool write_ply(const std::string& path, const acvd::Mesh& m) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return false;
+  std::fprintf(f, "ply\nformat ascii 1.0\nelement vertex %lld\nproperty double x\nproperty double y\n"
+                  "property double z\nelement face %lld\nproperty list uchar int vertex_indices\nend_header\n",
+               (long long)m.GetNumberOfPoints(), (long long)m.GetNumberOfCells());
+  for (int64_t i = 0; i < m.GetNumberOfPoints(); ++i)
+    std::fprintf(f, "%.17g %.17g %.17g\n", m.points[3 * i], m.points[3 * i + 1], m.points[3 * i + 2]);
+  for (int64_t t = 0; t < m.GetNumberOfCells(); ++t)
+    std::fprintf(f, "3 %d %d %d\n", m.triangles[3 * t], m.triangles[3 * t + 1], m.triangles[3 * t + 2]);
+  return std::fclose(f) == 0;
+}
+
+bool read_values(const std::string& path, int64_t n, std::vector<double>& out, std::string& err) {
+  std::ifstream in(path);
+  if (!in) return err = "cannot open " + path, false;
+  out.clear();
+  double v;
+  while (in >> v) out.push_back(v);
+  if (int64_t(out.size()) != n) return err = path + ": expected one value per vertex", false;
+  return true;
+}
+
+template <class Remesher>
+int run(Remesher* remesh, const acvd::Mesh& mesh, int K, double gradation, int subsampling, int q_level,
+        bool qem, const std::vector<double>* indicator, const std::string& out_path,
+        const std::string& smooth_path) {
+  remesh->SetInput(&mesh);
+  remesh->SetNumberOfClusters(K);
+  remesh->SetConsoleOutput(2);
+  remesh->SetSubsamplingThreshold(subsampling);
+  remesh->GetMetric()->SetGradation(gradation);
+  if (indicator) remesh->GetMetric()->SetCurvatureInfo(indicator);
+  if (qem) {
+    remesh->GetMetric()->SetQuadricsOptimizationLevel(q_level);
+    remesh->SetUnconstrainedInitialization(1);
+  }
+  if (!remesh->Remesh()) {
+    std::fprintf(stderr, "acvd_run: %s\n", remesh->GetLastError().c_str());
+    return 1;
+  }
+  if (!qem && q_level != 0) {
+    // ACVD.cxx:217-291: the clustering's mesh is written as smooth_<file>,
+    // then the vertices take the level-q quadric placement
+    if (!write_ply(smooth_path, *remesh->GetOutput())) {
+      std::fprintf(stderr, "acvd_run: cannot write %s\n", smooth_path.c_str());
+      return 1;
+    }
+    if (!remesh->BuildOutput(q_level)) {
+      std::fprintf(stderr, "acvd_run: %s\n", remesh->GetLastError().c_str());
+      return 1;
+    }
+  }
+  const acvd::Mesh* out = remesh->GetOutput();
+  if (!write_ply(out_path, *out)) {
+    std::fprintf(stderr, "acvd_run: cannot write %s\n", out_path.c_str());
+    return 1;
+  }
+  std::printf("output: %lld vertices, %lld triangles -> %s\n", (long long)out->GetNumberOfPoints(),
+              (long long)out->GetNumberOfCells(), out_path.c_str());
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::printf("Usage : acvd_run file nvertices gradation [options]\n"
+                "nvertices is the desired number of vertices\n"
+                "gradation defines the influence of local curvature (0=uniform meshing)\n"
+                "options: -q level, -s threshold, -o directory, -of file, -cd indicator file, -qem 0/1\n");
+    return 0;
+  }
+  const std::string input = argv[1];
+  const int K = std::atoi(argv[2]);
+  const double gradation = std::atof(argv[3]);
+  int subsampling = 10, q_level = -1, qem = 0;
+  std::string out_dir, out_file = "simplification.ply", density;
+  for (int a = 4; a + 1 < argc; a += 2) {
+    const std::string key = argv[a], value = argv[a + 1];
+    if (key == "-s") subsampling = std::atoi(value.c_str());
+    else if (key == "-q") q_level = std::atoi(value.c_str());
+    else if (key == "-o") out_dir = value;
+    else if (key == "-of") out_file = value;
+    else if (key == "-cd") density = value;
+    else if (key == "-qem") qem = std::atoi(value.c_str());
+    else std::printf("option %s %s is not supported here; ignored\n", key.c_str(), value.c_str());
+  }
+  if (q_level < 0) q_level = qem ? 3 : 1;  // ACVDQ / ACVD defaults
+  acvd::Mesh mesh;
+  std::string err;
+  const bool read_ok = ends_with(input, ".obj") ? read_obj(input, mesh, err) : read_ply(input, mesh, err);
+  if (!read_ok) {
+    std::fprintf(stderr, "acvd_run: %s\n", err.c_str());
+    return 1;
+  }
+  std::printf("load : %s (%lld vertices, %lld triangles)\n", input.c_str(), (long long)mesh.GetNumberOfPoints(),
+              (long long)mesh.GetNumberOfCells());
+  std::vector<double> indicator;
+  const std::vector<double>* ind = nullptr;
+  if (!density.empty()) {
+    if (!read_values(density, mesh.GetNumberOfPoints(), indicator, err)) {
+      std::fprintf(stderr, "acvd_run: %s\n", err.c_str());
+      return 1;
+    }
+    ind = &indicator;
+  } else if (gradation > 0) {
+    indicator.resize(size_t(mesh.GetNumberOfPoints()));
+    acvd_mean_curvature(mesh.GetNumberOfPoints(), nullptr, mesh.points.data(), mesh.GetNumberOfCells(),
+                        mesh.triangles.data(), indicator.data());
+    ind = &indicator;
+  }
+  const std::string prefix = out_dir.empty() ? "" : out_dir + "/";
+  const std::string out_path = prefix + out_file, smooth_path = prefix + "smooth_" + out_file;
+  int rc;
+  if (qem) {
+    auto* r = acvd::QIsotropicDiscreteRemeshing::New();
+    rc = run(r, mesh, K, gradation, subsampling, q_level, true, ind, out_path, smooth_path);
+    r->Delete();
+  } else {
+    auto* r = acvd::IsotropicDiscreteRemeshing::New();
+    rc = run(r, mesh, K, gradation, subsampling, q_level, false, ind, out_path, smooth_path);
+    r->Delete();
+  }
+  return rc;
+}
