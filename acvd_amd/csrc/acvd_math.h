@@ -656,16 +656,26 @@ ACVD_HD bool qem_choose(const SweepView& v, int32_t lab, const QemItem& s, int c
   return target != kNull && qem_accept(best_d, best_m);
 }
 
-// The QEM rule for rows of at most kRingCache neighbours with the neighbour
-// labels, join terms and cluster T read once into registers (independent
-// loads, no chains through the row): the same tests in the same CSR order as
-// qem_screen / qem_choose, so it returns the same move.
-ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained,
-                                   int level, const Scales& sc, int32_t* target_out) {
+// nb[p] for a run-time p, selected in registers (nb stays a register array).
+ACVD_HD int32_t ring_label(const int32_t* nb, int p) {
+  int32_t b = kNull;
+  ACVD_UNROLL
+  for (int q = 0; q < kRingCache; ++q)
+    if (q == p) b = nb[q];
+  return b;
+}
+
+// Screen of the QEM rule for rows of at most kRingCache neighbours: the
+// neighbour labels are read once into nb (independent loads, no chains
+// through the row), then the guards of qem_screen (size, ring check, leave
+// condition) and the exact prune of each neighbouring cluster at its first
+// CSR position.  Bit p of *open marks an open target nb[p]; returns *open != 0.
+ACVD_HD bool qem_screen_short_row(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained,
+                                  int32_t* nb, QemItem* s, uint32_t* open_out) {
   const int32_t r0 = v.row_ptr[i], deg = v.row_ptr[i + 1] - r0;
-  int32_t nb[kRingCache];
   ACVD_UNROLL
   for (int p = 0; p < kRingCache; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
+  *open_out = 0;
   if (v.sizes[lab] <= 1) return false;
   if (connexity) {
     const bool split = (v.ring_adj && deg <= kRingAdjMax)
@@ -674,27 +684,34 @@ ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, i
     if (split) return false;
   }
   if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
-  QemItem s;
-  qem_item_terms(v, i, lab, constrained, &s);
+  qem_item_terms(v, i, lab, constrained, s);
   uint32_t open = 0;
   ACVD_UNROLL
   for (int p = 0; p < kRingCache; ++p) {
     bool first = nb[p] != kNull && nb[p] != lab;
     ACVD_UNROLL
     for (int t = 0; t < p; ++t) first = first && nb[t] != nb[p];
-    if (first && qem_open(v, s, nb[p], constrained)) open |= 1u << p;
+    if (first && qem_open(v, *s, nb[p], constrained)) open |= 1u << p;
   }
-  if (!open) return false;
+  *open_out = open;
+  return open != 0;
+}
+
+// The QEM rule for rows of at most kRingCache neighbours: the same tests in
+// the same CSR order as qem_screen / qem_choose, so it returns the same move.
+ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained,
+                                   int level, const Scales& sc, int32_t* target_out) {
+  int32_t nb[kRingCache];
+  QemItem s;
+  uint32_t open;
+  if (!qem_screen_short_row(v, i, lab, connexity, constrained, nb, &s, &open)) return false;
   const double Ta_new = constrained ? qem_leave_T(v, i, lab, sc, level) : 0.0;
   int32_t target = kNull;
   double best_d = 0.0, best_m = 0.0;
   while (open) {
     const int p = __builtin_ctz(open);
     open &= open - 1;
-    int32_t b = kNull;  // nb[p], selected in registers
-    ACVD_UNROLL
-    for (int q = 0; q < kRingCache; ++q)
-      if (q == p) b = nb[q];
+    const int32_t b = ring_label(nb, p);
     const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
     const double Tb = constrained ? v.T[b] : 0.0;
     const double Tb_new = constrained ? qem_join_T(v, i, b, sc, level) : 0.0;

@@ -482,6 +482,179 @@ __device__ inline int64_t seg_append(int64_t seg, int32_t* s_n, bool want) {
   return want ? seg + base + __popcll(bal & ((1ull << lane) - 1ull)) : -1;
 }
 
+// ------------------------------------------------------ QEM solve pool (wave)
+// The constrained QEM rule needs one FP64 solve for the item leaving its
+// cluster and one per open target (qem_propose_short_row runs them one after
+// the other in the item's lane).  Here the lanes of a wave put these solves
+// into a shared pool and take them one per lane: a wave whose items need S
+// solves in all runs ceil(S / 64) solve steps instead of as many as its
+// busiest lane, and an item's solves run side by side.  The choice among the
+// targets is the lexicographic minimum of (dE, cluster id), which does not
+// depend on the order the solves finish in, so the move is the one of
+// qem_propose_short_row.
+constexpr int kPoolTasks = 64 * (1 + kRingCache);  // solves of one wave at most
+struct QemPool {
+  int32_t cl[kPoolTasks];     // target cluster, or ~a for the leave solve of cluster a
+  uint8_t owner[kPoolTasks];  // lane of the item
+  // results of one solve step (one per lane)
+  float tb[64];   // join term of the target
+  double Tn[64];  // T of the trial cluster
+  double Tb[64];  // T of the target
+};
+
+// Orders the LDS accesses of the lanes of one wave (their DS operations
+// execute in issue order; this keeps the compiler from moving them).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave-inclusive prefix sum of v.
+__device__ inline int wave_prefix(int v) {
+  const int lane = __lane_id();
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// Owner kinds of the pool: a labelled item of a constrained QEM mesh whose
+// screen passed, with a row of at most kRingCache neighbours (open targets in
+// nb / open) or a longer one (open targets found again by walking the row).
+enum PoolKind : int { kPoolNone = 0, kPoolShort = 1, kPoolLong = 2 };
+
+// Screen of a labelled item of a constrained QEM mesh for the pool: its kind
+// (kPoolNone when no target is open) and its number of solves.
+__device__ inline int pool_screen(const SweepView& v, int64_t i, int32_t lab, int connexity, int32_t* nb,
+                                  QemItem* s, uint32_t* open, int* ntask) {
+  *ntask = 0;
+  if (v.row_ptr[i + 1] - v.row_ptr[i] <= kRingCache) {
+    if (!qem_screen_short_row(v, i, lab, connexity, 1, nb, s, open)) return kPoolNone;
+    *ntask = 1 + __popc(*open);
+    return kPoolShort;
+  }
+  if (!qem_screen(v, i, lab, connexity, 1, s)) return kPoolNone;
+  int n = 1;
+  for (int32_t k = qem_next_open(v, lab, *s, 1, s->r0 - 1); k < s->r1; k = qem_next_open(v, lab, *s, 1, k)) ++n;
+  *ntask = n;
+  return kPoolLong;
+}
+
+// Posts solves [j0, j1) of a long-row owner at pool index first (solve 0 is
+// the leave, solve j > 0 the j-th open target in CSR order).
+__device__ inline void pool_post_long(const SweepView& v, QemPool* pool, int64_t i, int32_t lab, int j0, int j1,
+                                      int first) {
+  QemItem s;
+  qem_item_terms(v, i, lab, 1, &s);
+  int32_t k = s.r0 - 1;
+  for (int j = 1; j < j0; ++j) k = qem_next_open(v, lab, s, 1, k);
+  for (int j = j0; j < j1; ++j) {
+    int32_t c = ~lab;
+    if (j > 0) {
+      k = qem_next_open(v, lab, s, 1, k);
+      c = k < s.r1 ? v.labels[v.col[k]] : ~lab;  // the state is frozen: the count of pool_screen holds
+    }
+    pool->cl[first + j - j0] = c;
+    pool->owner[first + j - j0] = uint8_t(__lane_id());
+  }
+}
+
+// Every lane of the wave calls it.  An owner (kind != kPoolNone, from
+// pool_screen, mesh m) receives its item's move as qem_propose_short_row /
+// the generic QEM rule would return it.  Short-row owners post all their
+// solves first (a wave posts at most kPoolTasks of them), so that the lanes
+// keep only the choice state across the solve steps; long-row owners post
+// after them what fits and the rest in later fills.
+__device__ bool qem_pool_choose(const SweepArgs& a, const SweepView& v, QemPool* pool, int kind, int ntask,
+                                int64_t i, int32_t lab, int32_t m, const int32_t* nb, const QemItem& s,
+                                uint32_t open, int32_t* target_out) {
+  const int lane = __lane_id();
+  int first, total, post;
+  {
+    const int cs = kind == kPoolShort ? ntask : 0, cl = kind == kPoolLong ? ntask : 0;
+    const int is = wave_prefix(cs), il = wave_prefix(cl);
+    const int ts = __shfl(is, 63), tl = __shfl(il, 63);
+    first = kind == kPoolShort ? is - cs : ts + il - cl;
+    total = ts + tl < kPoolTasks ? ts + tl : kPoolTasks;
+    post = total - first < ntask ? total - first : ntask;
+    if (post < 0) post = 0;
+  }
+  if (kind == kPoolShort) {
+    pool->cl[first] = ~lab;
+    pool->owner[first] = uint8_t(lane);
+    for (int j = 1; j < ntask; ++j) {
+      pool->cl[first + j] = ring_label(nb, __builtin_ctz(open));
+      pool->owner[first + j] = uint8_t(lane);
+      open &= open - 1;
+    }
+  } else if (kind == kPoolLong && post > 0) {
+    pool_post_long(v, pool, i, lab, 0, post, first);
+  }
+  const float pw = s.pw, ta = s.ta;
+  const double Ta = s.Ta;
+  double Ta_new = 0.0, best_d = 0.0, best_m = 0.0;
+  int32_t target = kNull;
+  int done = 0;
+  while (total > 0) {
+    wave_lds_sync();
+    for (int base = 0; base < total; base += 64) {
+      const bool solve = lane < total - base;
+      const int ow = solve ? pool->owner[base + lane] : lane;
+      const int64_t ti = __shfl(int32_t(i), ow);
+      const int32_t tm = __shfl(m, ow);
+      if (solve) {
+        const int32_t tc = pool->cl[base + lane];
+        const int32_t c = tc < 0 ? ~tc : tc;
+        const DevMesh& dm = a.meshes[tm];
+        const Scales sc = dm.scales;
+        int64_t iw, is[3], iq[10], ws[4], q10[10];
+        item_fixed(v, ti, sc, &iw, is, iq);
+        load_acc(v, c, ws, q10);
+        pool->Tn[lane] = qem_T_trial(ws, q10, iw, is, iq, tc < 0 ? -1 : 1, sc, dm.level, v.evec + 9 * int64_t(c));
+        if (tc >= 0) {
+          pool->tb[lane] =
+              join_term(v.xw[4 * ti], v.xw[4 * ti + 1], v.xw[4 * ti + 2], v.xw[4 * ti + 3], v.geom[tc]);
+          pool->Tb[lane] = v.T[tc];
+        }
+      }
+      wave_lds_sync();
+      // collect this lane's solves among pool entries [base, base + 64)
+      const int j0 = first > base ? first : base;
+      const int j1 = first + post < base + 64 ? first + post : base + 64;
+      for (int j = j0; j < j1; ++j) {
+        if (j - first + done == 0) {
+          Ta_new = pool->Tn[j - base];
+          continue;
+        }
+        const int32_t b = pool->cl[j];
+        const float tb = pool->tb[j - base];
+        const double Tb_new = pool->Tn[j - base], Tb = pool->Tb[j - base];
+        const double d = qem_delta(pw, ta, tb, Ta_new, Ta, Tb_new, Tb);
+        if (target == kNull || d < best_d || (d == best_d && b < target)) {
+          best_d = d;
+          best_m = qem_mag(pw, ta, tb, Ta_new, Ta, Tb_new, Tb);
+          target = b;
+        }
+      }
+      wave_lds_sync();
+    }
+    // next fill: long-row solves that did not fit
+    done += post;
+    const int rem = kind == kPoolLong ? ntask - done : 0;
+    const int inc = wave_prefix(rem);
+    const int tot = __shfl(inc, 63);
+    first = inc - rem;
+    total = tot < kPoolTasks ? tot : kPoolTasks;
+    post = total - first < rem ? total - first : rem;
+    if (post < 0) post = 0;
+    if (post > 0) pool_post_long(v, pool, i, lab, done, done + post, first);
+  }
+  *target_out = target;
+  return target != kNull && qem_accept(best_d, best_m);
+}
+
 // Evaluate (round 0): best move of each work item (shared rule of
 // acvd_math.h) against the state at the pass start; a proposal registers its
 // round-0 accept key at both clusters and joins the block's candidate
@@ -492,6 +665,7 @@ template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
   __shared__ WorkIndex wi;
   __shared__ KeyAgg agg;
+  __shared__ QemPool pools[METRIC == 1 ? kBlock / 64 : 1];
   __shared__ int32_t s_n;
   if (threadIdx.x == 0) s_n = append ? a.seg_cand[blockIdx.x] : 0;
   keyagg_clear(&agg);
@@ -502,25 +676,42 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
   for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
     const int32_t t = base + int32_t(threadIdx.x);
     bool want = false;
-    int32_t i = 0, lab = kNull, target = kNull, L = 0;
+    int32_t i = 0, lab = kNull, target = kNull, L = 0, m = 0;
+    int32_t nb[kRingCache];
+    QemItem s;
+    uint32_t open = 0;
+    int kind = kPoolNone, ntask = 0;
     if (t < n) {
       i = work_entry(a, &wi, t);
-      const int m = mesh_of_item(a, i);
+      m = mesh_of_item(a, i);
       const DevMesh& dm = a.meshes[m];
       if (dm.metric == mesh_metric(METRIC)) {
-        MeshState* st = &a.state[m];
+        const MeshState* st = &a.state[m];
         L = st->loops;
         lab = a.labels[i];
-        if (propose_item<METRIC>(v, i, lab, st->connexity, st->constrained, dm.level, dm.scales, &target)) {
-          const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
-          if (lab != kNull) keyagg_min(&agg, lab, key);
-          keyagg_min(&agg, target, key);
-          a.cand_pass[i] = L;
-          a.tile_prop[a.item_tile[i]] = L;
-          if (a.item_mesh) atomicAdd(&st->pass_props, 1);
-          want = true;
-        }
+        if (METRIC == 1 && lab != kNull && st->constrained)
+          kind = pool_screen(v, i, lab, st->connexity, nb, &s, &open, &ntask);
+        else  // no FP64 solve: the null-item rule or the isotropic one
+          want = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st->connexity, st->constrained, dm.level,
+                                                        dm.scales, &target);
       }
+    }
+    if constexpr (METRIC == 1) {
+      int32_t pt = kNull;
+      const bool r = qem_pool_choose(a, v, &pools[threadIdx.x >> 6], kind, ntask, i, lab, m, nb, s, open, &pt);
+      if (kind != kPoolNone) {
+        want = r;
+        target = pt;
+      }
+    }
+    if (want) {
+      const DevMesh& dm = a.meshes[m];
+      const unsigned long long key = accept_key(dm.seed, L, 0, uint32_t(i - dm.item_off));
+      if (lab != kNull) keyagg_min(&agg, lab, key);
+      keyagg_min(&agg, target, key);
+      a.cand_pass[i] = L;
+      a.tile_prop[a.item_tile[i]] = L;
+      if (a.item_mesh) atomicAdd(&a.state[m].pass_props, 1);
     }
     const int64_t e = seg_append(seg, &s_n, want);
     if (want) {
@@ -544,35 +735,55 @@ template <int METRIC>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ KeyAgg agg;
   __shared__ SegIndex si;
+  __shared__ QemPool pools[METRIC == 1 ? kBlock / 64 : 1];
   if (round_was_empty(a, round - 1)) return;
   keyagg_clear(&agg);
   __syncthreads();
   const SweepView v = make_view(a);
   walk_candidates(a, &si, [&](bool valid, int64_t e) {
     const int32_t b = valid ? a.cand_tgt[e] : -1;
+    bool live = false, stale = false, use_pool = false;
+    int32_t i = 0, lab = kNull, target = b, m = 0;
+    unsigned long long key = 0;
+    int32_t nb[kRingCache];
+    QemItem s;
+    uint32_t open = 0;
+    int kind = kPoolNone, ntask = 0;
     if (b >= 0) {
-      const int64_t i = a.cand[e];
-      const int32_t lab = a.cand_lab[e];  // a live candidate has not moved
+      i = a.cand[e];
+      lab = a.cand_lab[e];  // a live candidate has not moved
       const int32_t since = a.cand_since[e];
-      const int m = mesh_of_item(a, i);
+      m = mesh_of_item(a, i);
       const DevMesh& dm = a.meshes[m];
       if (dm.metric == mesh_metric(METRIC)) {
         const MeshState& st = a.state[m];
-        const bool stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
-        const unsigned long long key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
-        int32_t target = b;
-        bool live = true;
+        stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
+        key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
+        live = true;
         if (stale) {
           a.cand_since[e] = st.loops * kMatchRounds + round;
           target = kNull;
-          live = propose_item<METRIC>(v, i, lab, st.connexity, st.constrained, dm.level, dm.scales, &target);
-          a.cand_tgt[e] = live ? target : -3;
-        }
-        if (live) {
-          if (lab != kNull) keyagg_min(&agg, lab, key);
-          keyagg_min(&agg, target, key);
+          use_pool = METRIC == 1 && lab != kNull && st.constrained;
+          if (use_pool)
+            kind = pool_screen(v, i, lab, st.connexity, nb, &s, &open, &ntask);
+          else  // no FP64 solve: the null-item rule or the isotropic one
+            live = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st.connexity, st.constrained, dm.level,
+                                                          dm.scales, &target);
         }
       }
+    }
+    if constexpr (METRIC == 1) {
+      int32_t pt = kNull;
+      const bool r = qem_pool_choose(a, v, &pools[threadIdx.x >> 6], kind, ntask, i, lab, m, nb, s, open, &pt);
+      if (use_pool) {
+        live = r;
+        target = pt;
+      }
+    }
+    if (stale) a.cand_tgt[e] = live ? target : -3;
+    if (live) {
+      if (lab != kNull) keyagg_min(&agg, lab, key);
+      keyagg_min(&agg, target, key);
     }
     keyagg_flush(&agg, a.prio);
   });

@@ -19,6 +19,19 @@ pytestmark = pytest.mark.gpu
 ISO, QEM = acvd_amd.METRIC_ISOTROPIC, acvd_amd.METRIC_QEM
 
 
+def centroid_split(xyz, tri):
+    """Each triangle split 1-to-3 at its centroid: the original vertices get
+    twice their valence (10 and 12 on an icosphere), so a third of the rows
+    are longer than 8 neighbours."""
+    xyz = np.asarray(xyz)
+    tri = np.asarray(tri, np.int64)
+    c = xyz[tri].mean(axis=1).astype(xyz.dtype)
+    mid = len(xyz) + np.arange(len(tri))
+    out = np.concatenate([np.stack([tri[:, 0], tri[:, 1], mid], 1), np.stack([tri[:, 1], tri[:, 2], mid], 1),
+                          np.stack([tri[:, 2], tri[:, 0], mid], 1)])
+    return np.concatenate([xyz, c]), out.astype(np.int32)
+
+
 def small_cases():
     return {
         "ico5-iso": (meshgen.icosphere(5, 1, 0.02), ISO, 500, 0, None, 0.0),
@@ -28,6 +41,8 @@ def small_cases():
         "C1-iso": (meshgen.icosphere(6, 1, 0.02), ISO, 2000, 0, None, 0.0),
         "torus-graded-qem": (meshgen.torus(128, 64, 3), QEM, 600, 1, "H", 1.5),
         "ico4-qem-level1": (meshgen.icosphere(4, 5, 0.05), QEM, 100, 1, None, 0.0),
+        "split-ico4-qem": (centroid_split(*meshgen.icosphere(4, 9, 0.02)), QEM, 300, 1, None, 0.0),
+        "split-ico4-qem-constrained": (centroid_split(*meshgen.icosphere(4, 9, 0.02)), QEM, 250, 0, None, 0.0),
     }
 
 
