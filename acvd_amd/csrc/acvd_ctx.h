@@ -108,16 +108,12 @@ struct SweepArgs {
   Ctrl* ctrl;
   // Block-owned segments: evaluate block b appends the proposals of its
   // items to candidate segment b (entries [b * cand_cap, + seg_cand[b]));
-  // apply block b walks segment b and appends the clusters it modified to
-  // dirty segment b (capacity dirty_cap, per round) and the items it moved to
-  // moved segment b (capacity cand_cap, per pass).
+  // apply block b appends the items it moved to moved segment b (capacity
+  // cand_cap, per pass).
   int32_t n_seg;               // segments = evaluate / apply blocks
   int32_t cand_cap;
-  int32_t dirty_cap;
   int32_t* cand;               // item of each candidate
   int32_t* seg_cand;
-  int32_t* dirty;
-  int32_t* seg_dirty;
   int32_t* moved;
   int32_t* seg_moved;
   // scan tiles
@@ -159,7 +155,7 @@ hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* me
 hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbox, int32_t* tile_items,
                              int32_t* item_tile, void** tmp, size_t* tmp_bytes);
 hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained);
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool unconstrained);
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool constrained);
 hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a);
 hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained,
                          int round);

@@ -104,8 +104,8 @@ struct acvd_ctx {
   DBuf<ClusterGeom> d_geom;
   DBuf<double> d_T, d_evec;
   DBuf<int32_t> d_cand_tgt, d_cand_lab, d_cand_since, d_cl_round;
-  DBuf<int32_t> d_work, d_work_count, d_cand, d_seg_cand, d_dirty, d_seg_dirty, d_moved, d_seg_moved;
-  int32_t work_cap = 0, n_seg = 0, cand_cap = 0, dirty_cap = 0;
+  DBuf<int32_t> d_work, d_work_count, d_cand, d_seg_cand, d_moved, d_seg_moved;
+  int32_t work_cap = 0, n_seg = 0, cand_cap = 0;
   // scan tiles
   int32_t n_tiles = 0;
   DBuf<int32_t> d_tile_items, d_item_tile, d_tile_mesh, d_tile_cnt, d_tile_set, d_tile_prop;
@@ -173,10 +173,7 @@ struct acvd_ctx {
     a.work_cap = work_cap;
     a.n_seg = n_seg;
     a.cand_cap = cand_cap;
-    a.dirty_cap = dirty_cap;
     a.seg_cand = d_seg_cand.p;
-    a.dirty = d_dirty.p;
-    a.seg_dirty = d_seg_dirty.p;
     a.moved = d_moved.p;
     a.seg_moved = d_seg_moved.p;
     a.cand = d_cand.p;
@@ -473,23 +470,20 @@ int build(acvd_ctx* c) {
     HIP_TRY(hipMemsetAsync(c->d_work_count.p, 0, kWorkSlices * sizeof(int32_t), s));
     // candidate segments: evaluate block b takes work positions
     // b * kBlock + j * n_seg * kBlock (at most N of them) and proposes at most
-    // once per item; apply block b appends two dirty clusters per move and
-    // one moved item per candidate (once per pass)
+    // once per item; apply block b appends one moved item per candidate
+    // (once per pass)
     c->n_seg = int32_t(sweep_grid_for(N));
     const int64_t steps = (N + int64_t(c->n_seg) * kBlock - 1) / (int64_t(c->n_seg) * kBlock);
     const int64_t cap = std::max<int64_t>(1, steps) * kBlock;
-    if (2 * cap * c->n_seg > (int64_t(1) << 31) - 1) return fail(ACVD_ERR_ARG, "too many items for one context");
+    if (cap * c->n_seg > (int64_t(1) << 31) - 1) return fail(ACVD_ERR_ARG, "too many items for one context");
     c->cand_cap = int32_t(cap);
-    c->dirty_cap = int32_t(2 * cap);
     const size_t nc = size_t(c->cand_cap) * size_t(c->n_seg);
     HIP_TRY(c->d_cand.alloc(nc));
     HIP_TRY(c->d_cand_tgt.alloc(nc));
     HIP_TRY(c->d_cand_lab.alloc(nc));
     HIP_TRY(c->d_cand_since.alloc(nc));
     HIP_TRY(c->d_moved.alloc(nc));
-    HIP_TRY(c->d_dirty.alloc(2 * nc));
     HIP_TRY(c->d_seg_cand.alloc(c->n_seg));
-    HIP_TRY(c->d_seg_dirty.alloc(c->n_seg));
     HIP_TRY(c->d_seg_moved.alloc(c->n_seg));
   }
   HIP_TRY(c->d_parent.alloc(N));
@@ -551,10 +545,10 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc) {
   for (int p = 0; p < kChunk; ++p) {
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
-    HIP_TRY(launch_apply(s, a, 0, qem_unc));
+    HIP_TRY(launch_apply(s, a, 0, !qem_unc));
     for (int r = 1; r < kMatchRounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
-      HIP_TRY(launch_apply(s, a, r, qem_unc));
+      HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
     HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(launch_controller(s, a));
@@ -574,10 +568,10 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc) {
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
-    HIP_TRY(launch_apply(s, a, 0, qem_unc));
+    HIP_TRY(launch_apply(s, a, 0, !qem_unc));
     for (int r = 1; r < kMatchRounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
-      HIP_TRY(launch_apply(s, a, r, qem_unc));
+      HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
     HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));

@@ -825,23 +825,27 @@ __device__ inline int block_sum(int v) {
 
 // Apply round `round`: a live candidate holding the minimum round key at both
 // of its clusters moves: label, fixed-point accumulator deltas (plain
-// read-modify-write: the round owns both clusters), sizes and stamps.  The
-// blocks walk all candidates evenly; the two clusters join the block's dirty
-// segment (their centroid and T are derived by k_derive_dirty) and the item
-// its moved segment (tile sets are extended by k_tile_update after the pass).
-template <bool kDeriveHere>
+// read-modify-write: the round owns both clusters), sizes and stamps; then the
+// centroid, W and (QEM) T of both clusters are derived from their final
+// accumulators (the round's only writer of them).  The blocks walk all
+// candidates evenly; the item joins its block's moved segment (tile sets are
+// extended by k_tile_update after the pass).
+//  kSolve = false (no mesh constrained): a derive is a centroid, computed by
+//    the mover itself;
+//  kSolve = true: a derive may need an FP64 solve; the clusters the movers of
+//    a wave modified are listed in LDS and derived one per lane.
+struct DeriveList {
+  int32_t cl[2 * 64];
+};
+
+template <bool kSolve>
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
-  __shared__ int32_t s_dirty, s_moved;
+  __shared__ int32_t s_moved;
   __shared__ SegIndex si;
-  const int64_t dseg = int64_t(blockIdx.x) * a.dirty_cap, mseg = int64_t(blockIdx.x) * a.cand_cap;
-  if (round > 0 && round_was_empty(a, round - 1)) {
-    if (threadIdx.x == 0) a.seg_dirty[blockIdx.x] = 0;
-    return;
-  }
-  if (threadIdx.x == 0) {
-    s_dirty = 0;
-    s_moved = round == 0 ? 0 : a.seg_moved[blockIdx.x];
-  }
+  __shared__ DeriveList dlist[kSolve ? kBlock / 64 : 1];
+  const int64_t mseg = int64_t(blockIdx.x) * a.cand_cap;
+  if (round > 0 && round_was_empty(a, round - 1)) return;
+  if (threadIdx.x == 0) s_moved = round == 0 ? 0 : a.seg_moved[blockIdx.x];
   __syncthreads();
   int mods = 0;
   walk_candidates(a, &si, [&](bool valid, int64_t e) {
@@ -894,38 +898,40 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
         a.cl_round[b] = rtag;
         ++mods;
         if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
-        if (kDeriveHere) {  // centroid only (no mesh constrained): one division per cluster
+        if (!kSolve) {
           if (lab != kNull) derive_cluster(a, lab, dm, 0);
           derive_cluster(a, b, dm, 0);
         }
       }
     }
-    if (!kDeriveHere) {
-      const int64_t ea = seg_append(dseg, &s_dirty, win && lab != kNull);
-      if (ea >= 0) a.dirty[ea] = lab;
-      const int64_t eb = seg_append(dseg, &s_dirty, win);
-      if (eb >= 0) a.dirty[eb] = b;
+    if constexpr (kSolve) {
+      DeriveList* dl = &dlist[threadIdx.x >> 6];
+      const int cnt = win ? (lab != kNull ? 2 : 1) : 0;
+      const int inc = wave_prefix(cnt);
+      const int total = __shfl(inc, 63);
+      if (cnt) {
+        dl->cl[inc - cnt] = b;
+        if (cnt == 2) dl->cl[inc - 1] = lab;
+      }
+      // the movers' accumulator stores before the other lanes' loads (one
+      // CU: workgroup scope orders them through its L1)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      for (int k = __lane_id(); k < total; k += 64) {
+        const int64_t c = dl->cl[k];
+        const int m = mesh_of_cluster(a, c);
+        derive_cluster(a, c, a.meshes[m], a.state[m].constrained);
+      }
+      wave_lds_sync();
     }
     const int64_t em = seg_append(mseg, &s_moved, win);
     if (em >= 0) a.moved[em] = int32_t(i);
   });
   const int tm = block_sum(mods);  // synchronises the block
   if (threadIdx.x == 0) {
-    a.seg_dirty[blockIdx.x] = s_dirty;
     a.seg_moved[blockIdx.x] = s_moved;
     if (tm) atomicAdd(&a.ctrl->mods[round][blockIdx.x % kSlices], tm);
-  }
-}
-
-// Centroid, W and (QEM) T of the clusters the apply block of the same index
-// modified in this round, from their final accumulators.
-__global__ __launch_bounds__(kBlock) void k_derive_dirty(SweepArgs a) {
-  const int32_t n = a.seg_dirty[blockIdx.x];
-  const int64_t seg = int64_t(blockIdx.x) * a.dirty_cap;
-  for (int32_t k = threadIdx.x; k < n; k += kBlock) {
-    const int64_t c = a.dirty[seg + k];
-    const int m = mesh_of_cluster(a, c);
-    derive_cluster(a, c, a.meshes[m], a.state[m].constrained);
   }
 }
 
@@ -1277,19 +1283,13 @@ hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any
   return hipGetLastError();
 }
 
-// One apply block per candidate segment (its dirty and moved segments have
-// the same index), then the derive of the round's dirty clusters.
-// One apply block per candidate segment (its dirty and moved segments have
-// the same index), then the derive of the round's dirty clusters.  When no
-// mesh is constrained (unconstrained = true) a derive is the centroid alone
-// and the movers compute it themselves.
-hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool unconstrained) {
-  if (unconstrained) {
+// One apply block per candidate segment (its moved segment has the same
+// index).  constrained: some mesh is constrained (derives may need a solve).
+hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool constrained) {
+  if (constrained)
     hipLaunchKernelGGL(k_apply<true>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-  } else {
+  else
     hipLaunchKernelGGL(k_apply<false>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-    hipLaunchKernelGGL(k_derive_dirty, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a);
-  }
   return hipGetLastError();
 }
 

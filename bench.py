@@ -112,7 +112,7 @@ def run_gpu_leg(args):
     ctx.process()
     tst = ctx.stats(0)
     ctx.set_timing(False)
-    names = ["k_scan", "k_eval<QEM>", "accept rounds (k_apply, k_reeval, k_derive_dirty, k_tile_update)", "k_controller"]
+    names = ["k_scan", "k_eval<QEM>", "accept rounds (k_apply, k_reeval, k_tile_update)", "k_controller"]
     launches = [max(1, v) for v in tst["kernel_launches"]]
     avg_ms = [tst["kernel_ms"][k] / launches[k] for k in range(4)]
     work_per = tst["ctx_evaluated"] / launches[0]
