@@ -243,11 +243,14 @@ ACVD_HD ClusterGeom geom_from_fixed(int64_t W, int64_t Sx, int64_t Sy, int64_t S
 }
 
 // T = W |q - c|^2 of a cluster given its fixed accumulators (QEM metric with
-// active constraints; 0 otherwise).  Optionally returns q.
+// active constraints; 0 otherwise).  Optionally returns q.  Vout (when given)
+// always receives orthonormal vectors: the identity when there is no solve.
 ACVD_HD double qem_T_from_fixed(const int64_t* q10, const ClusterGeom& g, const Scales& s, int level,
                                 double* q_out, const double* V0 = nullptr, double* Vout = nullptr) {
   if (!(g.W > 0.0) || level <= 0) {
     if (q_out) { q_out[0] = g.cx; q_out[1] = g.cy; q_out[2] = g.cz; }
+    if (Vout)
+      for (int k = 0; k < 9; ++k) Vout[k] = (k % 4 == 0) ? 1.0 : 0.0;
     return 0.0;
   }
   double Q[9];

@@ -103,7 +103,11 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(SweepArgs a, const int32_
   if (threadIdx.x == 0 && mesh_count && s_count) atomicAdd(&mesh_count[m], s_count);
 }
 
-__device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm, int constrained) {
+// Centroid, W and (QEM, constrained) T of cluster c from its accumulators;
+// the solve leaves the cluster's eigenvectors in evec, which warm-start the
+// trial solves on the cluster and, with warm = true, this solve itself (the
+// derive after a move, whose quadric differs from the last one by one item).
+__device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm, int constrained, bool warm) {
   const int64_t W = ld_i64(&a.acc_ws[4 * c]);
   const ClusterGeom g = geom_from_fixed(W, ld_i64(&a.acc_ws[4 * c + 1]), ld_i64(&a.acc_ws[4 * c + 2]),
                                         ld_i64(&a.acc_ws[4 * c + 3]), dm.scales);
@@ -113,8 +117,10 @@ __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm,
     if (constrained) {
       int64_t q10[10];
       for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
-      // cold solve; its eigenvectors warm-start the trial solves on this cluster
-      T = qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr, nullptr, a.evec + 9 * c);
+      double V0[9];
+      if (warm)
+        for (int k = 0; k < 9; ++k) V0[k] = a.evec[9 * c + k];
+      T = qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr, warm ? V0 : nullptr, a.evec + 9 * c);
     }
     a.T[c] = T;
   }
@@ -126,7 +132,7 @@ __global__ void k_derive(SweepArgs a, const int32_t* __restrict__ mask, int set_
   const int m = mesh_of_cluster(a, c);
   if (!mask[m]) return;
   const DevMesh dm = a.meshes[m];
-  derive_cluster(a, c, dm, a.state[m].constrained);
+  derive_cluster(a, c, dm, a.state[m].constrained, false);
   if (set_stamp) a.stamp[c] = a.state[m].loops;  // SetAllClustersToModified
 }
 
@@ -726,6 +732,58 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
   if (threadIdx.x == 0) a.seg_cand[blockIdx.x] = s_n;
 }
 
+// Re-evaluation of candidate entry e (when valid) for round `round`; every
+// lane of the wave calls it (the QEM solve pool is shared by the wave).
+// reg(c, key) registers an accept key at cluster c.
+template <int METRIC, class Reg>
+__device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemPool* pool, bool valid, int64_t e,
+                                    int round, Reg&& reg) {
+  const int32_t b = valid ? a.cand_tgt[e] : -1;
+  bool live = false, stale = false, use_pool = false;
+  int32_t i = 0, lab = kNull, target = b, m = 0;
+  unsigned long long key = 0;
+  int32_t nb[kRingCache];
+  QemItem s;
+  uint32_t open = 0;
+  int kind = kPoolNone, ntask = 0;
+  if (b >= 0) {
+    i = a.cand[e];
+    lab = a.cand_lab[e];  // a live candidate has not moved
+    const int32_t since = a.cand_since[e];
+    m = mesh_of_item(a, i);
+    const DevMesh& dm = a.meshes[m];
+    if (dm.metric == mesh_metric(METRIC)) {
+      const MeshState& st = a.state[m];
+      stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
+      key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
+      live = true;
+      if (stale) {
+        a.cand_since[e] = st.loops * kMatchRounds + round;
+        target = kNull;
+        use_pool = METRIC == 1 && lab != kNull && st.constrained;
+        if (use_pool)
+          kind = pool_screen(v, i, lab, st.connexity, nb, &s, &open, &ntask);
+        else  // no FP64 solve: the null-item rule or the isotropic one
+          live = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st.connexity, st.constrained, dm.level,
+                                                        dm.scales, &target);
+      }
+    }
+  }
+  if constexpr (METRIC == 1) {
+    int32_t pt = kNull;
+    const bool r = qem_pool_choose(a, v, pool, kind, ntask, i, lab, m, nb, s, open, &pt);
+    if (use_pool) {
+      live = r;
+      target = pt;
+    }
+  }
+  if (stale) a.cand_tgt[e] = live ? target : -3;
+  if (live) {
+    if (lab != kNull) reg(lab, key);
+    reg(target, key);
+  }
+}
+
 // Re-evaluate (round r > 0): a live candidate (cand_tgt >= 0) whose own
 // cluster or target moved since its last evaluation is evaluated again
 // against the state left by round r-1: it registers its round-r key with its
@@ -740,51 +798,10 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   keyagg_clear(&agg);
   __syncthreads();
   const SweepView v = make_view(a);
+  QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
   walk_candidates(a, &si, [&](bool valid, int64_t e) {
-    const int32_t b = valid ? a.cand_tgt[e] : -1;
-    bool live = false, stale = false, use_pool = false;
-    int32_t i = 0, lab = kNull, target = b, m = 0;
-    unsigned long long key = 0;
-    int32_t nb[kRingCache];
-    QemItem s;
-    uint32_t open = 0;
-    int kind = kPoolNone, ntask = 0;
-    if (b >= 0) {
-      i = a.cand[e];
-      lab = a.cand_lab[e];  // a live candidate has not moved
-      const int32_t since = a.cand_since[e];
-      m = mesh_of_item(a, i);
-      const DevMesh& dm = a.meshes[m];
-      if (dm.metric == mesh_metric(METRIC)) {
-        const MeshState& st = a.state[m];
-        stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
-        key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
-        live = true;
-        if (stale) {
-          a.cand_since[e] = st.loops * kMatchRounds + round;
-          target = kNull;
-          use_pool = METRIC == 1 && lab != kNull && st.constrained;
-          if (use_pool)
-            kind = pool_screen(v, i, lab, st.connexity, nb, &s, &open, &ntask);
-          else  // no FP64 solve: the null-item rule or the isotropic one
-            live = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st.connexity, st.constrained, dm.level,
-                                                          dm.scales, &target);
-        }
-      }
-    }
-    if constexpr (METRIC == 1) {
-      int32_t pt = kNull;
-      const bool r = qem_pool_choose(a, v, &pools[threadIdx.x >> 6], kind, ntask, i, lab, m, nb, s, open, &pt);
-      if (use_pool) {
-        live = r;
-        target = pt;
-      }
-    }
-    if (stale) a.cand_tgt[e] = live ? target : -3;
-    if (live) {
-      if (lab != kNull) keyagg_min(&agg, lab, key);
-      keyagg_min(&agg, target, key);
-    }
+    reeval_entry<METRIC>(a, v, pool, valid, e, round,
+                         [&](int32_t c, unsigned long long key) { keyagg_min(&agg, c, key); });
     keyagg_flush(&agg, a.prio);
   });
 }
@@ -838,6 +855,89 @@ struct DeriveList {
   int32_t cl[2 * 64];
 };
 
+// Apply of candidate entry e (when valid) in round `round`; every lane of the
+// wave calls it.  Returns true when the candidate moved (item *moved_item).
+template <bool kSolve>
+__device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool valid, int64_t e, int round,
+                                   int32_t* moved_item) {
+  bool win = false;
+  int64_t i = 0;
+  int32_t lab = kNull;
+  const int32_t b = valid ? a.cand_tgt[e] : -1;
+  if (b >= 0) {
+    i = a.cand[e];
+    lab = a.cand_lab[e];
+    const int m = mesh_of_item(a, i);
+    MeshState* st = &a.state[m];
+    const int32_t L = st->loops;
+    const DevMesh& dm = a.meshes[m];
+    const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
+    win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
+    if (win) {
+      // exclusive owner of clusters lab and b for this round
+      a.labels[i] = b;
+      a.cand_tgt[e] = -3;
+      const float4 p = a.xw[i];
+      const Scales sc = dm.scales;
+      const unsigned long long iw = (unsigned long long)fix_w(p.w, sc);
+      const unsigned long long is[3] = {(unsigned long long)fix_s(p.w, p.x, sc),
+                                        (unsigned long long)fix_s(p.w, p.y, sc),
+                                        (unsigned long long)fix_s(p.w, p.z, sc)};
+      const int32_t rtag = L * kMatchRounds + round;
+      if (lab != kNull) {
+        unsigned long long* wa = a.acc_ws + 4 * int64_t(lab);
+        wa[0] -= iw;
+        for (int q = 0; q < 3; ++q) wa[1 + q] -= is[q];
+        a.sizes[lab] -= 1;
+        a.stamp[lab] = L;
+        a.cl_round[lab] = rtag;
+      }
+      unsigned long long* wb = a.acc_ws + 4 * int64_t(b);
+      wb[0] += iw;
+      for (int q = 0; q < 3; ++q) wb[1 + q] += is[q];
+      if (dm.metric == 1) {
+        unsigned long long* qa = lab != kNull ? a.acc_q + 10 * int64_t(lab) : nullptr;
+        unsigned long long* qb = a.acc_q + 10 * int64_t(b);
+        for (int q = 0; q < 10; ++q) {
+          const unsigned long long iq = (unsigned long long)fix_q(a.q[10 * i + q], sc);
+          if (qa) qa[q] -= iq;
+          qb[q] += iq;
+        }
+      }
+      a.sizes[b] += 1;
+      a.stamp[b] = L;
+      a.cl_round[b] = rtag;
+      if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
+      if (!kSolve) {
+        if (lab != kNull) derive_cluster(a, lab, dm, 0, false);
+        derive_cluster(a, b, dm, 0, false);
+      }
+    }
+  }
+  if constexpr (kSolve) {
+    const int cnt = win ? (lab != kNull ? 2 : 1) : 0;
+    const int inc = wave_prefix(cnt);
+    const int total = __shfl(inc, 63);
+    if (cnt) {
+      dl->cl[inc - cnt] = b;
+      if (cnt == 2) dl->cl[inc - 1] = lab;
+    }
+    // the movers' accumulator stores before the other lanes' loads (one
+    // CU: workgroup scope orders them through its L1)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int k = __lane_id(); k < total; k += 64) {
+      const int64_t c = dl->cl[k];
+      const int m = mesh_of_cluster(a, c);
+      derive_cluster(a, c, a.meshes[m], a.state[m].constrained, true);
+    }
+    wave_lds_sync();
+  }
+  *moved_item = int32_t(i);
+  return win;
+}
+
 template <bool kSolve>
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ int32_t s_moved;
@@ -847,86 +947,14 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   if (round > 0 && round_was_empty(a, round - 1)) return;
   if (threadIdx.x == 0) s_moved = round == 0 ? 0 : a.seg_moved[blockIdx.x];
   __syncthreads();
+  DeriveList* dl = &dlist[kSolve ? threadIdx.x >> 6 : 0];
   int mods = 0;
   walk_candidates(a, &si, [&](bool valid, int64_t e) {
-    bool win = false;
-    int64_t i = 0;
-    int32_t lab = kNull;
-    const int32_t b = valid ? a.cand_tgt[e] : -1;
-    if (b >= 0) {
-      i = a.cand[e];
-      lab = a.cand_lab[e];
-      const int m = mesh_of_item(a, i);
-      MeshState* st = &a.state[m];
-      const int32_t L = st->loops;
-      const DevMesh& dm = a.meshes[m];
-      const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
-      win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
-      if (win) {
-        // exclusive owner of clusters lab and b for this round
-        a.labels[i] = b;
-        a.cand_tgt[e] = -3;
-        const float4 p = a.xw[i];
-        const Scales sc = dm.scales;
-        const unsigned long long iw = (unsigned long long)fix_w(p.w, sc);
-        const unsigned long long is[3] = {(unsigned long long)fix_s(p.w, p.x, sc),
-                                          (unsigned long long)fix_s(p.w, p.y, sc),
-                                          (unsigned long long)fix_s(p.w, p.z, sc)};
-        const int32_t rtag = L * kMatchRounds + round;
-        if (lab != kNull) {
-          unsigned long long* wa = a.acc_ws + 4 * int64_t(lab);
-          wa[0] -= iw;
-          for (int q = 0; q < 3; ++q) wa[1 + q] -= is[q];
-          a.sizes[lab] -= 1;
-          a.stamp[lab] = L;
-          a.cl_round[lab] = rtag;
-        }
-        unsigned long long* wb = a.acc_ws + 4 * int64_t(b);
-        wb[0] += iw;
-        for (int q = 0; q < 3; ++q) wb[1 + q] += is[q];
-        if (dm.metric == 1) {
-          unsigned long long* qa = lab != kNull ? a.acc_q + 10 * int64_t(lab) : nullptr;
-          unsigned long long* qb = a.acc_q + 10 * int64_t(b);
-          for (int q = 0; q < 10; ++q) {
-            const unsigned long long iq = (unsigned long long)fix_q(a.q[10 * i + q], sc);
-            if (qa) qa[q] -= iq;
-            qb[q] += iq;
-          }
-        }
-        a.sizes[b] += 1;
-        a.stamp[b] = L;
-        a.cl_round[b] = rtag;
-        ++mods;
-        if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
-        if (!kSolve) {
-          if (lab != kNull) derive_cluster(a, lab, dm, 0);
-          derive_cluster(a, b, dm, 0);
-        }
-      }
-    }
-    if constexpr (kSolve) {
-      DeriveList* dl = &dlist[threadIdx.x >> 6];
-      const int cnt = win ? (lab != kNull ? 2 : 1) : 0;
-      const int inc = wave_prefix(cnt);
-      const int total = __shfl(inc, 63);
-      if (cnt) {
-        dl->cl[inc - cnt] = b;
-        if (cnt == 2) dl->cl[inc - 1] = lab;
-      }
-      // the movers' accumulator stores before the other lanes' loads (one
-      // CU: workgroup scope orders them through its L1)
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      for (int k = __lane_id(); k < total; k += 64) {
-        const int64_t c = dl->cl[k];
-        const int m = mesh_of_cluster(a, c);
-        derive_cluster(a, c, a.meshes[m], a.state[m].constrained);
-      }
-      wave_lds_sync();
-    }
+    int32_t i;
+    const bool win = apply_entry<kSolve>(a, dl, valid, e, round, &i);
+    mods += win;
     const int64_t em = seg_append(mseg, &s_moved, win);
-    if (em >= 0) a.moved[em] = int32_t(i);
+    if (em >= 0) a.moved[em] = i;
   });
   const int tm = block_sum(mods);  // synchronises the block
   if (threadIdx.x == 0) {

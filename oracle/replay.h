@@ -86,14 +86,18 @@ struct Replay {
     }
   }
 
-  void derive(int32_t c) {
+  // warm: the derive after a move, warm-started from the cluster's last
+  // eigenvectors (as on the device); the recompute solves cold
+  void derive(int32_t c, bool warm) {
     const int64_t* ws = reinterpret_cast<const int64_t*>(&acc_ws[4 * int64_t(c)]);
     geom[c] = acvd::geom_from_fixed(ws[0], ws[1], ws[2], ws[3], sc);
     if (metric == 1) {
       const int64_t* q10 = reinterpret_cast<const int64_t*>(&acc_q[10 * int64_t(c)]);
-      T[c] = st.constrained
-                 ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr, nullptr, &evec[9 * int64_t(c)])
-                 : 0.0;
+      double V0[9];
+      for (int k = 0; k < 9; ++k) V0[k] = evec[9 * int64_t(c) + k];
+      T[c] = st.constrained ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr, warm ? V0 : nullptr,
+                                                     &evec[9 * int64_t(c)])
+                            : 0.0;
     }
   }
 
@@ -115,7 +119,7 @@ struct Replay {
       count++;
     }
     for (int32_t c = 0; c < K; ++c) {
-      derive(c);
+      derive(c, false);
       if (set_stamp) stamp[c] = st.loops;
     }
     return count;
@@ -126,7 +130,7 @@ struct Replay {
     for (int k = 0; k < 3; ++k) acc_ws[4 * int64_t(c) + 1 + k] += (unsigned long long)(sign * is[k]);
     if (metric == 1)
       for (int k = 0; k < 10; ++k) acc_q[10 * int64_t(c) + k] += (unsigned long long)(sign * iq[k]);
-    derive(c);
+    derive(c, true);
   }
 
   void apply_move(int64_t i, int32_t lab, int32_t b, int32_t L, int32_t rtag) {
