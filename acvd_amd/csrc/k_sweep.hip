@@ -895,7 +895,10 @@ __device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool vali
       unsigned long long* wb = a.acc_ws + 4 * int64_t(b);
       wb[0] += iw;
       for (int q = 0; q < 3; ++q) wb[1 + q] += is[q];
-      if (dm.metric == 1) {
+      // quadric accumulators: while no mesh is constrained (kSolve = false)
+      // nothing reads them, and the recompute at the convergence that
+      // constrains a mesh rebuilds them from the labels
+      if (kSolve && dm.metric == 1) {
         unsigned long long* qa = lab != kNull ? a.acc_q + 10 * int64_t(lab) : nullptr;
         unsigned long long* qb = a.acc_q + 10 * int64_t(b);
         for (int q = 0; q < 10; ++q) {
