@@ -122,8 +122,10 @@ struct acvd_ctx {
   size_t cub_bytes = 0;
   // captured pass chunks: [0] some QEM mesh constrained (QEM kernels), [1] no
   // QEM mesh constrained (their isotropic-rule kernels)
-  hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};
-  hipGraph_t chunk_graph[2] = {nullptr, nullptr};
+  // pass-chunk graphs by (QEM meshes unconstrained, tile sets maintained):
+  // index qem_unc + 2 * tiles
+  hipGraphExec_t chunk_exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraph_t chunk_graph[4] = {nullptr, nullptr, nullptr, nullptr};
   double build_ms = 0.0, converge_ms = 0.0;
   Ctrl last_ctrl{};
   bool timing = false;
@@ -188,7 +190,7 @@ struct acvd_ctx {
   }
 
   void drop_graphs() {
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 4; ++k) {
       if (chunk_exec[k]) (void)hipGraphExecDestroy(chunk_exec[k]);
       if (chunk_graph[k]) (void)hipGraphDestroy(chunk_graph[k]);
       chunk_exec[k] = nullptr;
@@ -539,7 +541,7 @@ int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsign
   return ACVD_OK;
 }
 
-int enqueue_chunk(acvd_ctx* c, bool qem_unc) {
+int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   for (int p = 0; p < kChunk; ++p) {
@@ -550,14 +552,14 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
       HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
-    HIP_TRY(launch_tile_update(s, a));
+    if (tiles) HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(launch_controller(s, a));
   }
   return ACVD_OK;
 }
 
 // Passes launched one by one with events around each kernel.
-int run_chunk_timed(acvd_ctx* c, bool qem_unc) {
+int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   std::vector<hipEvent_t> ev(5 * kChunk);
@@ -573,7 +575,7 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
       HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
-    HIP_TRY(launch_tile_update(s, a));
+    if (tiles) HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
     HIP_TRY(launch_controller(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 4], s));
@@ -590,14 +592,14 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc) {
   return ACVD_OK;
 }
 
-int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc) {
+int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles) {
   hipStream_t s = c->stream;
-  if (c->timing) return run_chunk_timed(c, qem_unc);
+  if (c->timing) return run_chunk_timed(c, qem_unc, tiles);
   if (use_graph) {
-    const int k = qem_unc ? 1 : 0;
+    const int k = int(qem_unc) + 2 * int(tiles);
     if (!c->chunk_exec[k]) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_chunk(c, qem_unc);
+      int rc = enqueue_chunk(c, qem_unc, tiles);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(s, &g);
       if (rc != ACVD_OK) return rc;
@@ -608,7 +610,7 @@ int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc) {
     HIP_TRY(hipGraphLaunch(c->chunk_exec[k], s));
     return ACVD_OK;
   }
-  return enqueue_chunk(c, qem_unc);
+  return enqueue_chunk(c, qem_unc, tiles);
 }
 
 // True when no QEM mesh of the context is constrained (the state flips only
@@ -666,7 +668,9 @@ int minimize_energy(acvd_ctx* c) {
   HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill_i32(s, c->d_cand_pass.p, c->N, -2));
   HIP_TRY(launch_fill_i32(s, c->d_tile_prop.p, c->n_tiles, -2));
-  HIP_TRY(launch_tile_sets(s, a, nullptr));
+  // tile sets off: every tile is scanned until the moves per pass are few
+  bool tiles = false;
+  HIP_TRY(launch_fill_i32(s, c->d_tile_cnt.p, c->n_tiles, kTileSet + 1));
   HIP_TRY(launch_fill_i32(s, c->d_cl_round.p, c->K, -1));
   HIP_TRY(launch_fill_u64(s, c->d_prio.p, c->K, ~0ull));
   HIP_TRY(hipMemsetAsync(c->d_ctrl.p, 0, sizeof(Ctrl), s));
@@ -682,7 +686,7 @@ int minimize_energy(acvd_ctx* c) {
   for (int m = 0; m < M; ++m) loop_output |= c->meshes[m].params.console_output > 1;
   unsigned long long printed = 0;
   while (true) {
-    int rc2 = run_chunk(c, use_graph, qem_unconstrained(c));
+    int rc2 = run_chunk(c, use_graph, qem_unconstrained(c), tiles);
     if (rc2) return rc2;
     HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -711,7 +715,9 @@ int minimize_energy(acvd_ctx* c) {
       // with every cluster marked modified
       rc = recompute_statistics(c, c->d_mask.p, 1, nullptr);
       if (rc) return rc;
-      HIP_TRY(launch_tile_sets(s, a, c->d_mask.p));
+      // a phase starts with many moves per pass: tile sets off again
+      tiles = false;
+      HIP_TRY(launch_fill_i32(s, c->d_tile_cnt.p, c->n_tiles, kTileSet + 1));
       HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
       HIP_TRY(hipStreamSynchronize(s));
       bool any = false;
@@ -720,6 +726,13 @@ int minimize_energy(acvd_ctx* c) {
       continue;
     }
     if (!c->h_ctrl->any_sweep) break;
+    // tile sets on once the last pass moved at most K / kTileOnDivisor items:
+    // most tiles then hold no changed cluster and the scan skips them
+    if (!tiles && c->h_ctrl->passes > 0 &&
+        int64_t(c->h_ctrl->pass_log[(c->h_ctrl->passes - 1) % kPassLogCap][3]) * kTileOnDivisor <= c->K) {
+      HIP_TRY(launch_tile_sets(s, a, nullptr));
+      tiles = true;
+    }
     if (++guard > (int64_t(1) << 40)) return fail(ACVD_ERR_STATE, "sweep did not terminate");
   }
   HIP_TRY(hipMemcpyAsync(c->h_state, c->d_state.p, M * sizeof(MeshState), hipMemcpyDeviceToHost, s));
