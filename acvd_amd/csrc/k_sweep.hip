@@ -169,18 +169,19 @@ __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t la
   for (int k = 0; k < 8; ++k) nb[k] = k < deg ? a.col[r0 + k] : 0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) lb[k] = k < deg ? a.labels[nb[k]] : lab;
+  // neighbour stamps are read only while the item is not already dirty
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     if (lb[k] != lab) {
       boundary = true;
-      if (lb[k] != kNull && a.stamp[lb[k]] + 1 >= L) dirty = true;
+      if (!dirty && lb[k] != kNull && a.stamp[lb[k]] + 1 >= L) dirty = true;
     }
   }
   for (int32_t k = r0 + 8; k < r1; ++k) {
     const int32_t b = a.labels[a.col[k]];
     if (b != lab) {
       boundary = true;
-      if (b != kNull && a.stamp[b] + 1 >= L) dirty = true;
+      if (!dirty && b != kNull && a.stamp[b] + 1 >= L) dirty = true;
     }
   }
   return boundary && dirty;
