@@ -48,10 +48,13 @@ class Context:
         check(L.acvd_create(ctypes.byref(h), device))
         self._h = h
         self._nv = []
+        # held by the instance: module globals may already be gone when a
+        # context is collected at interpreter exit
+        self._destroy = L.acvd_destroy
 
     def close(self):
         if getattr(self, "_h", None):
-            N.lib().acvd_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     def __del__(self):
