@@ -31,7 +31,7 @@ class Params(ctypes.Structure):
         ("console_output", ctypes.c_int32),
         ("seed", ctypes.c_uint32),
         ("initial_sampling", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("curvature_indicator", ctypes.c_int32),
         ("gradation", ctypes.c_double),
     ]
 
@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
 EXPORTS = [
     "acvd_version", "acvd_last_error", "acvd_device_count", "acvd_params_init", "acvd_create", "acvd_destroy",
     "acvd_set_input_f64", "acvd_set_input_f32", "acvd_add_input_f64", "acvd_add_input_f32", "acvd_num_meshes",
-    "acvd_set_params", "acvd_get_params", "acvd_set_indicator", "acvd_set_initial_labels",
+    "acvd_set_params", "acvd_get_params", "acvd_set_indicator", "acvd_get_indicator", "acvd_set_initial_labels",
     "acvd_process_clustering", "acvd_get_labels", "acvd_get_cluster_sizes", "acvd_get_cluster_points",
     "acvd_get_energy", "acvd_get_stats", "acvd_get_adjacency_size", "acvd_get_adjacency", "acvd_get_items",
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
@@ -111,6 +111,7 @@ def lib():
     L.acvd_set_params.argtypes = [P, i32, ctypes.POINTER(Params)]
     L.acvd_get_params.argtypes = [P, i32, ctypes.POINTER(Params)]
     L.acvd_set_indicator.argtypes = [P, i32, P]
+    L.acvd_get_indicator.argtypes = [P, i32, P]
     L.acvd_set_initial_labels.argtypes = [P, i32, P]
     L.acvd_process_clustering.argtypes = [P]
     L.acvd_build.argtypes = [P]

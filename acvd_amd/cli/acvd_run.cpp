@@ -12,8 +12,9 @@
 //   -o directory  output directory;  -of file  output file (simplification.ply)
 //   -cd file      per-vertex density indicator, one value per line (the
 //                 reference reads a VTK image instead); without it and with
-//                 gradation > 0 the indicator is |H|, the cotangent-formula mean
-//                 curvature (the reference estimates curvature with VTK)
+//                 gradation > 0, Remesh weights the items with the reference's
+//                 curvature indicator (vtkCurvatureMeasure), computed on the
+//                 device
 //   -qem 0/1      QEM metric with unconstrained initialisation (ACVDQ)
 //   -m -d -b -l -w -cmin -cmax -cf -np -p   accepted; not supported (warned)
 //
@@ -32,8 +33,6 @@
 
 #include "acvd/remeshing.hpp"
 
-extern "C" int acvd_mean_curvature(int64_t V, const float* xyz32, const double* xyz64, int64_t F,
-                                   const int32_t* tri, double* absH);
 
 namespace {
 
@@ -277,11 +276,6 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "acvd_run: %s\n", err.c_str());
       return 1;
     }
-    ind = &indicator;
-  } else if (gradation > 0) {
-    indicator.resize(size_t(mesh.GetNumberOfPoints()));
-    acvd_mean_curvature(mesh.GetNumberOfPoints(), nullptr, mesh.points.data(), mesh.GetNumberOfCells(),
-                        mesh.triangles.data(), indicator.data());
     ind = &indicator;
   }
   const std::string prefix = out_dir.empty() ? "" : out_dir + "/";

@@ -142,6 +142,9 @@ hipError_t exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, in
                               size_t* tmp_bytes);
 hipError_t launch_ring_adj(hipStream_t s, int64_t n, const int32_t* row_ptr, const int32_t* col,
                             unsigned long long* ring_adj);
+hipError_t launch_curvature(hipStream_t s, int64_t off, int64_t n, const double* pos, const int32_t* tri,
+                            const int32_t* row_ptr, const int32_t* col, const int32_t* vfptr, const int32_t* vfcol,
+                            double* out);
 hipError_t launch_item_build(hipStream_t s, const SweepArgs& a, const double* pos, const int32_t* tri_global,
                              const int32_t* vfptr, const int32_t* vfcol, const double* indicator,
                              double* raw_w, float4* xw, float* q);

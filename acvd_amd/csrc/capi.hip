@@ -66,7 +66,8 @@ struct HostMesh {
   int64_t nv = 0, nf = 0;
   std::vector<double> xyz;
   std::vector<int32_t> tri;
-  std::vector<double> indicator;
+  std::vector<double> indicator;  // set by the caller (SetCurvatureInfo)
+  std::vector<double> curvature;  // computed by the last build (curvature_indicator)
   std::vector<int32_t> init_labels;
   acvd_params params;
   // layout in the packed context
@@ -296,6 +297,17 @@ int build(acvd_ctx* c) {
   std::vector<int32_t> tile_mesh(c->n_tiles);
   std::vector<float> bbox(6 * M);
   bool any_ind = false;
+  // meshes whose indicator is the reference's curvature measure, computed
+  // below once the adjacency exists (vtkDiscreteRemeshing::SamplingPreProcessing,
+  // DiscreteRemeshing/vtkDiscreteRemeshing.h:583-653, when the metric needs it:
+  // gradation > 0, vtkQEMetricForClustering.h:63-67)
+  std::vector<char> curv(M, 0);
+  bool any_curv = false;
+  for (int m = 0; m < M; ++m) {
+    const HostMesh& h = c->meshes[m];
+    curv[m] = h.indicator.empty() && h.params.curvature_indicator && h.params.gradation > 0.0;
+    any_curv |= curv[m] != 0;
+  }
   int64_t foff = 0, toff = 0;
   for (int m = 0; m < M; ++m) {
     HostMesh& h = c->meshes[m];
@@ -307,7 +319,7 @@ int build(acvd_ctx* c) {
     d.metric = h.params.metric == ACVD_METRIC_QEM ? 1 : 0;
     d.level = h.params.quadrics_optimization_level;
     d.seed = h.params.seed;
-    d.has_indicator = h.indicator.empty() ? 0 : 1;
+    d.has_indicator = (h.indicator.empty() && !curv[m]) ? 0 : 1;
     d.gradation = h.params.gradation;
     d.clamp_ratio = d.metric == 1 ? 10000.0 : 100000.0;
     d.tile_off = int32_t(toff);
@@ -366,7 +378,7 @@ int build(acvd_ctx* c) {
   DBuf<double> d_pos, d_ind, d_raw;
   HIP_TRY(d_pos.alloc(3 * N));
   HIP_TRY(hipMemcpyAsync(d_pos.p, pos.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
-  if (any_ind) {
+  if (any_ind || any_curv) {
     HIP_TRY(d_ind.alloc(N));
     HIP_TRY(hipMemcpyAsync(d_ind.p, ind.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
   }
@@ -406,13 +418,25 @@ int build(acvd_ctx* c) {
     HIP_TRY(launch_csr_compact(s, N, tptr.p, tcol.p, c->d_row_ptr.p, c->d_col.p));
     HIP_TRY(c->d_ring_adj.alloc(N));
     HIP_TRY(launch_ring_adj(s, N, c->d_row_ptr.p, c->d_col.p, c->d_ring_adj.p));
+    // ---- curvature indicators (before the weights that use them)
+    for (int m = 0; m < M; ++m) {
+      HostMesh& h = c->meshes[m];
+      h.curvature.clear();
+      if (!curv[m]) continue;
+      HIP_TRY(launch_curvature(s, h.item_off, h.nv, d_pos.p, c->d_tri.p, c->d_row_ptr.p, c->d_col.p, vfptr.p,
+                               vfcol.p, d_ind.p));
+      h.curvature.resize(h.nv);
+      HIP_TRY(hipMemcpyAsync(h.curvature.data(), d_ind.p + h.item_off, h.nv * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    }
+    if (any_curv) HIP_TRY(hipStreamSynchronize(s));
     // ---- items
     HIP_TRY(c->d_xw.alloc(N));
     HIP_TRY(c->d_q.alloc(10 * N));  // vertex quadrics of every mesh
     HIP_TRY(d_raw.alloc(N));
     SweepArgs a = c->args();
-    HIP_TRY(launch_item_build(s, a, d_pos.p, c->d_tri.p, vfptr.p, vfcol.p, any_ind ? d_ind.p : nullptr, d_raw.p,
-                              c->d_xw.p, c->d_q.p));
+    HIP_TRY(launch_item_build(s, a, d_pos.p, c->d_tri.p, vfptr.p, vfcol.p, (any_ind || any_curv) ? d_ind.p : nullptr,
+                              d_raw.p, c->d_xw.p, c->d_q.p));
   }
   // ---- scan tiles in spatial order
   {
@@ -751,8 +775,10 @@ int initial_labels(acvd_ctx* c) {
       std::vector<int32_t> lab(h.nv);
       if (h.init_labels.empty()) {
         // the reference's region growing, then local ids -> global
+        // the indicator of the weights: the set one, else the computed curvature
+        const std::vector<double>& ind = !h.indicator.empty() ? h.indicator : h.curvature;
         if (!reference_initial_sampling(h.nv, h.xyz.data(), h.nf, h.tri.data(), h.params.metric,
-                                        h.indicator.empty() ? nullptr : h.indicator.data(), h.params.gradation, K,
+                                        ind.empty() ? nullptr : ind.data(), h.params.gradation, K,
                                         lab.data()))
           return fail(ACVD_ERR_STATE, "initial sampling failed");
         for (int64_t i = 0; i < h.nv; ++i) lab[i] = lab[i] == K ? kNull : int32_t(lab[i] + h.clus_off);
@@ -808,7 +834,7 @@ void acvd_params_init(acvd_params* p) {
   p->console_output = 0;
   p->seed = 0;
   p->initial_sampling = 0;
-  p->reserved = 0;
+  p->curvature_indicator = 0;
   p->gradation = 0.0;
 }
 
@@ -887,6 +913,17 @@ int acvd_set_indicator(acvd_ctx* ctx, int32_t mesh_id, const double* indicator) 
   if (!indicator) h.indicator.clear();
   else h.indicator.assign(indicator, indicator + h.nv);
   ctx->invalidate();
+  return ACVD_OK;
+}
+
+int acvd_get_indicator(acvd_ctx* ctx, int32_t mesh_id, double* out) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!out) return fail(ACVD_ERR_ARG, "null argument");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  const std::vector<double>& src = !h.indicator.empty() ? h.indicator : h.curvature;
+  if (src.empty())
+    return fail(ACVD_ERR_STATE, "no indicator: none was set and no build computed the curvature indicator");
+  std::copy(src.begin(), src.end(), out);
   return ACVD_OK;
 }
 

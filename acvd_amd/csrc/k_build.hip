@@ -297,6 +297,230 @@ hipError_t exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, in
   return hipcub::DeviceScan::ExclusiveSum(*tmp, need, in, out, int(n), s);
 }
 
+// ------------------------------------------------------ curvature indicator
+// The reference's per-vertex curvature indicator (vtkCurvatureMeasure with
+// polynomial fitting on vertices, RingSize 3, as
+// vtkDiscreteRemeshing::SamplingPreProcessing sets it up,
+// DiscreteRemeshing/vtkDiscreteRemeshing.h:644-653): the faces with a vertex
+// within two edges of v (Common/vtkNeighbourhoodComputation.cxx:38-110), a
+// quadratic height field fitted to their barycentres in the area-weighted
+// normal frame, and sqrt(k1^2 + k2^2) of its shape operator
+// (Common/vtkCurvatureMeasure.cxx:188-509).  One thread per vertex, FP64.
+
+// True when row r of the (sorted) CSR holds w.
+__device__ inline bool row_has(const int32_t* rp, const int32_t* col, int32_t r, int32_t w) {
+  int32_t lo = rp[r], hi = rp[r + 1];
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    const int32_t c = col[mid];
+    if (c == w) return true;
+    if (c < w) lo = mid + 1;
+    else hi = mid;
+  }
+  return false;
+}
+
+// True when w lies within two edges of v.
+__device__ inline bool within_two(const int32_t* rp, const int32_t* col, int32_t v, int32_t w) {
+  if (w == v || row_has(rp, col, v, w)) return true;
+  for (int32_t k = rp[v]; k < rp[v + 1]; ++k)
+    if (row_has(rp, col, col[k], w)) return true;
+  return false;
+}
+
+// Calls f(face) once for every face with a vertex within two edges of v: a
+// face is taken at its lowest-id vertex among those within two edges, and
+// each such vertex u is visited once (v, its neighbours, then the
+// neighbours' neighbours at their first occurrence).
+template <class F>
+__device__ inline void for_each_star2_face(const int32_t* rp, const int32_t* col, const int32_t* vfptr,
+                                           const int32_t* vfcol, const int32_t* tri, int32_t v, F&& f) {
+  auto visit = [&](int32_t u) {
+    for (int32_t t = vfptr[u]; t < vfptr[u + 1]; ++t) {
+      const int32_t face = vfcol[t];
+      bool first = true;
+      for (int c = 0; c < 3; ++c) {
+        const int32_t w = tri[3 * int64_t(face) + c];
+        if (w < u && within_two(rp, col, v, w)) {
+          first = false;
+          break;
+        }
+      }
+      if (first) f(face);
+    }
+  };
+  visit(v);
+  for (int32_t k = rp[v]; k < rp[v + 1]; ++k) visit(col[k]);
+  for (int32_t k = rp[v]; k < rp[v + 1]; ++k) {
+    const int32_t p = col[k];
+    for (int32_t t = rp[p]; t < rp[p + 1]; ++t) {
+      const int32_t u = col[t];
+      if (u == v || row_has(rp, col, v, u)) continue;  // already visited
+      bool seen = false;
+      for (int32_t k2 = rp[v]; k2 < k && !seen; ++k2) seen = row_has(rp, col, col[k2], u);
+      if (!seen) visit(u);
+    }
+  }
+}
+
+__global__ void k_curvature(int64_t off, int64_t n, const double* __restrict__ pos, const int32_t* __restrict__ tri,
+                            const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ vfptr, const int32_t* __restrict__ vfcol,
+                            double* __restrict__ out) {
+  const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int32_t v = int32_t(off + idx);
+  // face data: area (Common/vtkSurface.cxx:1280-1283), unit normal, barycentre
+  auto face = [&](int32_t f, double* area, double* nrm, double* bary) {
+    const int32_t i0 = tri[3 * int64_t(f)], i1 = tri[3 * int64_t(f) + 1], i2 = tri[3 * int64_t(f) + 2];
+    const double p1[3] = {pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]};
+    const double p2[3] = {pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]};
+    const double p3[3] = {pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]};
+    double ea = 0, eb = 0, ec = 0;
+    for (int k = 0; k < 3; ++k) {
+      ea += (p1[k] - p2[k]) * (p1[k] - p2[k]);
+      eb += (p2[k] - p3[k]) * (p2[k] - p3[k]);
+      ec += (p3[k] - p1[k]) * (p3[k] - p1[k]);
+    }
+    const double t = 4.0 * ea * ec - (ea - eb + ec) * (ea - eb + ec);
+    *area = 0.25 * sqrt(t < 0 ? -t : t);
+    const double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    const double w[3] = {p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2]};
+    nrm[0] = u[1] * w[2] - u[2] * w[1];
+    nrm[1] = u[2] * w[0] - u[0] * w[2];
+    nrm[2] = u[0] * w[1] - u[1] * w[0];
+    const double ln = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    for (int k = 0; k < 3; ++k) {
+      nrm[k] = ln > 0.0 ? nrm[k] / ln : 0.0;
+      bary[k] = *area > 0.0 ? (p1[k] + p2[k] + p3[k]) / 3.0 : 0.0;  // GetCellMassProperties
+    }
+  };
+  double sa = 0.0, org[3] = {0, 0, 0}, f0[3] = {0, 0, 0};
+  int cnt = 0;
+  for_each_star2_face(rp, col, vfptr, vfcol, tri, v, [&](int32_t f) {
+    double ar, nr[3], bc[3];
+    face(f, &ar, nr, bc);
+    sa += ar;
+    for (int k = 0; k < 3; ++k) {
+      org[k] += ar * bc[k];
+      f0[k] += ar * nr[k];
+    }
+    ++cnt;
+  });
+  double ind = 0.0;
+  const double ln = sqrt(f0[0] * f0[0] + f0[1] * f0[1] + f0[2] * f0[2]);
+  if (cnt > 6 && sa > 0.0 && ln > 0.0) {
+    // frame: normal f2, tangents from the permuted normal
+    double f2[3], f1[3], t0[3], t1[3];
+    for (int k = 0; k < 3; ++k) {
+      org[k] /= sa;
+      f2[k] = f0[k] / ln;
+    }
+    f1[0] = f2[2];
+    f1[1] = f2[0];
+    f1[2] = f2[1];
+    t0[0] = f1[1] * f2[2] - f1[2] * f2[1];
+    t0[1] = f1[2] * f2[0] - f1[0] * f2[2];
+    t0[2] = f1[0] * f2[1] - f1[1] * f2[0];
+    const double lt = sqrt(t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2]);
+    if (lt > 0.0) {
+      for (int k = 0; k < 3; ++k) t0[k] /= lt;
+      t1[0] = f2[1] * t0[2] - f2[2] * t0[1];
+      t1[1] = f2[2] * t0[0] - f2[0] * t0[2];
+      t1[2] = f2[0] * t0[1] - f2[1] * t0[0];
+      // normal equations of z ~ [1, x, y, x^2, xy, y^2] . q (upper triangle)
+      double M[6][6], r[6], hs = 0.0;
+      for (int i = 0; i < 6; ++i) {
+        r[i] = 0.0;
+        for (int j = 0; j < 6; ++j) M[i][j] = 0.0;
+      }
+      for_each_star2_face(rp, col, vfptr, vfcol, tri, v, [&](int32_t f) {
+        double ar, nr[3], bc[3];
+        face(f, &ar, nr, bc);
+        double d[3] = {bc[0] - org[0], bc[1] - org[1], bc[2] - org[2]};
+        const double x = d[0] * t0[0] + d[1] * t0[1] + d[2] * t0[2];
+        const double y = d[0] * t1[0] + d[1] * t1[1] + d[2] * t1[2];
+        const double z = d[0] * f2[0] + d[1] * f2[1] + d[2] * f2[2];
+        hs += sqrt(x * x + y * y);
+        const double c[6] = {1.0, x, y, x * x, x * y, y * y};
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          r[i] += c[i] * z;
+#pragma unroll
+          for (int j = i; j < 6; ++j) M[i][j] += c[i] * c[j];
+        }
+      });
+      // the columns scaled by the mean |(x, y)| (conditioning, :302-315)
+      const double h = hs / double(cnt);
+      const double sc[6] = {1.0, h, h, h * h, h * h, h * h};
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        r[i] /= sc[i];
+#pragma unroll
+        for (int j = i; j < 6; ++j) M[i][j] /= sc[i] * sc[j];
+      }
+      // Cholesky of the scaled normal matrix
+      double L[6][6];
+      bool ok = h > 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double dj = M[j][j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k];
+        ok = ok && dj > 0.0;
+        L[j][j] = sqrt(dj > 0.0 ? dj : 1.0);
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+          double s = M[j][i];
+#pragma unroll
+          for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+          L[i][j] = s / L[j][j];
+        }
+      }
+      if (ok) {
+        double yv[6], q[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          double s = r[i];
+#pragma unroll
+          for (int k = 0; k < i; ++k) s -= L[i][k] * yv[k];
+          yv[i] = s / L[i][i];
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; --i) {
+          double s = yv[i];
+#pragma unroll
+          for (int k = i + 1; k < 6; ++k) s -= L[k][i] * q[k];
+          q[i] = s / L[i][i];
+        }
+        for (int i = 1; i < 6; ++i) q[i] /= sc[i];
+        // shape operator -II I^-1 with the reference's f = 2 q4 / s (:331-347)
+        const double E = 1.0 + q[1] * q[1], Fm = q[1] * q[2], G = 1.0 + q[2] * q[2];
+        const double sq = sqrt(q[1] * q[1] + 1.0 + q[2] * q[2]);
+        const double e = 2.0 * q[3] / sq, fm = 2.0 * q[4] / sq, g = 2.0 * q[5] / sq;
+        const double det = E * G - Fm * Fm;  // 1 + q1^2 + q2^2 > 0
+        const double B00 = G / det, B01 = -Fm / det, B10 = -Fm / det, B11 = E / det;
+        const double a00 = -(e * B00 + fm * B10);
+        const double a01 = -(fm * B00 + g * B10);
+        const double a11 = -(fm * B01 + g * B11);
+        // vtkMath::JacobiN reads the upper triangle: k1^2 + k2^2 is the
+        // squared Frobenius norm of the upper-symmetrised matrix
+        ind = sqrt(a00 * a00 + a11 * a11 + 2.0 * a01 * a01);
+      }
+    }
+  }
+  out[v] = ind;
+}
+
+hipError_t launch_curvature(hipStream_t s, int64_t off, int64_t n, const double* pos, const int32_t* tri,
+                            const int32_t* row_ptr, const int32_t* col, const int32_t* vfptr, const int32_t* vfcol,
+                            double* out) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_curvature, dim3(grid_for(n)), dim3(kBlock), 0, s, off, n, pos, tri, row_ptr, col, vfptr,
+                     vfcol, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_item_build(hipStream_t s, const SweepArgs& a, const double* pos, const int32_t* tri,
                              const int32_t* vfptr, const int32_t* vfcol, const double* indicator, double* raw_w,
                              float4* xw, float* q) {

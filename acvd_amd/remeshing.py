@@ -122,6 +122,13 @@ class Context:
             raise ValueError("initial clustering length must equal the number of vertices")
         check(N.lib().acvd_set_initial_labels(self._h, mesh_id, _p(lab)))
 
+    def indicator(self, mesh_id=0):
+        """The indicator the last build weighted the mesh with: the one set,
+        else the computed curvature indicator (params curvature_indicator)."""
+        out = np.empty(self._n(mesh_id), np.float64)
+        check(N.lib().acvd_get_indicator(self._h, mesh_id, _p(out)))
+        return out
+
     def build(self):
         check(N.lib().acvd_build(self._h))
 
@@ -303,9 +310,12 @@ class _DiscreteRemeshing:
         return 1  # items are vertices
 
     # ---- run ----
-    def ProcessClustering(self):
+    def ProcessClustering(self, curvature_indicator=0):
         """vtkUniformClustering::ProcessClustering; returns the labels (0..K, K = null).
-        Returns None when NumberOfClusters is zero, like the reference (:659-664)."""
+        Returns None when NumberOfClusters is zero, like the reference (:659-664).
+        curvature_indicator = 1 (set by Remesh) weights the items with the
+        reference's curvature indicator when the gradation is positive and no
+        indicator was set."""
         if self._params["number_of_clusters"] <= 0:
             return None
         if self._input is None:
@@ -314,7 +324,7 @@ class _DiscreteRemeshing:
             self._ctx = Context()
         ctx = self._ctx
         ctx.set_input(self._input.points, self._input.triangles)
-        ctx.set_params(0, **self._params)
+        ctx.set_params(0, curvature_indicator=int(curvature_indicator), **self._params)
         ctx.set_indicator(0, self._indicator)
         ctx.set_initial_labels(0, self._initial)
         ctx.process()
@@ -326,14 +336,17 @@ class _DiscreteRemeshing:
 
     def Remesh(self):
         """vtkDiscreteRemeshing::Remesh (vtkDiscreteRemeshing.h:877-954):
-        ProcessClustering, then the output triangulation.  Subdivision of
-        meshes with fewer than SubsamplingThreshold * K vertices, curvature
-        estimation, ForceManifold and boundary fixing are not part of this path."""
+        SamplingPreProcessing (with a positive gradation and no indicator set,
+        the items are weighted by the reference's curvature indicator, computed
+        on the device, :583-653), ProcessClustering, then the output
+        triangulation.  Subdivision of meshes with fewer than
+        SubsamplingThreshold * K vertices, ForceManifold and boundary fixing
+        are not part of this path."""
         n = self.GetNumberOfItems()
         if n < self._subsampling_threshold * self._params["number_of_clusters"]:
             raise AcvdError(f"input has {n} vertices, fewer than SubsamplingThreshold * NumberOfClusters; "
                             "subdivide the input first")
-        if self.ProcessClustering() is None:
+        if self.ProcessClustering(curvature_indicator=1) is None:
             return
         xyz, tri = self._ctx.output(0, 0)
         self._output = Mesh(xyz, tri)

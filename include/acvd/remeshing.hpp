@@ -87,8 +87,10 @@ class DiscreteRemeshing {
   const std::string& GetLastError() const { return error_; }
 
   // ProcessClustering (vtkUniformClustering.h:654-714).  Returns the labels
-  // (borrowed) or nullptr when K == 0 or on error.
-  const Labels* ProcessClustering() {
+  // (borrowed) or nullptr when K == 0 or on error.  curvature_indicator = 1
+  // (set by Remesh) weights the items with the reference's curvature
+  // indicator when the gradation is positive and no indicator was set.
+  const Labels* ProcessClustering(int curvature_indicator = 0) {
     if (params_.number_of_clusters <= 0) return fail("NumberOfClusters is 0"), nullptr;
     if (!input_) return fail("no input"), nullptr;
     if (!ensure_ctx()) return nullptr;
@@ -96,6 +98,7 @@ class DiscreteRemeshing {
     if (!ok(acvd_set_input_f64(ctx_, nv, input_->points.data(), nf, input_->triangles.data()))) return nullptr;
     acvd_params p = params_;
     p.metric = METRIC;
+    p.curvature_indicator = curvature_indicator;
     if (!ok(acvd_set_params(ctx_, 0, &p))) return nullptr;
     if (indicator_ && !ok(acvd_set_indicator(ctx_, 0, indicator_->data()))) return nullptr;
     if (initial_ && !ok(acvd_set_initial_labels(ctx_, 0, initial_->data()))) return nullptr;
@@ -107,12 +110,14 @@ class DiscreteRemeshing {
   }
   const Labels* GetClustering() const { return clustered_ ? &clustering_ : nullptr; }
 
-  // Remesh (vtkDiscreteRemeshing.h:877-954): clustering, then the output mesh.
-  // Returns 1 on success, 0 on failure.
+  // Remesh (vtkDiscreteRemeshing.h:877-954): SamplingPreProcessing (the
+  // curvature indicator when the gradation is positive and none was set,
+  // :583-653), clustering, then the output mesh.  Returns 1 on success, 0 on
+  // failure.
   int Remesh() {
     if (input_ && GetNumberOfItems() < int64_t(subsampling_threshold_) * params_.number_of_clusters)
       return fail("input has fewer than SubsamplingThreshold * NumberOfClusters vertices; subdivide it first"), 0;
-    if (!ProcessClustering()) return 0;
+    if (!ProcessClustering(1)) return 0;
     return BuildOutput(0);
   }
   // Output mesh with vertices moved by the quadric placement of

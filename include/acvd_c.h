@@ -53,6 +53,12 @@ typedef struct acvd_ctx acvd_ctx;
  *   initial_sampling            <- InitSamples type (vtkUniformClustering.h:1123-1175): 0 = the reference's
  *                                  default region growing (type 1, :1177-1316, sequential, on the host),
  *                                  1 = weighted random seeds grown on the device (no counterpart; faster)
+ *   curvature_indicator         <- vtkDiscreteRemeshing::SamplingPreProcessing (vtkDiscreteRemeshing.h:583-653):
+ *                                  1 = when gradation > 0 and no indicator is set, the build computes the
+ *                                  reference's curvature indicator (vtkCurvatureMeasure, polynomial fit on
+ *                                  the faces within two edges of each vertex, vtkCurvatureMeasure.cxx:188-509)
+ *                                  on the device and weights the items with it; Remesh() of the class API
+ *                                  sets it, as the reference's Remesh runs SamplingPreProcessing
  */
 typedef struct acvd_params {
   int32_t number_of_clusters;
@@ -64,7 +70,7 @@ typedef struct acvd_params {
   int32_t console_output;
   uint32_t seed;
   int32_t initial_sampling;
-  int32_t reserved;
+  int32_t curvature_indicator;
   double gradation;
 } acvd_params;
 
@@ -129,6 +135,12 @@ int acvd_get_params(const acvd_ctx* ctx, int32_t mesh_id, acvd_params* p);
  * vtkIsotropicMetricForClustering.h:66-72): per-vertex indicator used as
  * weight = area * indicator^gradation (:329-336 / :256-259).  NULL clears. */
 int acvd_set_indicator(acvd_ctx* ctx, int32_t mesh_id, const double* indicator);
+/* The indicator the last build weighted mesh `mesh_id` with: the one set by
+ * acvd_set_indicator, else the curvature indicator computed by the build
+ * (params.curvature_indicator; the CurvatureCollection of
+ * vtkCurvatureMeasure::GetCurvatureIndicator, vtkCurvatureMeasure.cxx:721-760).
+ * out[nv].  ACVD_ERR_STATE when there is neither. */
+int acvd_get_indicator(acvd_ctx* ctx, int32_t mesh_id, double* out);
 /* SetInitialClustering(vtkIntArray*) (vtkUniformClustering.h:336-342,
  * 1166-1169): labels in 0..K (K = null), one per vertex.  NULL clears (the
  * library then seeds its own initial clustering on the device). */

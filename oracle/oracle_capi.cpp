@@ -7,6 +7,7 @@
 //   B  threaded restatement of vtkThreadedClustering (threaded.h)
 //   C  replay of the GPU schedule (replay.h)
 //   E  FP64 unshifted energy of a labelling (energy below)
+//   K  the reference's per-vertex curvature indicator (curvature.h)
 // plus the reference's quadric tools and ring check for unit tests, and a
 // host build of the device sweep inputs (devinputs.h).
 
@@ -18,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "curvature.h"
 #include "devinputs.h"
 #include "energy_ref.h"
 #include "metrics.h"
@@ -152,6 +154,22 @@ void orc_mesh_edges(void* mh, int32_t* ev1, int32_t* ev2) {
   RefMesh* m = static_cast<RefMesh*>(mh);
   std::memcpy(ev1, m->ev1.data(), m->ev1.size() * sizeof(int32_t));
   std::memcpy(ev2, m->ev2.data(), m->ev2.size() * sizeof(int32_t));
+}
+
+// K: curvature indicator of every vertex (curvature.h).
+void orc_curvature_indicator(void* mh, double* out) {
+  std::vector<double> k;
+  curvature_indicator(*static_cast<RefMesh*>(mh), k);
+  std::memcpy(out, k.data(), k.size() * sizeof(double));
+}
+
+// Face list of vertex v for the curvature fit; returns its length (at most
+// cap entries are written).
+int64_t orc_curvature_face_list(void* mh, int64_t v, int32_t* out, int64_t cap) {
+  std::vector<int32_t> fl;
+  curvature_face_list(*static_cast<RefMesh*>(mh), v, fl);
+  for (int64_t k = 0; k < int64_t(fl.size()) && k < cap; ++k) out[k] = fl[k];
+  return int64_t(fl.size());
 }
 
 void orc_mesh_vertex_areas(void* mh, double* out) {

@@ -8,6 +8,7 @@ Oracles (all restate the reference in C++, see the headers for file:line):
   threaded_run  B: vtkThreadedClustering (ACVDP / ACVDQP), the timed CPU baseline
   replay_run    C: the GPU pass schedule replayed on the host (bit-exact target)
   energy        FP64 unshifted energy of a labelling
+  RefMesh.curvature_indicator  the reference's per-vertex curvature indicator
 """
 import ctypes
 import os
@@ -46,6 +47,9 @@ def lib():
         L.orc_mesh_num_edges.argtypes = [P]
         L.orc_mesh_edges.argtypes = [P, P, P]
         L.orc_mesh_vertex_areas.argtypes = [P, P]
+        L.orc_curvature_indicator.argtypes = [P, P]
+        L.orc_curvature_face_list.restype = ctypes.c_int64
+        L.orc_curvature_face_list.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
         L.orc_build_items.argtypes = [P, ctypes.c_int, P, ctypes.c_double, P, P]
         L.orc_serial_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int, ctypes.c_int,
                                      P, ctypes.c_int, P, P]
@@ -109,6 +113,18 @@ class RefMesh:
         b = np.empty(ne, np.int32)
         lib().orc_mesh_edges(self.h, _p(a), _p(b))
         return a, b
+
+    def curvature_indicator(self):
+        """vtkCurvatureMeasure's per-vertex indicator (oracle/curvature.h)"""
+        out = np.empty(self.nv, np.float64)
+        lib().orc_curvature_indicator(self.h, _p(out))
+        return out
+
+    def curvature_face_list(self, v):
+        cap = 4096
+        out = np.empty(cap, np.int32)
+        n = lib().orc_curvature_face_list(self.h, int(v), _p(out), cap)
+        return out[:min(n, cap)].copy()
 
     def vertex_areas(self):
         out = np.empty(self.nv, np.float64)
