@@ -94,14 +94,16 @@ def test_bad_input_is_an_error(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("qem", [0, 1])
-def test_cli_output_equals_the_python_api(tmp_path, qem):
+@pytest.mark.parametrize("qem,gradation", [(0, 0.0), (1, 0.0), (1, 1.5)])
+def test_cli_output_equals_the_python_api(tmp_path, qem, gradation):
     import acvd_amd
     xyz, tri = meshgen.icosphere(4, 1, 0.02)
     K = 200
     write_obj(tmp_path / "in.obj", xyz, tri)
-    print(f"acvd_run qem={qem} K={K}", flush=True)
-    r = run_cli(tmp_path / "in.obj", K, 0, "-qem", qem, "-of", "out.ply", cwd=tmp_path)
+    print(f"acvd_run qem={qem} K={K} gradation={gradation}", flush=True)
+    # with a positive gradation both sides weight the items by the curvature
+    # indicator that Remesh computes on the device
+    r = run_cli(tmp_path / "in.obj", K, gradation, "-qem", qem, "-of", "out.ply", cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     cx, ct = read_ply_ascii(tmp_path / "out.ply")
     cls = acvd_amd.QIsotropicDiscreteRemeshing if qem else acvd_amd.IsotropicDiscreteRemeshing
@@ -109,7 +111,7 @@ def test_cli_output_equals_the_python_api(tmp_path, qem):
     rem.SetInput((xyz, tri))
     rem.SetNumberOfClusters(K)
     rem.SetSubsamplingThreshold(10)
-    rem.GetMetric().SetGradation(0.0)
+    rem.GetMetric().SetGradation(gradation)
     if qem:
         rem.GetMetric().SetQuadricsOptimizationLevel(3)
         rem.SetUnconstrainedInitialization(1)
