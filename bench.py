@@ -10,8 +10,9 @@ A step is one full clustering convergence (MinimizeEnergy: all passes, cleans
 and fills until every stopping rule is met) from the same initial labelling
 (the reference's region growing), with the mesh, CSR and items resident in
 HBM.  `value` is
-the median device time of a step in ms (HIP events inside the library);
-`ms_per_step` the wall time per step around the call.
+the median device time of the timed steps in ms (HIP events inside the
+library; 5 steps after 1 warm-up by default); `ms_per_step` the wall time per
+step around the call.
 
 Extra legs:
   roofline     the dominant sweep kernel's algorithmic bytes per launch over its
@@ -389,7 +390,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM byte counters)")
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", choices=["full", "half"], default="full",
