@@ -21,7 +21,7 @@ constexpr int kBlock = 256;
 constexpr int kSlices = 64;
 // Work-list slices: scan tile t appends to slice t % kWorkSlices, so that a
 // slice counter sees few same-address atomics per pass.
-constexpr int kWorkSlices = 4096;
+constexpr int kWorkSlices = 2048;
 // Upper bound of the block-owned segments (= evaluate / apply grid).
 constexpr int kMaxSeg = 2048;
 // Candidate lists of at most this many proposals (last pass) are walked one
