@@ -34,6 +34,7 @@ constexpr int kSegWalkMax = 16384;
 // skipped after reading that set.
 constexpr int kTileItems = 256;
 constexpr int kTileSet = 64;  // set capacity; a larger set marks the tile always scanned
+constexpr int kTilesPerScanBlock = kBlock / 64;  // k_scan: one wavefront per tile
 // The sets are kept up to date (k_tile_update after each pass) only once a
 // pass moves at most K / kTileOnDivisor items: before that nearly every tile
 // holds a changed cluster, and every tile is marked always scanned instead.

@@ -193,9 +193,12 @@ __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t la
 // cluster stamps (and the tile's last proposal loop); only an active tile
 // reads its items and CSR rows.  The test is conservative: every item
 // needs_eval would select lies in an active tile.
-__global__ __launch_bounds__(64) void k_scan(SweepArgs a) {
-  const int32_t t = blockIdx.x;
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
+  // one wavefront per tile, kTilesPerScanBlock tiles per block (the waves
+  // are independent: no block-level synchronisation)
+  const int32_t t = int32_t(blockIdx.x) * kTilesPerScanBlock + int32_t(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= a.n_tiles) return;  // wave-uniform
   const int m = a.item_mesh ? a.tile_mesh[t] : 0;
   const MeshState& st = a.state[m];
   if (st.phase != kPhaseSweep) return;
@@ -1299,7 +1302,8 @@ hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* me
 
 hipError_t launch_scan(hipStream_t s, const SweepArgs& a) {
   if (a.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan, dim3(unsigned(a.n_tiles)), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_scan, dim3(unsigned((a.n_tiles + kTilesPerScanBlock - 1) / kTilesPerScanBlock)), dim3(kBlock),
+                     0, s, a);
   return hipGetLastError();
 }
 
