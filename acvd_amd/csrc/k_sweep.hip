@@ -742,7 +742,11 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
 template <int METRIC, class Reg>
 __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemPool* pool, bool valid, int64_t e,
                                     int round, Reg&& reg) {
+  // the candidate record in one round trip
   const int32_t b = valid ? a.cand_tgt[e] : -1;
+  const int32_t i_e = valid ? a.cand[e] : 0;
+  const int32_t lab_e = valid ? a.cand_lab[e] : kNull;
+  const int32_t since = valid ? a.cand_since[e] : 0;
   bool live = false, stale = false, use_pool = false;
   int32_t i = 0, lab = kNull, target = b, m = 0;
   unsigned long long key = 0;
@@ -751,14 +755,14 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
   uint32_t open = 0;
   int kind = kPoolNone, ntask = 0;
   if (b >= 0) {
-    i = a.cand[e];
-    lab = a.cand_lab[e];  // a live candidate has not moved
-    const int32_t since = a.cand_since[e];
+    i = i_e;
+    lab = lab_e;  // a live candidate has not moved
     m = mesh_of_item(a, i);
     const DevMesh& dm = a.meshes[m];
     if (dm.metric == mesh_metric(METRIC)) {
       const MeshState& st = a.state[m];
-      stale = (lab != kNull && a.cl_round[lab] >= since) || a.cl_round[b] >= since;
+      const int32_t ra = lab != kNull ? a.cl_round[lab] : -1, rb = a.cl_round[b];
+      stale = ra >= since || rb >= since;
       key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
       live = true;
       if (stale) {
@@ -867,16 +871,20 @@ __device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool vali
   bool win = false;
   int64_t i = 0;
   int32_t lab = kNull;
+  // the candidate record in one round trip
   const int32_t b = valid ? a.cand_tgt[e] : -1;
+  const int32_t i_e = valid ? a.cand[e] : 0;
+  const int32_t lab_e = valid ? a.cand_lab[e] : kNull;
   if (b >= 0) {
-    i = a.cand[e];
-    lab = a.cand_lab[e];
+    i = i_e;
+    lab = lab_e;
     const int m = mesh_of_item(a, i);
     MeshState* st = &a.state[m];
     const int32_t L = st->loops;
     const DevMesh& dm = a.meshes[m];
     const unsigned long long key = accept_key(dm.seed, L, round, uint32_t(i - dm.item_off));
-    win = (lab == kNull || a.prio[lab] == key) && a.prio[b] == key;
+    const unsigned long long pa = lab != kNull ? a.prio[lab] : key, pb = a.prio[b];
+    win = pa == key && pb == key;
     if (win) {
       // exclusive owner of clusters lab and b for this round
       a.labels[i] = b;
