@@ -407,12 +407,24 @@ constexpr int mesh_metric(int kernel_metric) { return kernel_metric == 2 ? 1 : k
 // blocks through the LDS index; a short one by block b over segment b alone,
 // which spares the index (grid = n_seg).  The results do not depend on the
 // walk.
+// Control values of the candidate walk, loaded by a kernel before its first
+// barrier so that they share a memory round trip with its other first loads.
+struct WalkCtl {
+  bool by_seg;    // this pass walks one segment per block
+  int32_t n_own;  // fill count of this block's segment
+};
+__device__ inline WalkCtl load_walk_ctl(const SweepArgs& a) {
+  WalkCtl w;
+  w.by_seg = a.ctrl->seg_walk != 0;
+  w.n_own = a.seg_cand[blockIdx.x];
+  return w;
+}
+
 template <class F>
-__device__ inline void walk_candidates(const SweepArgs& a, SegIndex* si, F&& f) {
-  const bool by_seg = a.ctrl->seg_walk != 0;  // block-uniform
+__device__ inline void walk_candidates(const SweepArgs& a, const WalkCtl& w, SegIndex* si, F&& f) {
   int32_t n, first, stride;
-  if (by_seg) {
-    n = a.seg_cand[blockIdx.x];
+  if (w.by_seg) {  // block-uniform
+    n = w.n_own;
     first = 0;
     stride = kBlock;
   } else {
@@ -425,7 +437,7 @@ __device__ inline void walk_candidates(const SweepArgs& a, SegIndex* si, F&& f) 
   for (int32_t base = first; base < n; base += stride) {
     const int32_t k = base + int32_t(threadIdx.x);
     int64_t e = 0;
-    if (k < n) e = by_seg ? seg + k : seg_pos(si, a.cand_cap, k);
+    if (k < n) e = w.by_seg ? seg + k : seg_pos(si, a.cand_cap, k);
     f(k < n, e);
   }
 }
@@ -802,12 +814,13 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ KeyAgg agg;
   __shared__ SegIndex si;
   __shared__ QemPool pools[METRIC == 1 ? kBlock / 64 : 1];
+  const WalkCtl w = load_walk_ctl(a);
   if (round_was_empty(a, round - 1)) return;
   keyagg_clear(&agg);
   __syncthreads();
   const SweepView v = make_view(a);
   QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
-  walk_candidates(a, &si, [&](bool valid, int64_t e) {
+  walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
     reeval_entry<METRIC>(a, v, pool, valid, e, round,
                          [&](int32_t c, unsigned long long key) { keyagg_min(&agg, c, key); });
     keyagg_flush(&agg, a.prio);
@@ -959,12 +972,14 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ SegIndex si;
   __shared__ DeriveList dlist[kSolve ? kBlock / 64 : 1];
   const int64_t mseg = int64_t(blockIdx.x) * a.cand_cap;
+  const WalkCtl w = load_walk_ctl(a);
+  const int32_t moved0 = round == 0 ? 0 : a.seg_moved[blockIdx.x];
   if (round > 0 && round_was_empty(a, round - 1)) return;
-  if (threadIdx.x == 0) s_moved = round == 0 ? 0 : a.seg_moved[blockIdx.x];
+  if (threadIdx.x == 0) s_moved = moved0;
   __syncthreads();
   DeriveList* dl = &dlist[kSolve ? threadIdx.x >> 6 : 0];
   int mods = 0;
-  walk_candidates(a, &si, [&](bool valid, int64_t e) {
+  walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
     int32_t i;
     const bool win = apply_entry<kSolve>(a, dl, valid, e, round, &i);
     mods += win;
