@@ -113,3 +113,29 @@ def test_remesh_weights_by_curvature(gpu_ready):
         rem2.GetContext().indicator(0)
     rem.Delete()
     rem2.Delete()
+
+
+def test_batch_computes_the_indicator_per_mesh(gpu_ready):
+    # mesh 0 asks for the curvature indicator, mesh 1 has a set one, mesh 2 none
+    m0 = meshgen.torus(96, 48, 5)
+    m1 = meshgen.icosphere(4, 2, 0.05)
+    m2 = meshgen.icosphere(4, 3, 0.05)
+    ctx = acvd_amd.Context()
+    ids = [ctx.add_input(*m) for m in (m0, m1, m2)]
+    ctx.set_params(ids[0], number_of_clusters=200, metric=QEM, gradation=1.0, curvature_indicator=1)
+    ctx.set_params(ids[1], number_of_clusters=150, metric=ISO, gradation=1.0, curvature_indicator=1)
+    ctx.set_params(ids[2], number_of_clusters=150, metric=ISO, gradation=0.0, curvature_indicator=1)
+    mine = np.linspace(0.5, 1.5, len(m1[0]))
+    ctx.set_indicator(ids[1], mine)
+    ctx.build()
+    ref0 = po.RefMesh(np.asarray(m0[0], np.float64), m0[1]).curvature_indicator()
+    np.testing.assert_allclose(ctx.indicator(ids[0]), ref0, rtol=1e-7, atol=1e-12)
+    np.testing.assert_array_equal(ctx.indicator(ids[1]), mine)
+    with pytest.raises(acvd_amd.AcvdError):
+        ctx.indicator(ids[2])
+    # each mesh's weights are those of its own inputs
+    for mid, (xyz, tri), metric, ind, grad in [(ids[0], m0, QEM, ref0, 1.0), (ids[1], m1, ISO, mine, 1.0),
+                                                (ids[2], m2, ISO, None, 0.0)]:
+        _, _, hxw, _, _ = po.device_inputs(np.asarray(xyz, np.float64), tri, metric, ind, grad)
+        xw, _ = ctx.items(mid)
+        np.testing.assert_allclose(xw[:, 3], hxw[:, 3], rtol=1e-6)
