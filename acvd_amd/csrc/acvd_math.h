@@ -676,17 +676,23 @@ ACVD_HD int32_t ring_label(const int32_t* nb, int p) {
 ACVD_HD bool qem_screen_short_row(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained,
                                   int32_t* nb, QemItem* s, uint32_t* open_out) {
   const int32_t r0 = v.row_ptr[i], deg = v.row_ptr[i + 1] - r0;
+  // the words of the item and of its cluster, read with the row bounds (they
+  // do not depend on the row), so that the guards below wait on one round trip
+  const int32_t size_a = v.sizes[lab];
+  const unsigned long long adj = v.ring_adj ? v.ring_adj[i] : 0ull;
+  const float w_i = v.xw[4 * i + 3];
+  const ClusterGeom ga = v.geom[lab];
   ACVD_UNROLL
   for (int p = 0; p < kRingCache; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
   *open_out = 0;
-  if (v.sizes[lab] <= 1) return false;
+  if (size_a <= 1) return false;
   if (connexity) {
     const bool split = (v.ring_adj && deg <= kRingAdjMax)
-                           ? ring_problem_mask(v.ring_adj[i], nb, deg, lab)
+                           ? ring_problem_mask(adj, nb, deg, lab)
                            : ring_problem(v.row_ptr, v.col, v.labels, int32_t(i), lab);
     if (split) return false;
   }
-  if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
+  if (!can_leave(w_i, ga)) return false;
   qem_item_terms(v, i, lab, constrained, s);
   uint32_t open = 0;
   ACVD_UNROLL
@@ -745,6 +751,9 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
   const float px = v.xw[4 * i], py = v.xw[4 * i + 1], pz = v.xw[4 * i + 2], pw = v.xw[4 * i + 3];
   const int32_t r0 = v.row_ptr[i], r1 = v.row_ptr[i + 1];
   int32_t target = kNull;
+  // the item's cluster words, read with the item (independent of the row)
+  const int32_t size_a = lab != kNull ? v.sizes[lab] : 0;
+  const ClusterGeom ga = lab != kNull ? v.geom[lab] : ClusterGeom{0.0, 0.0, 0.0, 0.0};
   if (lab == kNull) {
     float best = 3.0e38f;
     for (int32_t k = r0; k < r1; ++k) {
@@ -775,9 +784,8 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
         v, lab, s, constrained, Ta_new,
         [&](int32_t, int32_t b) { return constrained ? qem_join_T(v, i, b, sc, level) : 0.0; }, target_out);
   }
-  if (v.sizes[lab] <= 1) return false;
+  if (size_a <= 1) return false;
   if (connexity && ring_check(v, i, lab)) return false;
-  const ClusterGeom ga = v.geom[lab];
   if (!can_leave(pw, ga)) return false;
   const float ta = leave_term(px, py, pz, pw, ga);
   float best = 3.0e38f;
