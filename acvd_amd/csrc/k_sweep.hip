@@ -815,9 +815,8 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ SegIndex si;
   __shared__ QemPool pools[METRIC == 1 ? kBlock / 64 : 1];
   const WalkCtl w = load_walk_ctl(a);
+  keyagg_clear(&agg);  // made visible by the barrier in round_was_empty
   if (round_was_empty(a, round - 1)) return;
-  keyagg_clear(&agg);
-  __syncthreads();
   const SweepView v = make_view(a);
   QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
   walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
