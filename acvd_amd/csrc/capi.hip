@@ -4,8 +4,9 @@
 // Host work per ProcessClustering is limited to launching kernels and, between
 // chunks of passes, reading back a small control block (and the per-mesh
 // states when a mesh reached a convergence).  The chunk of passes
-// (propose -> apply -> controller) x kChunk is captured once into a hipGraph
-// and replayed; all pass-dependent values live in device memory.
+// (propose -> apply -> controller) x kChunk (kShortChunk near the end of a
+// phase) is captured once into a hipGraph and replayed; all pass-dependent
+// values live in device memory.
 
 #include <hip/hip_runtime.h>
 
@@ -80,6 +81,7 @@ struct HostMesh {
 };
 
 constexpr int kChunk = 16;        // passes per graph launch
+constexpr int kShortChunk = 4;    // passes per graph launch near the end of a phase
 static_assert(kPassLogCap > kChunk, "the pass log must hold a chunk of passes");
 constexpr int kReduceItems = 16384;  // items per block of the deterministic reductions
 
@@ -121,12 +123,10 @@ struct acvd_ctx {
   std::vector<DevMesh> h_meshes;
   void* cub_tmp = nullptr;
   size_t cub_bytes = 0;
-  // captured pass chunks: [0] some QEM mesh constrained (QEM kernels), [1] no
-  // QEM mesh constrained (their isotropic-rule kernels)
-  // pass-chunk graphs by (QEM meshes unconstrained, tile sets maintained):
-  // index qem_unc + 2 * tiles
-  hipGraphExec_t chunk_exec[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipGraph_t chunk_graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  // pass-chunk graphs by (QEM meshes unconstrained, tile sets maintained,
+  // short chunk): index qem_unc + 2 * tiles + 4 * short
+  hipGraphExec_t chunk_exec[8] = {};
+  hipGraph_t chunk_graph[8] = {};
   double build_ms = 0.0, converge_ms = 0.0;
   Ctrl last_ctrl{};
   bool timing = false;
@@ -191,7 +191,7 @@ struct acvd_ctx {
   }
 
   void drop_graphs() {
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 8; ++k) {
       if (chunk_exec[k]) (void)hipGraphExecDestroy(chunk_exec[k]);
       if (chunk_graph[k]) (void)hipGraphDestroy(chunk_graph[k]);
       chunk_exec[k] = nullptr;
@@ -565,10 +565,10 @@ int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsign
   return ACVD_OK;
 }
 
-int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles) {
+int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
-  for (int p = 0; p < kChunk; ++p) {
+  for (int p = 0; p < n_pass; ++p) {
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(launch_apply(s, a, 0, !qem_unc));
@@ -583,12 +583,12 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles) {
 }
 
 // Passes launched one by one with events around each kernel.
-int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles) {
+int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
-  std::vector<hipEvent_t> ev(5 * kChunk);
+  std::vector<hipEvent_t> ev(5 * n_pass);
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-  for (int p = 0; p < kChunk; ++p) {
+  for (int p = 0; p < n_pass; ++p) {
     HIP_TRY(hipEventRecord(ev[5 * p], s));
     HIP_TRY(launch_scan(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
@@ -605,7 +605,7 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles) {
     HIP_TRY(hipEventRecord(ev[5 * p + 4], s));
   }
   HIP_TRY(hipStreamSynchronize(s));
-  for (int p = 0; p < kChunk; ++p)
+  for (int p = 0; p < n_pass; ++p)
     for (int k = 0; k < 4; ++k) {
       float ms = 0;
       HIP_TRY(hipEventElapsedTime(&ms, ev[5 * p + k], ev[5 * p + k + 1]));
@@ -616,14 +616,15 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles) {
   return ACVD_OK;
 }
 
-int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles) {
+int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles, bool short_chunk) {
   hipStream_t s = c->stream;
-  if (c->timing) return run_chunk_timed(c, qem_unc, tiles);
+  const int n_pass = short_chunk ? kShortChunk : kChunk;
+  if (c->timing) return run_chunk_timed(c, qem_unc, tiles, n_pass);
   if (use_graph) {
-    const int k = int(qem_unc) + 2 * int(tiles);
+    const int k = int(qem_unc) + 2 * int(tiles) + 4 * int(short_chunk);
     if (!c->chunk_exec[k]) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_chunk(c, qem_unc, tiles);
+      int rc = enqueue_chunk(c, qem_unc, tiles, n_pass);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(s, &g);
       if (rc != ACVD_OK) return rc;
@@ -634,7 +635,7 @@ int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles) {
     HIP_TRY(hipGraphLaunch(c->chunk_exec[k], s));
     return ACVD_OK;
   }
-  return enqueue_chunk(c, qem_unc, tiles);
+  return enqueue_chunk(c, qem_unc, tiles, n_pass);
 }
 
 // True when no QEM mesh of the context is constrained (the state flips only
@@ -709,8 +710,20 @@ int minimize_energy(acvd_ctx* c) {
   bool loop_output = false;
   for (int m = 0; m < M; ++m) loop_output |= c->meshes[m].params.console_output > 1;
   unsigned long long printed = 0;
+  // early-stop threshold of the proposals summed over the sweeping meshes
+  // that can still stop early (controller_step); set again after each clean
+  auto early_stop_props = [&]() {
+    int64_t t = 0;
+    for (int m = 0; m < M; ++m) {
+      const MeshState& st = c->h_state[m];
+      if (st.phase == kPhaseSweep && st.convergences <= 1) t += st.early_stop_items / kEarlyStopDivisor;
+    }
+    return t;
+  };
+  int64_t stop_props = early_stop_props();
+  bool near_end = false;  // short chunks: the phase is about to end
   while (true) {
-    int rc2 = run_chunk(c, use_graph, qem_unconstrained(c), tiles);
+    int rc2 = run_chunk(c, use_graph, qem_unconstrained(c), tiles, near_end);
     if (rc2) return rc2;
     HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -741,6 +754,8 @@ int minimize_energy(acvd_ctx* c) {
       if (rc) return rc;
       // a phase starts with many moves per pass: tile sets off again
       tiles = false;
+      near_end = false;
+      stop_props = early_stop_props();
       HIP_TRY(launch_fill_i32(s, c->d_tile_cnt.p, c->n_tiles, kTileSet + 1));
       HIP_TRY(hipMemcpyAsync(c->d_state.p, c->h_state, M * sizeof(MeshState), hipMemcpyHostToDevice, s));
       HIP_TRY(hipStreamSynchronize(s));
@@ -750,6 +765,11 @@ int minimize_energy(acvd_ctx* c) {
       continue;
     }
     if (!c->h_ctrl->any_sweep) break;
+    // the last pass proposed at most 3/2 of the early-stop threshold: the
+    // phase ends within a few passes, and short chunks keep the passes run
+    // after its end (which change nothing) few
+    if (c->h_ctrl->passes > 0 && stop_props > 0)
+      near_end = 2 * int64_t(c->h_ctrl->pass_log[(c->h_ctrl->passes - 1) % kPassLogCap][2]) <= 3 * stop_props;
     // tile sets on once the last pass moved at most K / kTileOnDivisor items:
     // most tiles then hold no changed cluster and the scan skips them
     if (!tiles && c->h_ctrl->passes > 0 &&
