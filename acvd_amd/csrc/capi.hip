@@ -247,7 +247,7 @@ bool morton_tiles() {
 }
 
 // eval grid (blocks of kBlock threads, grid-stride over the work list)
-int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxSeg)); }
+int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kSweepSegs)); }
 
 // ------------------------------------------------------------------- build
 int build(acvd_ctx* c) {
