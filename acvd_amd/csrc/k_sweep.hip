@@ -1001,10 +1001,23 @@ __global__ __launch_bounds__(kBlock) void k_tile_update(SweepArgs a) {
     const int64_t i = a.moved[seg + k];
     const int32_t b = a.labels[i];
     const int32_t ti = a.item_tile[i];
+    const int32_t r0 = a.row_ptr[i], r1 = a.row_ptr[i + 1];
+    // the tiles of the first kRingCache neighbours read at once; each distinct
+    // tile other than the item's gets one insert
+    int32_t tj[kRingCache];
+    ACVD_UNROLL
+    for (int p = 0; p < kRingCache; ++p) tj[p] = r0 + p < r1 ? a.item_tile[a.col[r0 + p]] : ti;
     tile_insert(a, ti, b);
-    for (int32_t r = a.row_ptr[i]; r < a.row_ptr[i + 1]; ++r) {
-      const int32_t tj = a.item_tile[a.col[r]];
-      if (tj != ti) tile_insert(a, tj, b);
+    ACVD_UNROLL
+    for (int p = 0; p < kRingCache; ++p) {
+      bool first = tj[p] != ti;
+      ACVD_UNROLL
+      for (int q = 0; q < p; ++q) first = first && tj[q] != tj[p];
+      if (first) tile_insert(a, tj[p], b);
+    }
+    for (int32_t r = r0 + kRingCache; r < r1; ++r) {
+      const int32_t t = a.item_tile[a.col[r]];
+      if (t != ti) tile_insert(a, t, b);
     }
   }
 }
