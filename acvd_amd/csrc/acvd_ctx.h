@@ -167,7 +167,8 @@ hipError_t launch_zero_clusters(hipStream_t s, const SweepArgs& a, const int32_t
 hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask,
                              unsigned long long* mesh_count);
 hipError_t launch_derive(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask, int set_stamp);
-hipError_t launch_scan(hipStream_t s, const SweepArgs& a);
+// tiles: the tile sets are kept (else every tile is marked always scanned).
+hipError_t launch_scan(hipStream_t s, const SweepArgs& a, bool tiles);
 hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask);
 hipError_t launch_tile_order(hipStream_t s, const SweepArgs& a, const float* bbox, int32_t* tile_items,
                              int32_t* item_tile, void** tmp, size_t* tmp_bytes);

@@ -569,7 +569,7 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   for (int p = 0; p < n_pass; ++p) {
-    HIP_TRY(launch_scan(s, a));
+    HIP_TRY(launch_scan(s, a, tiles));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(launch_apply(s, a, 0, !qem_unc));
     for (int r = 1; r < kMatchRounds; ++r) {
@@ -590,7 +590,7 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   for (int p = 0; p < n_pass; ++p) {
     HIP_TRY(hipEventRecord(ev[5 * p], s));
-    HIP_TRY(launch_scan(s, a));
+    HIP_TRY(launch_scan(s, a, tiles));
     HIP_TRY(hipEventRecord(ev[5 * p + 1], s));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));

@@ -193,6 +193,10 @@ __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t la
 // cluster stamps (and the tile's last proposal loop); only an active tile
 // reads its items and CSR rows.  The test is conservative: every item
 // needs_eval would select lies in an active tile.
+// kTiles: the tile sets are kept (else every tile is marked always scanned
+// and the kernel does not read the tile words).
+static_assert(kTileSet == 64, "k_scan: one set entry per lane");
+template <bool kTiles>
 __global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
   // one wavefront per tile, kTilesPerScanBlock tiles per block (the waves
   // are independent: no block-level synchronisation)
@@ -200,17 +204,18 @@ __global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
   const int lane = threadIdx.x & 63;
   if (t >= a.n_tiles) return;  // wave-uniform
   const int m = a.item_mesh ? a.tile_mesh[t] : 0;
+  // the tile's count, last proposal and set entry of this lane (the set holds
+  // kTileSet = 64 entries; entries past cnt are ignored) read with the state
+  const int32_t cnt = kTiles ? a.tile_cnt[t] : kTileSet + 1;
+  const int32_t prop = kTiles ? a.tile_prop[t] : 0;
+  const int32_t c_lane = kTiles ? a.tile_set[int64_t(t) * kTileSet + lane] : 0;
   const MeshState& st = a.state[m];
   if (st.phase != kPhaseSweep) return;
   const int32_t L = st.loops;
-  const int32_t cnt = a.tile_cnt[t];
-  bool active = cnt > kTileSet || a.tile_prop[t] == L - 1;
+  bool active = cnt > kTileSet || prop == L - 1;
   if (!active) {  // wave-uniform branch
     bool hit = false;
-    if (lane < cnt) {
-      const int32_t c = a.tile_set[int64_t(t) * kTileSet + lane];
-      hit = a.stamp[c] + 1 >= L;
-    }
+    if (lane < cnt) hit = a.stamp[c_lane] + 1 >= L;
     active = __ballot(hit) != 0ull;
   }
   if (!active) return;
@@ -830,18 +835,20 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
 // Adds cluster c to the set of tile t unless present (sets only grow between
 // rebuilds; duplicates from concurrent inserts only use capacity).
 __device__ inline void tile_insert(const SweepArgs& a, int32_t t, int32_t c) {
+  // the count and all set entries read at once (16-byte loads; the set holds
+  // kTileSet entries), entries past cnt ignored
   const int32_t cnt = a.tile_cnt[t];
-  if (cnt > kTileSet) return;
-  // all set entries read at once (16-byte loads), entries past cnt ignored
   const int4* set4 = reinterpret_cast<const int4*>(a.tile_set + int64_t(t) * kTileSet);
+  int4 sv[kTileSet / 4];
+#pragma unroll
+  for (int q = 0; q < kTileSet / 4; ++q) sv[q] = set4[q];
+  if (cnt > kTileSet) return;
   bool found = false;
 #pragma unroll
   for (int q = 0; q < kTileSet / 4; ++q) {
-    if (4 * q < cnt) {
-      const int4 v = set4[q];
-      found |= (v.x == c) || (4 * q + 1 < cnt && v.y == c) || (4 * q + 2 < cnt && v.z == c) ||
-               (4 * q + 3 < cnt && v.w == c);
-    }
+    const int4 v = sv[q];
+    found |= (4 * q < cnt && v.x == c) || (4 * q + 1 < cnt && v.y == c) || (4 * q + 2 < cnt && v.z == c) ||
+             (4 * q + 3 < cnt && v.w == c);
   }
   if (found) return;
   const int32_t slot = atomicAdd(&a.tile_cnt[t], 1);
@@ -1335,10 +1342,11 @@ hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* me
   return hipGetLastError();
 }
 
-hipError_t launch_scan(hipStream_t s, const SweepArgs& a) {
+hipError_t launch_scan(hipStream_t s, const SweepArgs& a, bool tiles) {
   if (a.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scan, dim3(unsigned((a.n_tiles + kTilesPerScanBlock - 1) / kTilesPerScanBlock)), dim3(kBlock),
-                     0, s, a);
+  const dim3 grid(unsigned((a.n_tiles + kTilesPerScanBlock - 1) / kTilesPerScanBlock));
+  if (tiles) hipLaunchKernelGGL(k_scan<true>, grid, dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL(k_scan<false>, grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
