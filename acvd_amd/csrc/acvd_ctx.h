@@ -24,12 +24,13 @@ constexpr int kSlices = 64;
 constexpr int kWorkSlices = 2048;
 // Upper bound of the block-owned segments (= evaluate / apply grid).
 constexpr int kMaxSeg = 2048;
-// Segments (evaluate / accept-round grid) of a large context: 5 blocks per CU
-// of the 256.  The round kernels fit 4 to 8 blocks per CU (LDS and VGPRs), so
-// a larger grid runs their dependent loads in a second wave of blocks.
+// Segments (evaluate / accept-round grid) of a large context: 4.5 blocks per
+// CU of the 256.  The round kernels fit 3 to 8 blocks per CU (VGPRs and LDS),
+// so a larger grid runs their dependent loads in a further wave of blocks.
 // QEM torus of 8.39M vertices, K = 100k, to convergence: 2048 blocks
-// 260.7 ms, 1536 245.0, 1280 243.8, 1024 245.4, 768 245.1, 512 261.1.
-constexpr int kSweepSegs = 1280;
+// 260.7 ms, 1536 245.0, 1280 243.8, 1024 245.4, 768 245.1, 512 261.1; with
+// the later kernels 1408 241.2, 1280 241.2-241.6, 1152 239.5.
+constexpr int kSweepSegs = 1152;
 static_assert(kSweepSegs <= kMaxSeg, "segment index bound");
 // Candidate lists of at most this many proposals (last pass) are walked one
 // segment per block, longer ones evenly through an index of the segments.
