@@ -32,6 +32,7 @@ class Params(ctypes.Structure):
         ("seed", ctypes.c_uint32),
         ("initial_sampling", ctypes.c_int32),
         ("curvature_indicator", ctypes.c_int32),
+        ("first_phase_rounds", ctypes.c_int32),
         ("gradation", ctypes.c_double),
     ]
 

@@ -828,7 +828,23 @@ struct MeshState {
   int64_t proposals;        // candidates proposed (statistics)
   int32_t last_disconnected;
   int32_t pass_props;       // proposals (improving moves found) of the current pass
+  int32_t first_rounds;     // accept rounds per pass until the first convergence (1..kMatchRounds)
 };
+
+// Accept rounds of the mesh's current pass.  Until the first convergence a
+// pass runs first_rounds rounds, afterwards kMatchRounds.  The phase before
+// the first convergence decides which local minimum the run reaches: with
+// one round per pass every cluster moves at most one item per pass and the
+// clusters relax evenly, as in the reference's loop where each cut edge is
+// visited once; with twelve the moves concentrate on the proposals of the
+// pass start.  Measured on the replay against the serial reference (same
+// initial labels, eight accept seeds): icosphere 40,962 vertices, K = 2,000,
+// seed vertices: gap median +0.02 % (range -0.14 .. +0.24) with 12 rounds,
+// -0.10 % (-0.19 .. +0.04) with 1; cube 960,002 vertices, K = 20,000, QEM:
+// +0.01 .. +0.03 % with 12, -0.001 .. +0.002 % with 1.
+constexpr int32_t kDefaultFirstRounds = 1;
+
+ACVD_HD int32_t pass_rounds(const MeshState& s) { return s.convergences == 0 ? s.first_rounds : kMatchRounds; }
 
 // Early stop of the first two phases.  The reference stops a phase when a
 // loop applies at most N/1000 moves (vtkUniformClustering.h:773-776), a loop

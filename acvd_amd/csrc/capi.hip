@@ -125,8 +125,10 @@ struct acvd_ctx {
   size_t cub_bytes = 0;
   // pass-chunk graphs by (QEM meshes unconstrained, tile sets maintained,
   // short chunk): index qem_unc + 2 * tiles + 4 * short
-  hipGraphExec_t chunk_exec[8] = {};
-  hipGraph_t chunk_graph[8] = {};
+  // captured chunks by (QEM unconstrained, tile sets on, short, rounds - 1)
+  static constexpr int kChunkKinds = 8 * kMatchRounds;
+  hipGraphExec_t chunk_exec[kChunkKinds] = {};
+  hipGraph_t chunk_graph[kChunkKinds] = {};
   double build_ms = 0.0, converge_ms = 0.0;
   Ctrl last_ctrl{};
   bool timing = false;
@@ -191,7 +193,7 @@ struct acvd_ctx {
   }
 
   void drop_graphs() {
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < kChunkKinds; ++k) {
       if (chunk_exec[k]) (void)hipGraphExecDestroy(chunk_exec[k]);
       if (chunk_graph[k]) (void)hipGraphDestroy(chunk_graph[k]);
       chunk_exec[k] = nullptr;
@@ -447,6 +449,8 @@ int build(acvd_ctx* c) {
     HIP_TRY(c->d_tile_cnt.alloc(nt));
     HIP_TRY(c->d_tile_set.alloc(nt * kTileSet));
     HIP_TRY(c->d_tile_prop.alloc(nt));
+    // no cluster id equals the fill value (free set slots, see k_tile_sets)
+    HIP_TRY(launch_fill_i32(s, c->d_tile_set.p, nt * kTileSet, -2));
     HIP_TRY(hipMemcpyAsync(c->d_tile_mesh.p, tile_mesh.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, s));
     DBuf<float> d_bbox;
     HIP_TRY(d_bbox.alloc(6 * M));
@@ -565,14 +569,14 @@ int recompute_statistics(acvd_ctx* c, const int32_t* mask, int set_stamp, unsign
   return ACVD_OK;
 }
 
-int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
+int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass, int n_rounds) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   for (int p = 0; p < n_pass; ++p) {
     HIP_TRY(launch_scan(s, a, tiles));
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(launch_apply(s, a, 0, !qem_unc));
-    for (int r = 1; r < kMatchRounds; ++r) {
+    for (int r = 1; r < n_rounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
       HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
@@ -583,7 +587,7 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
 }
 
 // Passes launched one by one with events around each kernel.
-int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
+int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass, int n_rounds) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   std::vector<hipEvent_t> ev(5 * n_pass);
@@ -595,7 +599,7 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
     HIP_TRY(launch_eval(s, a, c->any_iso, c->any_qem, qem_unc));
     HIP_TRY(hipEventRecord(ev[5 * p + 2], s));
     HIP_TRY(launch_apply(s, a, 0, !qem_unc));
-    for (int r = 1; r < kMatchRounds; ++r) {
+    for (int r = 1; r < n_rounds; ++r) {
       HIP_TRY(launch_reeval(s, a, c->any_iso, c->any_qem, qem_unc, r));
       HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
@@ -616,15 +620,26 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass) {
   return ACVD_OK;
 }
 
+// Accept rounds a chunk launches: the most any sweeping mesh runs per pass
+// (a mesh's count changes only at a convergence, which ends the chunk's work
+// for that mesh until the host cleans it).
+int chunk_rounds(const acvd_ctx* c) {
+  int r = 1;
+  for (size_t m = 0; m < c->meshes.size(); ++m)
+    if (c->h_state[m].phase == kPhaseSweep) r = std::max<int>(r, pass_rounds(c->h_state[m]));
+  return r;
+}
+
 int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles, bool short_chunk) {
   hipStream_t s = c->stream;
   const int n_pass = short_chunk ? kShortChunk : kChunk;
-  if (c->timing) return run_chunk_timed(c, qem_unc, tiles, n_pass);
+  const int n_rounds = chunk_rounds(c);
+  if (c->timing) return run_chunk_timed(c, qem_unc, tiles, n_pass, n_rounds);
   if (use_graph) {
-    const int k = int(qem_unc) + 2 * int(tiles) + 4 * int(short_chunk);
+    const int k = int(qem_unc) + 2 * int(tiles) + 4 * int(short_chunk) + 8 * (n_rounds - 1);
     if (!c->chunk_exec[k]) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_chunk(c, qem_unc, tiles, n_pass);
+      int rc = enqueue_chunk(c, qem_unc, tiles, n_pass, n_rounds);
       hipGraph_t g = nullptr;
       hipError_t ec = hipStreamEndCapture(s, &g);
       if (rc != ACVD_OK) return rc;
@@ -635,7 +650,7 @@ int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles, bool short_
     HIP_TRY(hipGraphLaunch(c->chunk_exec[k], s));
     return ACVD_OK;
   }
-  return enqueue_chunk(c, qem_unc, tiles, n_pass);
+  return enqueue_chunk(c, qem_unc, tiles, n_pass, n_rounds);
 }
 
 // True when no QEM mesh of the context is constrained (the state flips only
@@ -678,6 +693,7 @@ int minimize_energy(acvd_ctx* c) {
     st.max_loops = std::min<int32_t>(p.max_number_of_loops, kMaxLoops - 2);
     st.max_convergences = p.max_number_of_convergences;
     st.unconstrained_init = p.unconstrained_initialization;
+    st.first_rounds = p.first_phase_rounds > 0 ? p.first_phase_rounds : kDefaultFirstRounds;
     // ProcessClustering: SetConstrainedClustering(0) when unconstrained
     // initialisation is requested (vtkUniformClustering.h:683-688)
     st.constrained = p.unconstrained_initialization ? 0 : 1;
@@ -855,6 +871,7 @@ void acvd_params_init(acvd_params* p) {
   p->seed = 0;
   p->initial_sampling = 0;
   p->curvature_indicator = 0;
+  p->first_phase_rounds = 0;
   p->gradation = 0.0;
 }
 
@@ -910,6 +927,8 @@ int acvd_set_params(acvd_ctx* ctx, int32_t mesh_id, const acvd_params* p) {
     return fail(ACVD_ERR_ARG, "quadrics optimization level must be 0..3");
   if (p->initial_sampling != 0 && p->initial_sampling != 1) return fail(ACVD_ERR_ARG, "initial sampling must be 0 or 1");
   if (p->max_number_of_loops < 0) return fail(ACVD_ERR_ARG, "negative loop limit");
+  if (p->first_phase_rounds < 0 || p->first_phase_rounds > kMatchRounds)
+    return fail(ACVD_ERR_ARG, "first phase rounds must be 0..12");
   if (mesh_id == -1) {
     for (auto& h : ctx->meshes) h.params = *p;
   } else {

@@ -292,6 +292,11 @@ __global__ __launch_bounds__(64) void k_tile_sets(SweepArgs a, const int32_t* __
       const int32_t c = tab[k];
       if (c != kEmpty) a.tile_set[int64_t(t) * kTileSet + atomicAdd(&n_out, 1)] = c;
     }
+    // the free slots hold a value no cluster id equals: tile_insert may read a
+    // slot another insert has reserved but not written yet, and must not
+    // mistake a stale id left there for its own cluster
+    for (int k = threadIdx.x; k < kTileSet; k += 64)
+      if (k >= n_uniq) a.tile_set[int64_t(t) * kTileSet + k] = kEmpty;
   }
   if (threadIdx.x == 0) a.tile_cnt[t] = big ? kTileSet + 1 : n_uniq;
 }
@@ -400,6 +405,12 @@ __device__ inline bool round_was_empty(const SweepArgs& a, int r) {
   }
   __syncthreads();
   return s_empty != 0;
+}
+
+// True when round r lies past the pass's last round for the only mesh of the
+// context (a batch checks each candidate's mesh instead).
+__device__ inline bool past_last_round(const SweepArgs& a, int r) {
+  return !a.item_mesh && r >= pass_rounds(a.state[0]);
 }
 
 // Metric of the meshes a kernel instantiation sweeps: 0 isotropic, 1 QEM,
@@ -776,8 +787,9 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
     lab = lab_e;  // a live candidate has not moved
     m = mesh_of_item(a, i);
     const DevMesh& dm = a.meshes[m];
-    if (dm.metric == mesh_metric(METRIC)) {
-      const MeshState& st = a.state[m];
+    const MeshState& st = a.state[m];
+    // a mesh past its last round of the pass registers nothing (no move)
+    if (dm.metric == mesh_metric(METRIC) && round < pass_rounds(st)) {
       const int32_t ra = lab != kNull ? a.cl_round[lab] : -1, rb = a.cl_round[b];
       stale = ra >= since || rb >= since;
       key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));
@@ -820,8 +832,9 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ SegIndex si;
   __shared__ QemPool pools[METRIC == 1 ? kBlock / 64 : 1];
   const WalkCtl w = load_walk_ctl(a);
+  const bool past = past_last_round(a, round);
   keyagg_clear(&agg);  // made visible by the barrier in round_was_empty
-  if (round_was_empty(a, round - 1)) return;
+  if (round_was_empty(a, round - 1) || past) return;
   const SweepView v = make_view(a);
   QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
   walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
@@ -980,7 +993,8 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   const int64_t mseg = int64_t(blockIdx.x) * a.cand_cap;
   const WalkCtl w = load_walk_ctl(a);
   const int32_t moved0 = round == 0 ? 0 : a.seg_moved[blockIdx.x];
-  if (round > 0 && round_was_empty(a, round - 1)) return;
+  const bool past = past_last_round(a, round);
+  if (round > 0 && (round_was_empty(a, round - 1) || past)) return;
   if (threadIdx.x == 0) s_moved = moved0;
   __syncthreads();
   DeriveList* dl = &dlist[kSolve ? threadIdx.x >> 6 : 0];

@@ -59,6 +59,8 @@ typedef struct acvd_ctx acvd_ctx;
  *                                  the faces within two edges of each vertex, vtkCurvatureMeasure.cxx:188-509)
  *                                  on the device and weights the items with it; Remesh() of the class API
  *                                  sets it, as the reference's Remesh runs SamplingPreProcessing
+ *   first_phase_rounds          <- (no counterpart) accept rounds per pass until the first convergence,
+ *                                  1..12 (0: the library default, 1); later phases run 12
  */
 typedef struct acvd_params {
   int32_t number_of_clusters;
@@ -71,6 +73,7 @@ typedef struct acvd_params {
   uint32_t seed;
   int32_t initial_sampling;
   int32_t curvature_indicator;
+  int32_t first_phase_rounds;
   double gradation;
 } acvd_params;
 
@@ -105,7 +108,8 @@ const char* acvd_last_error(void);
 int acvd_device_count(void);
 /* Fills p with the reference defaults: K = 0, isotropic, level 3, gradation 0,
  * unconstrained initialisation off, max loops 5000000, max convergences 1e9,
- * console output 0, seed 0, the reference's initial sampling (0). */
+ * console output 0, seed 0, the reference's initial sampling (0), the
+ * library's default first-phase rounds (0). */
 void acvd_params_init(acvd_params* p);
 
 /* ---- context -------------------------------------------------------------- */

@@ -11,6 +11,7 @@
 // plus the reference's quadric tools and ring check for unit tests, and a
 // host build of the device sweep inputs (devinputs.h).
 
+#include <algorithm>
 #include <chrono>
 #include <random>
 #include <cstdint>
@@ -40,10 +41,18 @@ struct OrcStats {
   double shifted_energy;
 };
 
+// per-loop progress lines of the serial runs on stderr (orc_set_progress)
+int g_progress = 0;
+
 template <class M>
 int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int level, int unconstrained,
-               const int32_t* init, int max_loops, int32_t* out, OrcStats* st) {
+               const int32_t* init, int max_loops, int32_t* out, OrcStats* st,
+               std::vector<EdgeTrace>* etrace = nullptr, size_t ecap = 0, std::vector<LoopTrace>* ltrace = nullptr) {
   Clustering<M> cl(m);
+  cl.progress = g_progress;
+  cl.edge_trace = etrace;
+  cl.edge_trace_cap = ecap;
+  cl.loop_trace = ltrace;
   if constexpr (M::kQEM) cl.metric.level = level;
   cl.set_number_of_clusters(K);
   cl.unconstrained_init = unconstrained;
@@ -203,6 +212,46 @@ int orc_serial_run(void* mh, int metric, int K, double gradation, const double* 
     const RefMesh& m = *static_cast<RefMesh*>(mh);
     if (metric == 0) return run_serial<IsoMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, out, st);
     return run_serial<QEMMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, out, st);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+void orc_set_progress(int on) { g_progress = on; }
+
+// orc_serial_run with its trace: edge records (loop, decision, edge, i1, i2 in
+// edge_i[5k..]; try1..try3 in edge_d[3k..]) up to edge_cap, loop records
+// (loop, mods, connexity, constrained, convergence, disconnected in
+// loop_i[6k..]) up to loop_cap; n_out = {edge records, loop records}.
+int orc_serial_trace(void* mh, int metric, int K, double gradation, const double* ind, int level, int unconstrained,
+                     const int32_t* init, int max_loops, int32_t* out, OrcStats* st, int64_t edge_cap,
+                     int64_t* edge_i, double* edge_d, int64_t loop_cap, int32_t* loop_i, int64_t* n_out) {
+  try {
+    const RefMesh& m = *static_cast<RefMesh*>(mh);
+    std::vector<EdgeTrace> et;
+    std::vector<LoopTrace> lt;
+    int rc = metric == 0 ? run_serial<IsoMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, out, st,
+                                                 &et, size_t(edge_cap), &lt)
+                         : run_serial<QEMMetric>(m, K, gradation, ind, level, unconstrained, init, max_loops, out,
+                                                 st, &et, size_t(edge_cap), &lt);
+    for (size_t k = 0; k < et.size(); ++k) {
+      const EdgeTrace& r = et[k];
+      const int64_t v[5] = {r.loop, r.decision, r.edge, r.i1, r.i2};
+      for (int j = 0; j < 5; ++j) edge_i[5 * k + j] = v[j];
+      edge_d[3 * k] = r.try1;
+      edge_d[3 * k + 1] = r.try2;
+      edge_d[3 * k + 2] = r.try3;
+    }
+    const size_t nl = std::min(lt.size(), size_t(loop_cap));
+    for (size_t k = 0; k < nl; ++k) {
+      const LoopTrace& r = lt[k];
+      const int32_t v[6] = {r.loop, r.mods, r.connexity, r.constrained, r.convergence, r.disconnected};
+      for (int j = 0; j < 6; ++j) loop_i[6 * k + j] = v[j];
+    }
+    n_out[0] = int64_t(et.size());
+    n_out[1] = int64_t(nl);
+    return rc;
   } catch (std::exception& e) {
     g_err = e.what();
     return -1;
@@ -376,6 +425,8 @@ int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* c
   Replay r;
   r.generic_qem = (path & 1) != 0;
   r.all_rounds = (path & 2) != 0;
+  // bits 8..15: accept rounds per pass before the first convergence (0: default)
+  if ((path >> 8) & 0xFF) r.first_rounds = std::min((path >> 8) & 0xFF, acvd::kMatchRounds);
   r.N = n;
   r.K = K;
   r.metric = metric;

@@ -53,6 +53,9 @@ def lib():
         L.orc_build_items.argtypes = [P, ctypes.c_int, P, ctypes.c_double, P, P]
         L.orc_serial_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int, ctypes.c_int,
                                      P, ctypes.c_int, P, P]
+        L.orc_set_progress.argtypes = [ctypes.c_int]
+        L.orc_serial_trace.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int, ctypes.c_int,
+                                       P, ctypes.c_int, P, P, ctypes.c_int64, P, P, ctypes.c_int64, P, P]
         L.orc_threaded_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int,
                                        ctypes.c_int, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
         L.orc_initial_sampling.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, P]
@@ -150,6 +153,32 @@ class RefMesh:
             raise RuntimeError(lib().orc_last_error().decode())
         return out, st.as_dict()
 
+    def serial_trace(self, metric, K, gradation=0.0, indicator=None, level=3, unconstrained=0, init=None,
+                     max_loops=5000000, edge_cap=100000, loop_cap=100000):
+        """serial_run with its trace: (labels, stats, edges, loops) where edges
+        is a dict of arrays loop, decision (0 stay, 1 move I1, 2 move I2),
+        edge, i1, i2, try1, try2, try3 (one per evaluated cut edge, the first
+        edge_cap) and loops a dict of arrays loop, mods, connexity,
+        constrained, convergence, disconnected (one per loop)."""
+        out = np.empty(self.nv, np.int32)
+        st = OrcStats()
+        ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
+        ini = None if init is None else np.ascontiguousarray(init, np.int32)
+        ei = np.empty((edge_cap, 5), np.int64)
+        ed = np.empty((edge_cap, 3), np.float64)
+        li = np.empty((loop_cap, 6), np.int32)
+        n = np.zeros(2, np.int64)
+        rc = lib().orc_serial_trace(self.h, metric, K, gradation, _p(ind), level, unconstrained, _p(ini), max_loops,
+                                    _p(out), ctypes.byref(st), edge_cap, _p(ei), _p(ed), loop_cap, _p(li), _p(n))
+        if rc:
+            raise RuntimeError(lib().orc_last_error().decode())
+        ne, nl = int(n[0]), int(n[1])
+        edges = {k: ei[:ne, j].copy() for j, k in enumerate(("loop", "decision", "edge", "i1", "i2"))}
+        edges.update({k: ed[:ne, j].copy() for j, k in enumerate(("try1", "try2", "try3"))})
+        loops = {k: li[:nl, j].copy() for j, k in
+                 enumerate(("loop", "mods", "connexity", "constrained", "convergence", "disconnected"))}
+        return out, st.as_dict(), edges, loops
+
     def threaded_run(self, metric, K, gradation=0.0, indicator=None, level=3, unconstrained=0, init=None,
                      max_loops=5000000, threads=1, pooling=5):
         out = np.empty(self.nv, np.int32)
@@ -189,6 +218,11 @@ class RefMesh:
     def ring_problem(self, labels, item, cluster):
         lab = np.ascontiguousarray(labels, np.int32)
         return lib().orc_ring_problem(self.h, _p(lab), int(item), int(cluster))
+
+
+def set_progress(on):
+    """per-loop progress lines of the serial runs on stderr"""
+    lib().orc_set_progress(int(bool(on)))
 
 
 def rep_point(Q9, P, level=3, thr=1e-3):
@@ -243,7 +277,7 @@ def hdr_accept_key(seed, loop, item, round=0):
 
 
 def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, unconstrained=0, max_loops=5000000,
-               max_conv=1000000000, generic_qem=False, all_rounds=False):
+               max_conv=1000000000, generic_qem=False, all_rounds=False, first_rounds=0):
     n = len(row_ptr) - 1
     out = np.empty(n, np.int32)
     st = np.zeros(6, np.int64)
@@ -253,7 +287,7 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
     qq = None if q is None else np.ascontiguousarray(q, np.float32)
     sc = np.ascontiguousarray(scales, np.float64)
     ini = np.ascontiguousarray(init, np.int32)
-    lib().orc_replay_run_path(int(generic_qem) | (int(all_rounds) << 1), n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
+    lib().orc_replay_run_path(int(generic_qem) | (int(all_rounds) << 1) | (int(first_rounds) << 8), n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
                               unconstrained, max_loops, max_conv, _p(ini), _p(out), _p(st))
     keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
     return out, dict(zip(keys, (int(v) for v in st)))

@@ -34,6 +34,7 @@ struct Replay {
   int max_loops = 5000000, max_conv = 1000000000, unconstrained_init = 0;
   bool generic_qem = false;  // force the generic QEM proposal path (equivalence checks)
   bool all_rounds = false;   // run every round even after one without moves (equivalence checks)
+  int first_rounds = acvd::kDefaultFirstRounds;  // accept rounds per pass before the first convergence
   const int32_t* rp = nullptr;
   const int32_t* col = nullptr;
   const float* xw = nullptr;
@@ -181,7 +182,8 @@ struct Replay {
     int mods = 0;
     std::vector<int32_t> from(cand.size());
     std::vector<int32_t> since(cand.size(), L * acvd::kMatchRounds);
-    for (int round = 0; round < acvd::kMatchRounds; ++round) {
+    const int nrounds = acvd::pass_rounds(st);
+    for (int round = 0; round < nrounds; ++round) {
       const int mods_before = mods;
       if (round > 0) {
         // a live proposal whose own cluster or target moved since its last
@@ -293,6 +295,7 @@ struct Replay {
     st.max_loops = std::min<int32_t>(max_loops, acvd::kMaxLoops - 2);
     st.max_convergences = max_conv;
     st.unconstrained_init = unconstrained_init;
+    st.first_rounds = first_rounds;
     st.constrained = unconstrained_init ? 0 : 1;
     st.early_stop_items = recompute(true);
     while (st.phase == acvd::kPhaseSweep) {

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdint>
 #include <deque>
 #include <queue>
@@ -35,6 +36,18 @@
 #include "ref_mesh.h"
 
 namespace oracle {
+
+// Optional trace of a run (tests): one record per evaluated cut edge (the
+// three energies of :922-956 and the decision of :960-991: 0 stay, 1 move I1,
+// 2 move I2), one per loop (the state the loop ran with and what followed it).
+struct EdgeTrace {
+  int32_t loop, decision;
+  int64_t edge, i1, i2;
+  double try1, try2, try3;
+};
+struct LoopTrace {
+  int32_t loop, mods, connexity, constrained, convergence, disconnected;
+};
 
 struct RunStats {
   int64_t loops = 0;
@@ -66,7 +79,12 @@ struct Clustering {
   int initial_sampling_type = 1;
   const int32_t* initial_labels = nullptr;
   int console = 0;
+  int progress = 0;  // 1: one line per loop on stderr (long runs)
   RunStats stats;
+  std::vector<EdgeTrace>* edge_trace = nullptr;
+  size_t edge_trace_cap = 0;
+  std::vector<LoopTrace>* loop_trace = nullptr;
+  int constrained_now = 1;  // the metric's ActiveConstraints (QEM), for the trace
   // scratch for the ring check
   std::vector<int32_t> vlist;
   std::deque<int32_t> vqueue;
@@ -466,9 +484,14 @@ struct Clustering {
         M::compute_energy(&c32);
         try3 = M::energy(&c31) + M::energy(&c32);
       }
-      if (try1 <= try2 && try1 <= try3) {
+      // stay when Try1 is not above either move; else the strictly lower
+      // of Try2 (move I1) and Try3 (move I2), a tie moving I2
+      const int32_t decision = (try1 <= try2 && try1 <= try3) ? 0 : (try2 < try1 && try2 < try3) ? 1 : 2;
+      if (edge_trace && edge_trace->size() < edge_trace_cap)
+        edge_trace->push_back(EdgeTrace{loops, decision, edge, i1, i2, try1, try2, try3});
+      if (decision == 0) {
         queue.push_back(edge);
-      } else if (try2 < try1 && try2 < try3) {
+      } else if (decision == 1) {
         labels[i1] = v2;
         sizes[v2]++;
         sizes[v1]--;
@@ -502,8 +525,11 @@ struct Clustering {
     for (int i = 0; i < K; ++i) early_stop_items += sizes[i];
     while (true) {
       swap_queues();
+      const int loop_connexity = connexity;
       int mods = process_one_loop();
       stats.total_modifications += mods;
+      if (progress) std::fprintf(stderr, "%sLoop %d: %d Modifications\n", connexity ? "*" : "", loops, mods);
+      LoopTrace lt{loops, mods, loop_connexity, constrained_now, 0, 0};
       loops++;
       if (rel_loops == 255) {
         for (auto& b : edge_last_loop) b = 0;
@@ -511,13 +537,23 @@ struct Clustering {
       } else {
         rel_loops++;
       }
-      if (mods == 0 || loops > max_loops || (mods <= early_stop_items / 1000 && convergences <= 1)) {
-        if (unconstrained_init && convergences == 0) metric.set_constrained(1);
+      const bool conv = mods == 0 || loops > max_loops || (mods <= early_stop_items / 1000 && convergences <= 1);
+      if (!conv && loop_trace) loop_trace->push_back(lt);
+      if (conv) {
+        if (unconstrained_init && convergences == 0) {
+          metric.set_constrained(1);
+          constrained_now = 1;
+        }
         if (convergences >= 1) connexity = 1;
         convergences++;
         int ndisc = clean_clustering();
         fill_holes();
         recompute_sizes();
+        if (loop_trace) {
+          lt.convergence = 1;
+          lt.disconnected = ndisc;
+          loop_trace->push_back(lt);
+        }
         if (ndisc == 0 && mods == 0) break;
         if (loops >= max_loops) break;
         if (convergences >= max_convergences) break;
@@ -535,7 +571,10 @@ struct Clustering {
   void process(const double* indicator, double gradation) {
     init(indicator, gradation);
     init_samples();
-    if (unconstrained_init) metric.set_constrained(0);
+    if (unconstrained_init) {
+      metric.set_constrained(0);
+      constrained_now = 0;
+    }
     auto t0 = std::chrono::steady_clock::now();
     minimize_energy();
     auto t1 = std::chrono::steady_clock::now();
