@@ -73,6 +73,7 @@ EXPORTS = [
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
     "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
     "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
+    "acvd_energy_forward_cvt_noalloc",
 ]
 
 _lib = None
@@ -135,6 +136,7 @@ def lib():
     L.acvd_energy_num_items.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i32)]
     L.acvd_energy_forward.argtypes = [P, P, P, P, P, P]
     L.acvd_energy_backward.argtypes = [P, P, P, P, P, P, P, P]
+    L.acvd_energy_forward_cvt_noalloc.argtypes = [ctypes.c_int, i64, i32, P, P, P, P, P]
     _lib = L
     return L
 

@@ -40,6 +40,12 @@ struct EnergyArgs {
 };
 
 hipError_t energy_forward(hipStream_t s, const EnergyArgs& a, double* energy, int n_meshes);
+// Largest cluster count of energy_forward_cvt_noalloc (its static scratch).
+constexpr int32_t kNoAllocMaxClusters = 65536;
+// energy_forward for one isotropic mesh with every per-call buffer in the
+// library's static device scratch of the current device (no allocation).
+hipError_t energy_forward_cvt_noalloc(hipStream_t s, int64_t n, int32_t K, const float* xyz, const float* w,
+                                      const int32_t* labels, double* energy);
 hipError_t energy_backward(hipStream_t s, const EnergyArgs& a, const double* grad_energy, float* dxyz, float* dw);
 
 }  // namespace acvd

@@ -1404,6 +1404,21 @@ int acvd_energy_forward(acvd_energy* e, const float* xyz, const float* w, const 
   return ACVD_OK;
 }
 
+int acvd_energy_forward_cvt_noalloc(int device, int64_t n, int32_t n_clusters, const float* xyz, const float* w,
+                                    const int32_t* labels, double* energy, void* stream) {
+  if (n < 0 || n >= (int64_t(1) << 31) - 1) return fail(ACVD_ERR_LIMIT, "item count must be in 0..2^31-2");
+  if (n_clusters <= 0) return fail(ACVD_ERR_ARG, "number of clusters must be positive");
+  if (n_clusters > kNoAllocMaxClusters)
+    return fail(ACVD_ERR_LIMIT, "acvd_energy_forward_cvt_noalloc takes at most 65536 clusters (its static scratch)");
+  if ((n > 0 && (!xyz || !w || !labels)) || !energy) return fail(ACVD_ERR_ARG, "null device buffer");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(ACVD_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= count) return fail(ACVD_ERR_ARG, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(energy_forward_cvt_noalloc(static_cast<hipStream_t>(stream), n, n_clusters, xyz, w, labels, energy));
+  return ACVD_OK;
+}
+
 int acvd_energy_backward(acvd_energy* e, const float* xyz, const float* w, const int32_t* labels,
                          const double* grad_energy, float* dxyz, float* dw, void* stream) {
   if (!e) return fail(ACVD_ERR_ARG, "null energy context");

@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "acvd_energy.h"
 
 namespace acvd {
@@ -420,6 +422,67 @@ hipError_t energy_forward(hipStream_t s, const EnergyArgs& a, double* energy, in
   if (a.n_clusters > 0) hipLaunchKernelGGL(ke_points, dim3(egrid(a.n_clusters)), dim3(kEBlock), 0, s, a);
   if (a.n_items > 0) hipLaunchKernelGGL(ke_energy, dim3(egrid(a.n_items)), dim3(kEBlock), 0, s, a, energy);
   return hipGetLastError();
+}
+
+// ------------------------------------------------- forward without allocation
+namespace {
+
+// Scratch of energy_forward_cvt_noalloc: the mesh descriptor, then the
+// per-cluster sums (kAcc doubles) and points (6 doubles) of at most
+// kNoAllocMaxClusters clusters; one copy per device, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel (in host code the
+// name is only a stand-in): it uses the device address found once per device
+// by hipGetSymbolAddress.
+constexpr size_t kNoAllocMeshBytes = 256;
+constexpr size_t kNoAllocBytes =
+    kNoAllocMeshBytes + size_t(kNoAllocMaxClusters) * size_t(kAcc + 6) * sizeof(double);
+static_assert(sizeof(EnergyMesh) <= kNoAllocMeshBytes, "mesh descriptor slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+
+// Device address of `scratch` on the current device (looked up once per device).
+hipError_t noalloc_scratch(unsigned char** out) {
+  static std::mutex mu;
+  static unsigned char* ptr[kNoAllocMaxDevices] = {};
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess) return err;
+    ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = ptr[dev];
+  return hipSuccess;
+}
+
+__global__ void ke_set_mesh(EnergyMesh m, EnergyMesh* dst) { *dst = m; }
+
+}  // namespace
+
+hipError_t energy_forward_cvt_noalloc(hipStream_t s, int64_t n, int32_t K, const float* xyz, const float* w,
+                                      const int32_t* labels, double* energy) {
+  unsigned char* base = nullptr;
+  hipError_t err = noalloc_scratch(&base);
+  if (err != hipSuccess) return err;
+  EnergyMesh* mesh = reinterpret_cast<EnergyMesh*>(base);
+  double* acc = reinterpret_cast<double*>(base + kNoAllocMeshBytes);
+  EnergyArgs a{};
+  a.n_items = n;
+  a.n_clusters = K;
+  a.meshes = mesh;
+  a.xyz = xyz;
+  a.w = w;
+  a.labels = labels;
+  a.acc = acc;
+  a.pts = acc + size_t(kNoAllocMaxClusters) * kAcc;
+  // the descriptor is written by a kernel (no host copy: the call may be captured)
+  hipLaunchKernelGGL(ke_set_mesh, dim3(1), dim3(1), 0, s, EnergyMesh{0, n, 0, 0, K, 0, 0, 0}, mesh);
+  return energy_forward(s, a, energy, 1);
 }
 
 hipError_t energy_backward(hipStream_t s, const EnergyArgs& a, const double* grad_energy, float* dxyz, float* dw) {

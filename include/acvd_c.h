@@ -248,6 +248,19 @@ int acvd_energy_forward(acvd_energy* e, const float* xyz, const float* w, const 
 int acvd_energy_backward(acvd_energy* e, const float* xyz, const float* w, const int32_t* labels,
                          const double* grad_energy, float* dxyz, float* dw, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): acvd_energy_forward of ONE isotropic (CVT) mesh without a
+ * context and without any allocation.  xyz float[3n], w float[n], labels
+ * int32[n] (0..n_clusters-1; others = no cluster) and energy double[1] are
+ * device pointers on `device`; the per-cluster sums live in a static device
+ * array of the library (one per device, held until the process ends), so
+ * n_clusters is at most 65536 (ACVD_ERR_LIMIT beyond).  Asynchronous on
+ * `stream` (NULL: the null stream).  Calls that overlap on one device race on
+ * that array: serialise them (one stream, or events). */
+int acvd_energy_forward_cvt_noalloc(int device, int64_t n, int32_t n_clusters, const float* xyz, const float* w,
+                                    const int32_t* labels, double* energy, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
