@@ -54,6 +54,11 @@ class Stats(ctypes.Structure):
         ("n_adjacency", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double * 4),
         ("kernel_launches", ctypes.c_int64 * 4),
+        ("ctx_rounds", ctypes.c_int64),
+        ("ctx_walked", ctypes.c_int64),
+        ("ctx_reeval_live", ctypes.c_int64),
+        ("ctx_reevaluated", ctypes.c_int64),
+        ("ctx_accepted_constrained", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -73,6 +78,7 @@ EXPORTS = [
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
     "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
     "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
+    "acvd_set_fixed_clusters",
     "acvd_energy_forward_cvt_noalloc",
 ]
 
@@ -136,6 +142,7 @@ def lib():
     L.acvd_energy_num_items.argtypes = [P, ctypes.POINTER(i64), ctypes.POINTER(i32)]
     L.acvd_energy_forward.argtypes = [P, P, P, P, P, P]
     L.acvd_energy_backward.argtypes = [P, P, P, P, P, P, P, P]
+    L.acvd_set_fixed_clusters.argtypes = [P, i32, i64, P]
     L.acvd_energy_forward_cvt_noalloc.argtypes = [ctypes.c_int, i64, i32, P, P, P, P, P]
     _lib = L
     return L

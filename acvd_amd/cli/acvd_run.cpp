@@ -16,6 +16,12 @@
 //                 curvature indicator (vtkCurvatureMeasure), computed on the
 //                 device
 //   -qem 0/1      QEM metric with unconstrained initialisation (ACVDQ)
+//   -fv file      ACVDQ: fixed vertices, one id per entry (ACVDQ.cxx:246-253);
+//                 each becomes a cluster of its own anchored at that vertex
+//                 (SetFixedClusters + AnchorItem, :327-334; nvertices counts
+//                 the free clusters), and the output is checked to hold every
+//                 fixed vertex exactly (:340-366)
+//   -ft file      ACVDQ: fixed triangles; their vertices are fixed (:255-281)
 //   -m -d -b -l -w -cmin -cmax -cf -np -p   accepted; not supported (warned)
 //
 // Meshes are read from PLY (ascii or binary little-endian) or OBJ and written
@@ -184,6 +190,34 @@ bool write_ply(const std::string& path, const acvd::Mesh& m) {
   return std::fclose(f) == 0;
 }
 
+// Fixed vertices of -fv (ids) or -ft (triangle ids -> their vertices in
+// increasing order), Examples/ACVDQ.cxx:246-281.
+bool read_fixed(const std::string& path, bool triangles, const acvd::Mesh& m, std::vector<int32_t>& out,
+                std::string& err) {
+  std::ifstream in(path);
+  if (!in) return err = "cannot open " + path, false;
+  out.clear();
+  long long id;
+  if (!triangles) {
+    while (in >> id) {
+      if (id < 0 || id >= m.GetNumberOfPoints()) return err = path + ": vertex id out of range", false;
+      out.push_back(int32_t(id));
+    }
+    return true;
+  }
+  std::vector<char> fixed(size_t(m.GetNumberOfPoints()), 0);
+  int n = 0;
+  while (in >> id) {
+    if (id < 0 || id >= m.GetNumberOfCells()) return err = path + ": triangle id out of range", false;
+    for (int k = 0; k < 3; ++k) fixed[size_t(m.triangles[3 * id + k])] = 1;
+    ++n;
+  }
+  for (int64_t i = 0; i < m.GetNumberOfPoints(); ++i)
+    if (fixed[size_t(i)]) out.push_back(int32_t(i));
+  std::printf("Added %d constraints on triangles\n", n);
+  return true;
+}
+
 bool read_values(const std::string& path, int64_t n, std::vector<double>& out, std::string& err) {
   std::ifstream in(path);
   if (!in) return err = "cannot open " + path, false;
@@ -196,10 +230,16 @@ bool read_values(const std::string& path, int64_t n, std::vector<double>& out, s
 
 template <class Remesher>
 int run(Remesher* remesh, const acvd::Mesh& mesh, int K, double gradation, int subsampling, int q_level,
-        bool qem, const std::vector<double>* indicator, const std::string& out_path,
-        const std::string& smooth_path) {
+        bool qem, const std::vector<double>* indicator, const std::vector<int32_t>* fixed,
+        const std::string& out_path, const std::string& smooth_path) {
   remesh->SetInput(&mesh);
-  remesh->SetNumberOfClusters(K);
+  if (fixed) {
+    remesh->SetFixedClusters(fixed);
+    remesh->SetNumberOfClusters(K + int(fixed->size()));
+    std::printf("Read %zu fixed Ids\n", fixed->size());
+  } else {
+    remesh->SetNumberOfClusters(K);
+  }
   remesh->SetConsoleOutput(2);
   remesh->SetSubsamplingThreshold(subsampling);
   remesh->GetMetric()->SetGradation(gradation);
@@ -225,6 +265,19 @@ int run(Remesher* remesh, const acvd::Mesh& mesh, int K, double gradation, int s
     }
   }
   const acvd::Mesh* out = remesh->GetOutput();
+  if (fixed) {  // the vertex constraints are respected (ACVDQ.cxx:340-366)
+    for (size_t i = 0; i < fixed->size(); ++i) {
+      double c1[3], c2[3];
+      mesh.GetPoint((*fixed)[i], c1);
+      out->GetPoint(int64_t(i), c2);
+      for (int j = 0; j < 3; ++j)
+        if (c1[j] != c2[j]) {
+          std::fprintf(stderr, "Error, vertex %d has been lost\n", (*fixed)[i]);
+          return 1;
+        }
+    }
+    std::printf("Constraints on vertices have been checked\n");
+  }
   if (!write_ply(out_path, *out)) {
     std::fprintf(stderr, "acvd_run: cannot write %s\n", out_path.c_str());
     return 1;
@@ -241,14 +294,16 @@ int main(int argc, char** argv) {
     std::printf("Usage : acvd_run file nvertices gradation [options]\n"
                 "nvertices is the desired number of vertices\n"
                 "gradation defines the influence of local curvature (0=uniform meshing)\n"
-                "options: -q level, -s threshold, -o directory, -of file, -cd indicator file, -qem 0/1\n");
+                "options: -q level, -s threshold, -o directory, -of file, -cd indicator file, -qem 0/1,\n"
+                "         -fv fixed vertices file, -ft fixed triangles file (with -qem 1)\n");
     return 0;
   }
   const std::string input = argv[1];
   const int K = std::atoi(argv[2]);
   const double gradation = std::atof(argv[3]);
   int subsampling = 10, q_level = -1, qem = 0;
-  std::string out_dir, out_file = "simplification.ply", density;
+  std::string out_dir, out_file = "simplification.ply", density, fixed_file;
+  bool fixed_triangles = false;
   for (int a = 4; a + 1 < argc; a += 2) {
     const std::string key = argv[a], value = argv[a + 1];
     if (key == "-s") subsampling = std::atoi(value.c_str());
@@ -257,6 +312,10 @@ int main(int argc, char** argv) {
     else if (key == "-of") out_file = value;
     else if (key == "-cd") density = value;
     else if (key == "-qem") qem = std::atoi(value.c_str());
+    else if (key == "-fv" || key == "-ft") {
+      fixed_file = value;
+      fixed_triangles = key == "-ft";
+    }
     else std::printf("option %s %s is not supported here; ignored\n", key.c_str(), value.c_str());
   }
   if (q_level < 0) q_level = qem ? 3 : 1;  // ACVDQ / ACVD defaults
@@ -278,16 +337,28 @@ int main(int argc, char** argv) {
     }
     ind = &indicator;
   }
+  std::vector<int32_t> fixed;
+  if (!fixed_file.empty()) {
+    if (!qem) {
+      std::fprintf(stderr, "acvd_run: -fv / -ft need the QEM metric (-qem 1), as in ACVDQ\n");
+      return 1;
+    }
+    if (!read_fixed(fixed_file, fixed_triangles, mesh, fixed, err)) {
+      std::fprintf(stderr, "acvd_run: %s\n", err.c_str());
+      return 1;
+    }
+  }
   const std::string prefix = out_dir.empty() ? "" : out_dir + "/";
   const std::string out_path = prefix + out_file, smooth_path = prefix + "smooth_" + out_file;
   int rc;
   if (qem) {
     auto* r = acvd::QIsotropicDiscreteRemeshing::New();
-    rc = run(r, mesh, K, gradation, subsampling, q_level, true, ind, out_path, smooth_path);
+    rc = run(r, mesh, K, gradation, subsampling, q_level, true, ind, fixed_file.empty() ? nullptr : &fixed, out_path,
+             smooth_path);
     r->Delete();
   } else {
     auto* r = acvd::IsotropicDiscreteRemeshing::New();
-    rc = run(r, mesh, K, gradation, subsampling, q_level, false, ind, out_path, smooth_path);
+    rc = run(r, mesh, K, gradation, subsampling, q_level, false, ind, nullptr, out_path, smooth_path);
     r->Delete();
   }
   return rc;

@@ -78,10 +78,17 @@ struct Ctrl {
   int32_t fill_changed;
   int32_t seg_walk;  // candidate walk of the next pass: 1 segment per block, 0 dense
   int32_t mods[kMatchRounds][kSlices];  // applied moves of this pass, per accept round
+  int32_t live[kMatchRounds][kSlices];  // candidates registered by the re-evaluate of round r
+  int32_t stale[kMatchRounds][kSlices]; // candidates re-evaluated in round r
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
+  unsigned long long accepted_constrained;  // of which in passes where mesh 0 was constrained (QEM solves)
   unsigned long long passes;     // passes run (any mesh sweeping)
+  unsigned long long rounds;      // accept rounds that walked the candidates (all passes)
+  unsigned long long walked;      // candidate entries read by the round kernels, dead ones included
+  unsigned long long reeval_live; // live candidates registered by re-evaluate rounds
+  unsigned long long reevaluated; // candidates re-evaluated (their clusters moved)
   // Per-pass counts of the last kPassLogCap passes (ring indexed by pass
   // number): the per-loop console output reads them between pass chunks.
   int32_t pass_log[kPassLogCap][kPassLogFields];
@@ -100,6 +107,8 @@ struct SweepArgs {
   const int32_t* row_ptr;    // n_items + 1
   const int32_t* col;        // global item ids
   const unsigned long long* ring_adj;  // ring adjacency per item (ring_adjacency of acvd_math.h)
+  const int32_t* anchor;     // per cluster: global id of its fixed item or -1; null when no mesh has any
+  int32_t count_rounds;      // count the live and re-evaluated candidates of each round (timed runs)
   const float4* xw;          // position + weight
   const float* q;            // 10 floats per item (QEM meshes), may be null
   int32_t* labels;

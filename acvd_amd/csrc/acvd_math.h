@@ -167,13 +167,17 @@ ACVD_HD int rep_point_delta(const double* Q, const double* c, int level, double*
   int keep0 = 0, keep1 = 0, keep2 = 0;
   int rank_def = 0;
   int left = level;
+  // |lambda| * (1 / max|lambda|) > threshold, the reference's form
+  // (Common/vtkQuadricTools.cxx:117, 134), which can differ from
+  // |lambda| / max|lambda| in the last bit at the threshold
+  const double inv_maxw = maxw > 0.0 ? 1.0 / maxw : 0.0;
   double rem0 = aw[0], rem1 = aw[1], rem2 = aw[2];
   for (int it = 0; it < 3; ++it) {
     int idx = 0;
     double best = rem0;
     if (best < rem1) { best = rem1; idx = 1; }
     if (best < rem2) { best = rem2; idx = 2; }
-    bool ok = (maxw > 0.0) && (best / maxw > kSvThreshold) && (left > 0);
+    bool ok = (maxw > 0.0) && (best * inv_maxw > kSvThreshold) && (left > 0);
     if (ok) {
       if (idx == 0) keep0 = 1;
       else if (idx == 1) keep1 = 1;
@@ -346,15 +350,32 @@ ACVD_HD double qem_mag(float w, float ta, float tb, double Ta_new, double Ta, do
 }
 ACVD_HD bool qem_accept(double delta, double mag) { return delta < -double(kAcceptEps) * mag; }
 
-// T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item,
-// the solve warm-started from the cluster's own eigenvectors V0 when given.
+// Fixed clusters (SetFixedClusters with the QEM metric's AnchorItem,
+// Common/vtkUniformClustering.h:134, DiscreteRemeshing/vtkQEMetricForClustering.h:
+// 196-199, 259, 270-274): the point of an anchored cluster is its anchor
+// item's position in every phase, constrained or not, so its energy is
+// -|S|^2/W + T with T = W |a - c|^2 and no solve.  The anchor item itself
+// never leaves (the reference prices that at 1e100).
+ACVD_HD double anchored_T(const ClusterGeom& g, const double* a) {
+  if (!(g.W > 0.0)) return 0.0;
+  const double dx = a[0] - g.cx, dy = a[1] - g.cy, dz = a[2] - g.cz;
+  return g.W * (dx * dx + dy * dy + dz * dz);
+}
+
+// T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item:
+// with the anchor position apos when the cluster is anchored, else (solve)
+// the rank-truncated solve warm-started from the cluster's own eigenvectors
+// V0 when given, else 0 (the cluster's point is its centroid).
 ACVD_HD double qem_T_trial(const int64_t* ws, const int64_t* q10, int64_t iw, const int64_t* is,
-                           const int64_t* iq, int sign, const Scales& s, int level, const double* V0 = nullptr) {
+                           const int64_t* iq, int sign, const Scales& s, int level, const double* V0 = nullptr,
+                           const double* apos = nullptr, bool solve = true) {
   int64_t W = ws[0] + sign * iw;
   int64_t Sx = ws[1] + sign * is[0], Sy = ws[2] + sign * is[1], Sz = ws[3] + sign * is[2];
+  ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, s);
+  if (apos) return anchored_T(g, apos);
+  if (!solve) return 0.0;
   int64_t Q[10];
   for (int k = 0; k < 10; ++k) Q[k] = q10[k] + sign * iq[k];
-  ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, s);
   return qem_T_from_fixed(Q, g, s, level, nullptr, V0);
 }
 
@@ -508,7 +529,24 @@ struct SweepView {
   const int32_t* stamp;
   const int32_t* cand_pass;
   const unsigned long long* ring_adj;  // ring adjacency per item (rows <= kRingAdjMax), or null
+  const int32_t* anchor;             // per cluster: its anchor item (fixed clusters) or -1; null if none
 };
+
+// Anchor position of cluster c (FP32 item position, as the sweep's items), if any.
+ACVD_HD bool anchor_pos(const SweepView& v, int32_t c, double* p) {
+  if (!v.anchor) return false;
+  const int32_t it = v.anchor[c];
+  if (it < 0) return false;
+  p[0] = double(v.xw[4 * int64_t(it)]);
+  p[1] = double(v.xw[4 * int64_t(it) + 1]);
+  p[2] = double(v.xw[4 * int64_t(it) + 2]);
+  return true;
+}
+
+// Item i is the anchor of its cluster lab: it never moves.
+ACVD_HD bool is_anchor(const SweepView& v, int64_t i, int32_t lab) {
+  return v.anchor && lab != kNull && v.anchor[lab] == int32_t(i);
+}
 
 // Ring check of item i leaving cluster a: with the ring adjacency for short
 // rows, else by the row walk.
@@ -601,6 +639,7 @@ ACVD_HD int32_t qem_next_open(const SweepView& v, int32_t lab, const QemItem& s,
 // Returns true when at least one target is open (solves and choice follow).
 ACVD_HD bool qem_screen(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained, QemItem* s) {
   if (v.sizes[lab] <= 1) return false;
+  if (is_anchor(v, i, lab)) return false;
   if (connexity && ring_check(v, i, lab)) return false;
   if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
   qem_item_terms(v, i, lab, constrained, s);
@@ -621,17 +660,27 @@ ACVD_HD void item_fixed(const SweepView& v, int64_t i, const Scales& sc, int64_t
   is[2] = fix_s(pw, pz, sc);
   for (int k = 0; k < 10; ++k) iq[k] = fix_q(v.q[10 * i + k], sc);
 }
-ACVD_HD double qem_leave_T(const SweepView& v, int64_t i, int32_t lab, const Scales& sc, int level) {
+// mode (qem_mode): 1 every cluster's point is solved, 2 only anchored
+// clusters carry a T (the others keep their centroid), 0 no T at all.
+ACVD_HD double qem_leave_T(const SweepView& v, int64_t i, int32_t lab, const Scales& sc, int level, int mode) {
+  double a[3];
+  const bool an = anchor_pos(v, lab, a);
+  if (!an && mode != 1) return 0.0;
   int64_t iw, is[3], iq[10], ws[4], q10[10];
   item_fixed(v, i, sc, &iw, is, iq);
   load_acc(v, lab, ws, q10);
-  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level, v.evec ? v.evec + 9 * int64_t(lab) : nullptr);
+  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level, v.evec ? v.evec + 9 * int64_t(lab) : nullptr,
+                     an ? a : nullptr);
 }
-ACVD_HD double qem_join_T(const SweepView& v, int64_t i, int32_t b, const Scales& sc, int level) {
+ACVD_HD double qem_join_T(const SweepView& v, int64_t i, int32_t b, const Scales& sc, int level, int mode) {
+  double a[3];
+  const bool an = anchor_pos(v, b, a);
+  if (!an && mode != 1) return 0.0;
   int64_t iw, is[3], iq[10], ws[4], q10[10];
   item_fixed(v, i, sc, &iw, is, iq);
   load_acc(v, b, ws, q10);
-  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level, v.evec ? v.evec + 9 * int64_t(b) : nullptr);
+  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level, v.evec ? v.evec + 9 * int64_t(b) : nullptr,
+                     an ? a : nullptr);
 }
 
 // Choice among the open targets (CSR order; ties: lowest cluster id) and the
@@ -686,6 +735,7 @@ ACVD_HD bool qem_screen_short_row(const SweepView& v, int64_t i, int32_t lab, in
   for (int p = 0; p < kRingCache; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
   *open_out = 0;
   if (size_a <= 1) return false;
+  if (is_anchor(v, i, lab)) return false;
   if (connexity) {
     const bool split = (v.ring_adj && deg <= kRingAdjMax)
                            ? ring_problem_mask(adj, nb, deg, lab)
@@ -714,7 +764,7 @@ ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, i
   QemItem s;
   uint32_t open;
   if (!qem_screen_short_row(v, i, lab, connexity, constrained, nb, &s, &open)) return false;
-  const double Ta_new = constrained ? qem_leave_T(v, i, lab, sc, level) : 0.0;
+  const double Ta_new = qem_leave_T(v, i, lab, sc, level, constrained);
   int32_t target = kNull;
   double best_d = 0.0, best_m = 0.0;
   while (open) {
@@ -723,7 +773,7 @@ ACVD_HD bool qem_propose_short_row(const SweepView& v, int64_t i, int32_t lab, i
     const int32_t b = ring_label(nb, p);
     const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
     const double Tb = constrained ? v.T[b] : 0.0;
-    const double Tb_new = constrained ? qem_join_T(v, i, b, sc, level) : 0.0;
+    const double Tb_new = qem_join_T(v, i, b, sc, level, constrained);
     const double d = qem_delta(s.pw, s.ta, tb, Ta_new, s.Ta, Tb_new, Tb);
     if (target == kNull || d < best_d || (d == best_d && b < target)) {
       best_d = d;
@@ -768,21 +818,23 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
     *target_out = target;
     return target != kNull;
   }
-  // QEM with active constraints: the T terms take part.  Without them (the
-  // unconstrained initialisation, vtkUniformClustering.h:683-688) the QEM
-  // move energy is w (tb - ta), the isotropic one, and the isotropic rule
-  // below decides; METRIC 2 is that case alone (QEM meshes swept while none
-  // is constrained), compiled without the FP64 solves.
+  // QEM with active constraints (constrained = qem_mode: 1, or 2 when only
+  // anchored clusters carry a T): the T terms take part.  Without them (the
+  // unconstrained initialisation of a mesh without fixed clusters,
+  // vtkUniformClustering.h:683-688) the QEM move energy is w (tb - ta), the
+  // isotropic one, and the isotropic rule below decides; METRIC 2 is that
+  // case alone (QEM meshes swept while none is constrained), compiled without
+  // the FP64 solves.
   if (METRIC == 1 && constrained) {
     if (kShortRows && r1 - r0 <= kRingCache)
       return qem_propose_short_row(v, i, lab, connexity, constrained, level, sc, target_out);
     // QEM: screen, solves, choice
     QemItem s;
     if (!qem_screen(v, i, lab, connexity, constrained, &s)) return false;
-    const double Ta_new = constrained ? qem_leave_T(v, i, lab, sc, level) : 0.0;
+    const double Ta_new = qem_leave_T(v, i, lab, sc, level, constrained);
     return qem_choose(
-        v, lab, s, constrained, Ta_new,
-        [&](int32_t, int32_t b) { return constrained ? qem_join_T(v, i, b, sc, level) : 0.0; }, target_out);
+        v, lab, s, constrained, Ta_new, [&](int32_t, int32_t b) { return qem_join_T(v, i, b, sc, level, constrained); },
+        target_out);
   }
   if (size_a <= 1) return false;
   if (connexity && ring_check(v, i, lab)) return false;
@@ -829,20 +881,34 @@ struct MeshState {
   int32_t last_disconnected;
   int32_t pass_props;       // proposals (improving moves found) of the current pass
   int32_t first_rounds;     // accept rounds per pass until the first convergence (1..kMatchRounds)
+  int32_t anchored;         // QEM mesh with fixed clusters (anchored cluster points in every phase)
 };
+
+// T terms of a mesh's QEM rule: 1 every cluster point solved (constrained),
+// 2 only anchored clusters carry one, 0 none (the isotropic rule decides).
+ACVD_HD int qem_mode(const MeshState& s) { return s.constrained ? 1 : (s.anchored ? 2 : 0); }
 
 // Accept rounds of the mesh's current pass.  Until the first convergence a
 // pass runs first_rounds rounds, afterwards kMatchRounds.  The phase before
-// the first convergence decides which local minimum the run reaches: with
-// one round per pass every cluster moves at most one item per pass and the
-// clusters relax evenly, as in the reference's loop where each cut edge is
-// visited once; with twelve the moves concentrate on the proposals of the
-// pass start.  Measured on the replay against the serial reference (same
-// initial labels, eight accept seeds): icosphere 40,962 vertices, K = 2,000,
-// seed vertices: gap median +0.02 % (range -0.14 .. +0.24) with 12 rounds,
-// -0.10 % (-0.19 .. +0.04) with 1; cube 960,002 vertices, K = 20,000, QEM:
-// +0.01 .. +0.03 % with 12, -0.001 .. +0.002 % with 1.
-constexpr int32_t kDefaultFirstRounds = 1;
+// the first convergence decides which local minimum the run reaches and most
+// of its moves.  With one round per pass every cluster moves at most one item
+// per pass and the clusters relax evenly; with twelve the moves of a pass
+// follow the proposals of its start and far fewer passes are needed.
+// Measured on the replay against the serial reference (same initial labels):
+//   icosphere 40,962 vertices, K = 2,000, isotropic, seed-vertex start, eight
+//   accept seeds: gap median +0.02 % (range -0.14 .. +0.24) with 12 rounds,
+//   -0.10 % (-0.19 .. +0.04) with 1; 44 against 112 passes;
+//   cube 960,002 vertices, K = 20,000, QEM: +0.01 .. +0.03 % with 12,
+//   -0.001 .. +0.002 % with 1;
+//   graded QEM torus 524,288 vertices, K = 6,250: -0.04 / +0.01 % with 12
+//   rounds (173 passes), +0.02 / -0.03 % with 1 (~700 passes);
+//   on the GPU the 8,388,608-vertex torus (K = 100,000) takes 384 passes and
+//   239 ms with 12 rounds, 1,556 passes and 403 ms with 1.
+// So the default is one round for meshes of at most kFirstRoundsAutoItems
+// items, where the distinct local minima differ most and a pass costs little,
+// and twelve above, where the gap is within noise and the passes dominate.
+constexpr int32_t kFirstRoundsAutoItems = int32_t(1) << 20;
+ACVD_HD int32_t auto_first_rounds(int64_t n_items) { return n_items <= kFirstRoundsAutoItems ? 1 : kMatchRounds; }
 
 ACVD_HD int32_t pass_rounds(const MeshState& s) { return s.convergences == 0 ? s.first_rounds : kMatchRounds; }
 

@@ -70,6 +70,7 @@ struct HostMesh {
   std::vector<double> indicator;  // set by the caller (SetCurvatureInfo)
   std::vector<double> curvature;  // computed by the last build (curvature_indicator)
   std::vector<int32_t> init_labels;
+  std::vector<int32_t> fixed;  // SetFixedClusters: cluster k < fixed.size() holds item fixed[k]
   acvd_params params;
   // layout in the packed context
   int64_t item_off = 0, clus_off = 0, face_off = 0, n_adj = 0, adj_off = 0;
@@ -105,6 +106,8 @@ struct acvd_ctx {
   DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
   DBuf<unsigned long long> d_acc_ws, d_acc_q, d_prio, d_ring_adj;
   DBuf<ClusterGeom> d_geom;
+  DBuf<int32_t> d_anchor;  // per cluster: global id of its fixed item or -1
+  bool any_fixed = false;
   DBuf<double> d_T, d_evec;
   DBuf<int32_t> d_cand_tgt, d_cand_lab, d_cand_since, d_cl_round;
   DBuf<int32_t> d_work, d_work_count, d_cand, d_seg_cand, d_moved, d_seg_moved;
@@ -163,6 +166,8 @@ struct acvd_ctx {
     a.acc_ws = d_acc_ws.p;
     a.acc_q = any_qem ? d_acc_q.p : nullptr;
     a.geom = d_geom.p;
+    a.anchor = any_fixed ? d_anchor.p : nullptr;
+    a.count_rounds = timing ? 1 : 0;
     a.T = any_qem ? d_T.p : nullptr;
     a.evec = any_qem ? d_evec.p : nullptr;
     a.sizes = d_sizes.p;
@@ -491,6 +496,11 @@ int build(acvd_ctx* c) {
   }
   HIP_TRY(c->d_prio.alloc(K));
   HIP_TRY(c->d_geom.alloc(K));
+  // fixed items per cluster (a change of the fixed clusters rebuilds the
+  // context, so the captured chunks never see a stale pointer)
+  c->any_fixed = false;
+  for (auto& h : c->meshes) c->any_fixed |= !h.fixed.empty();
+  if (c->any_fixed) HIP_TRY(c->d_anchor.alloc(K));
   HIP_TRY(c->d_cl_round.alloc(K));
   {
     // work slices: scan tiles are dealt round-robin to the slices
@@ -658,7 +668,8 @@ int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles, bool short_
 // QEM meshes then take the isotropic rule and every derive is a centroid.
 bool qem_unconstrained(const acvd_ctx* c) {
   for (size_t m = 0; m < c->meshes.size(); ++m)
-    if (c->meshes[m].params.metric == ACVD_METRIC_QEM && c->h_state[m].constrained) return false;
+    if (c->meshes[m].params.metric == ACVD_METRIC_QEM && (c->h_state[m].constrained || c->h_state[m].anchored))
+      return false;  // an anchored mesh carries T terms in every phase
   return true;
 }
 
@@ -677,6 +688,24 @@ void print_pass_log(const Ctrl& ctrl, unsigned long long* printed) {
   *printed = ctrl.passes;
 }
 
+// Fixed items per cluster on the device (global ids; -1 for free clusters).
+int upload_anchors(acvd_ctx* c) {
+  if (!c->any_fixed) return ACVD_OK;
+  std::vector<int32_t> an(size_t(c->K), -1);
+  for (auto& h : c->meshes)
+    for (size_t k = 0; k < h.fixed.size(); ++k) an[size_t(h.clus_off) + k] = int32_t(h.item_off + h.fixed[k]);
+  HIP_TRY(hipMemcpy(c->d_anchor.p, an.data(), size_t(c->K) * sizeof(int32_t), hipMemcpyHostToDevice));
+  return ACVD_OK;
+}
+
+// The point of a fixed QEM cluster is its item's input position, exactly
+// (GetClusterCentroid of an anchored cluster, vtkQEMetricForClustering.h:270-274).
+void exact_anchor_points(const HostMesh& h, double* xyz) {
+  if (h.params.metric != ACVD_METRIC_QEM) return;
+  for (size_t k = 0; k < h.fixed.size(); ++k)
+    for (int d = 0; d < 3; ++d) xyz[3 * k + d] = h.xyz[3 * size_t(h.fixed[k]) + d];
+}
+
 int minimize_energy(acvd_ctx* c) {
   hipStream_t s = c->stream;
   const int M = int(c->meshes.size());
@@ -693,7 +722,8 @@ int minimize_energy(acvd_ctx* c) {
     st.max_loops = std::min<int32_t>(p.max_number_of_loops, kMaxLoops - 2);
     st.max_convergences = p.max_number_of_convergences;
     st.unconstrained_init = p.unconstrained_initialization;
-    st.first_rounds = p.first_phase_rounds > 0 ? p.first_phase_rounds : kDefaultFirstRounds;
+    st.anchored = p.metric == ACVD_METRIC_QEM && !c->meshes[m].fixed.empty();
+    st.first_rounds = p.first_phase_rounds > 0 ? p.first_phase_rounds : auto_first_rounds(c->meshes[m].nv);
     // ProcessClustering: SetConstrainedClustering(0) when unconstrained
     // initialisation is requested (vtkUniformClustering.h:683-688)
     st.constrained = p.unconstrained_initialization ? 0 : 1;
@@ -815,7 +845,7 @@ int initial_labels(acvd_ctx* c) {
         const std::vector<double>& ind = !h.indicator.empty() ? h.indicator : h.curvature;
         if (!reference_initial_sampling(h.nv, h.xyz.data(), h.nf, h.tri.data(), h.params.metric,
                                         ind.empty() ? nullptr : ind.data(), h.params.gradation, K,
-                                        lab.data()))
+                                        lab.data(), int64_t(h.fixed.size()), h.fixed.data()))
           return fail(ACVD_ERR_STATE, "initial sampling failed");
         for (int64_t i = 0; i < h.nv; ++i) lab[i] = lab[i] == K ? kNull : int32_t(lab[i] + h.clus_off);
       } else {
@@ -974,6 +1004,22 @@ int acvd_set_initial_labels(acvd_ctx* ctx, int32_t mesh_id, const int32_t* label
   return ACVD_OK;
 }
 
+int acvd_set_fixed_clusters(acvd_ctx* ctx, int32_t mesh_id, int64_t n, const int32_t* items) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (n < 0 || (n > 0 && !items)) return fail(ACVD_ERR_ARG, "null fixed-item list");
+  HostMesh& h = ctx->meshes[mesh_id];
+  std::vector<int32_t> ids(items, items + n);
+  for (int32_t v : ids)
+    if (v < 0 || v >= h.nv) return fail(ACVD_ERR_ARG, "fixed item out of range");
+  std::vector<int32_t> sorted(ids);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    return fail(ACVD_ERR_ARG, "an item is fixed to two clusters");
+  h.fixed.swap(ids);
+  ctx->invalidate();
+  return ACVD_OK;
+}
+
 int acvd_build(acvd_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   return build(ctx);
@@ -982,11 +1028,18 @@ int acvd_build(acvd_ctx* ctx) {
 int acvd_process_clustering(acvd_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   if (ctx->meshes.empty()) return fail(ACVD_ERR_STATE, "no input mesh");
-  for (auto& h : ctx->meshes)
+  for (auto& h : ctx->meshes) {
     if (h.params.number_of_clusters <= 0) return fail(ACVD_ERR_ARG, "must set NumberOfClusters to more than zero");
+    if (int64_t(h.fixed.size()) > h.params.number_of_clusters)
+      return fail(ACVD_ERR_ARG, "more fixed clusters than clusters");
+    if (!h.fixed.empty() && h.params.initial_sampling == 1 && h.init_labels.empty())
+      return fail(ACVD_ERR_ARG, "fixed clusters need the reference's initial sampling (0) or initial labels");
+  }
   int rc = build(ctx);
   if (rc) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  rc = upload_anchors(ctx);
+  if (rc) return rc;
   const int M = int(ctx->meshes.size());
   // (re)allocate the pinned state mirror for M meshes
   if (ctx->h_state) (void)hipHostFree(ctx->h_state);
@@ -1025,6 +1078,11 @@ int acvd_process_clustering(acvd_ctx* ctx) {
     S.ctx_evaluated = int64_t(ctx->last_ctrl.evaluated);
     S.ctx_proposed = int64_t(ctx->last_ctrl.proposed);
     S.ctx_accepted = int64_t(ctx->last_ctrl.accepted);
+    S.ctx_rounds = int64_t(ctx->last_ctrl.rounds);
+    S.ctx_walked = int64_t(ctx->last_ctrl.walked);
+    S.ctx_reeval_live = int64_t(ctx->last_ctrl.reeval_live);
+    S.ctx_reevaluated = int64_t(ctx->last_ctrl.reevaluated);
+    S.ctx_accepted_constrained = int64_t(ctx->last_ctrl.accepted_constrained);
     S.converge_ms = ms;
     S.build_ms = ctx->build_ms;
     S.n_items = h.nv;
@@ -1103,6 +1161,7 @@ int acvd_build_output(acvd_ctx* ctx, int32_t mesh_id, int32_t quadrics_level) {
   if (nf_out)
     HIP_TRY(hipMemcpyAsync(h.out_tri.data(), otri.p, 3 * size_t(nf_out) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  exact_anchor_points(h, h.out_xyz.data());
   return ACVD_OK;
 }
 
@@ -1156,6 +1215,7 @@ int acvd_get_cluster_points(acvd_ctx* ctx, int32_t mesh_id, double* xyz) {
   HIP_TRY(hipMemcpyAsync(xyz, d.p, 3 * size_t(h.params.number_of_clusters) * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  exact_anchor_points(h, xyz);
   return ACVD_OK;
 }
 

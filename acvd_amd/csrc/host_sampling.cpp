@@ -40,7 +40,8 @@ double face_area(const double* xyz, const int32_t* t) {
 }  // namespace
 
 bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
-                                const double* indicator, double gradation, int32_t K, int32_t* labels) {
+                                const double* indicator, double gradation, int32_t K, int32_t* labels,
+                                int64_t n_fixed, const int32_t* fixed) {
   // vertex -> faces, increasing face id (faces with a repeated first edge
   // vertex are inactive in the reference and skipped)
   std::vector<int64_t> vfptr(nv + 1, 0);
@@ -137,6 +138,8 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
   // region growing
   const int64_t n = nv;
   for (int64_t i = 0; i < n; ++i) labels[i] = K;
+  const int32_t offset = int32_t(n_fixed);
+  for (int32_t k = 0; k < offset; ++k) labels[fixed[k]] = k;
   std::vector<int32_t> items(n);
   for (int64_t i = 0; i < n; ++i) items[i] = int32_t(i);
   std::mt19937 rng;
@@ -146,7 +149,7 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
   for (int64_t i = 0; i < n; ++i) sw += w[i];
   const double target = sw / double(K);
   int64_t remaining_items = n;
-  int32_t remaining_regions = K;
+  int32_t remaining_regions = K - offset;
   int64_t first = 0;
   std::deque<int32_t> queue;
   while (remaining_items > 0 && remaining_regions > 0) {
@@ -167,7 +170,7 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
       const int32_t it = queue.front();
       queue.pop_front();
       if (labels[it] != K) continue;
-      labels[it] = remaining_regions;
+      labels[it] = remaining_regions + offset;
       sw += w[it];
       --remaining_items;
       for (int64_t r = rptr[it]; r < rptr[it + 1]; ++r) queue.push_back(ring[r]);
@@ -191,14 +194,14 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
       const int32_t c = labels[item];
       if (c == K) break;
       if (size[c] == 1) continue;
-      labels[item] = remaining_regions;
+      labels[item] = remaining_regions + offset;
       size[c]--;
-      size[remaining_regions]++;
+      size[remaining_regions + offset]++;
       break;
     }
     if (item < 0) return false;
     --remaining_regions;
-    labels[item] = remaining_regions;
+    labels[item] = remaining_regions + offset;
   }
   return true;
 }

@@ -138,7 +138,7 @@ __device__ inline void kept_set(const double lam[3], int level, bool keep[3]) {
         idx = k;
       }
     if (idx < 0) break;
-    if (mx > 0.0 && best / mx > kThreshold && left > 0) {
+    if (mx > 0.0 && best * (1.0 / mx) > kThreshold && left > 0) {
       keep[idx] = true;
       --left;
     } else {

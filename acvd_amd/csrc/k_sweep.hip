@@ -114,7 +114,12 @@ __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm,
   a.geom[c] = g;
   if (dm.metric == 1) {
     double T = 0.0;
-    if (constrained) {
+    const int32_t an = a.anchor ? a.anchor[c] : -1;
+    if (an >= 0) {  // a fixed cluster: its point is its anchor item's position
+      const float4 p = a.xw[an];
+      const double ap[3] = {double(p.x), double(p.y), double(p.z)};
+      T = anchored_T(g, ap);
+    } else if (constrained) {
       int64_t q10[10];
       for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
       double V0[9];
@@ -153,6 +158,7 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   v.stamp = a.stamp;
   v.cand_pass = a.cand_pass;
   v.ring_adj = a.ring_adj;
+  v.anchor = a.anchor;
   return v;
 }
 
@@ -647,10 +653,17 @@ __device__ bool qem_pool_choose(const SweepArgs& a, const SweepView& v, QemPool*
         const int32_t c = tc < 0 ? ~tc : tc;
         const DevMesh& dm = a.meshes[tm];
         const Scales sc = dm.scales;
-        int64_t iw, is[3], iq[10], ws[4], q10[10];
-        item_fixed(v, ti, sc, &iw, is, iq);
-        load_acc(v, c, ws, q10);
-        pool->Tn[lane] = qem_T_trial(ws, q10, iw, is, iq, tc < 0 ? -1 : 1, sc, dm.level, v.evec + 9 * int64_t(c));
+        double ap[3];
+        const bool an = anchor_pos(v, c, ap);
+        double Tn = 0.0;  // a centroid point (an unconstrained mesh's unanchored cluster)
+        if (an || a.state[tm].constrained) {
+          int64_t iw, is[3], iq[10], ws[4], q10[10];
+          item_fixed(v, ti, sc, &iw, is, iq);
+          load_acc(v, c, ws, q10);
+          Tn = qem_T_trial(ws, q10, iw, is, iq, tc < 0 ? -1 : 1, sc, dm.level, v.evec + 9 * int64_t(c),
+                           an ? ap : nullptr);
+        }
+        pool->Tn[lane] = Tn;
         if (tc >= 0) {
           pool->tb[lane] =
               join_term(v.xw[4 * ti], v.xw[4 * ti + 1], v.xw[4 * ti + 2], v.xw[4 * ti + 3], v.geom[tc]);
@@ -727,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
         const MeshState* st = &a.state[m];
         L = st->loops;
         lab = a.labels[i];
-        if (METRIC == 1 && lab != kNull && st->constrained)
+        if (METRIC == 1 && lab != kNull && qem_mode(*st))
           kind = pool_screen(v, i, lab, st->connexity, nb, &s, &open, &ntask);
         else  // no FP64 solve: the null-item rule or the isotropic one
           want = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st->connexity, st->constrained, dm.level,
@@ -767,9 +780,22 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
 // Re-evaluation of candidate entry e (when valid) for round `round`; every
 // lane of the wave calls it (the QEM solve pool is shared by the wave).
 // reg(c, key) registers an accept key at cluster c.
+// Block sum of v (the result is valid in thread 0; synchronises the block).
+__device__ inline int block_sum(int v) {
+  __shared__ int s_red[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (__lane_id() == 0) s_red[threadIdx.x / 64] = v;
+  __syncthreads();
+  int tot = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
+  __syncthreads();
+  return tot;  // valid in thread 0
+}
+
 template <int METRIC, class Reg>
 __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemPool* pool, bool valid, int64_t e,
-                                    int round, Reg&& reg) {
+                                    int round, Reg&& reg, int* n_live, int* n_stale) {
   // the candidate record in one round trip
   const int32_t b = valid ? a.cand_tgt[e] : -1;
   const int32_t i_e = valid ? a.cand[e] : 0;
@@ -797,7 +823,7 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
       if (stale) {
         a.cand_since[e] = st.loops * kMatchRounds + round;
         target = kNull;
-        use_pool = METRIC == 1 && lab != kNull && st.constrained;
+        use_pool = METRIC == 1 && lab != kNull && qem_mode(st);
         if (use_pool)
           kind = pool_screen(v, i, lab, st.connexity, nb, &s, &open, &ntask);
         else  // no FP64 solve: the null-item rule or the isotropic one
@@ -819,6 +845,8 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
     if (lab != kNull) reg(lab, key);
     reg(target, key);
   }
+  *n_live += live;
+  *n_stale += stale;
 }
 
 // Re-evaluate (round r > 0): a live candidate (cand_tgt >= 0) whose own
@@ -837,11 +865,23 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   if (round_was_empty(a, round - 1) || past) return;
   const SweepView v = make_view(a);
   QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
+  int n_live = 0, n_stale = 0;
   walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
-    reeval_entry<METRIC>(a, v, pool, valid, e, round,
-                         [&](int32_t c, unsigned long long key) { keyagg_min(&agg, c, key); });
+    reeval_entry<METRIC>(
+        a, v, pool, valid, e, round, [&](int32_t c, unsigned long long key) { keyagg_min(&agg, c, key); },
+        &n_live, &n_stale);
     keyagg_flush(&agg, a.prio);
   });
+  // candidates registered and re-evaluated in this round (statistics of the
+  // timed runs: the block sums cost a few barriers per round)
+  if (a.count_rounds) {
+    const int tl = block_sum(n_live);
+    const int ts = block_sum(n_stale);
+    if (threadIdx.x == 0) {
+      if (tl) atomicAdd(&a.ctrl->live[round][blockIdx.x % kSlices], tl);
+      if (ts) atomicAdd(&a.ctrl->stale[round][blockIdx.x % kSlices], ts);
+    }
+  }
 }
 
 // --------------------------------------------------------------------- apply
@@ -868,17 +908,6 @@ __device__ inline void tile_insert(const SweepArgs& a, int32_t t, int32_t c) {
   if (slot < kTileSet) a.tile_set[int64_t(t) * kTileSet + slot] = c;
 }
 
-__device__ inline int block_sum(int v) {
-  __shared__ int s_red[kBlock / 64];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if (__lane_id() == 0) s_red[threadIdx.x / 64] = v;
-  __syncthreads();
-  int tot = 0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < kBlock / 64; ++w) tot += s_red[w];
-  __syncthreads();
-  return tot;  // valid in thread 0
-}
 
 // Apply round `round`: a live candidate holding the minimum round key at both
 // of its clusters moves: label, fixed-point accumulator deltas (plain
@@ -1071,11 +1100,20 @@ __global__ void k_controller(SweepArgs a) {
       s_part[1][threadIdx.x >> 6] = p;
     }
   }
+  __shared__ int s_live, s_stale;
   if (threadIdx.x < 64) {  // lane k reads slice k of the move counters, wave sums
     int dr[kMatchRounds];
-    for (int r = 0; r < kMatchRounds; ++r) dr[r] = a.ctrl->mods[r][threadIdx.x];
-    for (int off = 32; off > 0; off >>= 1)
+    int lv = 0, st = 0;
+    for (int r = 0; r < kMatchRounds; ++r) {
+      dr[r] = a.ctrl->mods[r][threadIdx.x];
+      lv += a.ctrl->live[r][threadIdx.x];
+      st += a.ctrl->stale[r][threadIdx.x];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
       for (int r = 0; r < kMatchRounds; ++r) dr[r] += __shfl_xor(dr[r], off);
+      lv += __shfl_xor(lv, off);
+      st += __shfl_xor(st, off);
+    }
     if (threadIdx.x == 0) {
       int d = 0;
       for (int r = 0; r < kMatchRounds; ++r) {
@@ -1083,6 +1121,8 @@ __global__ void k_controller(SweepArgs a) {
         d += dr[r];
       }
       s_mods = d;
+      s_live = lv;
+      s_stale = st;
     }
   }
   __syncthreads();
@@ -1098,10 +1138,14 @@ __global__ void k_controller(SweepArgs a) {
   __syncthreads();
   bool ran = false;
   __shared__ int s_flags;  // phase flags of mesh 0 during this pass
+  __shared__ int s_rounds; // most accept rounds a sweeping mesh ran this pass
+  if (threadIdx.x == 0) s_rounds = 1;
+  __syncthreads();
   for (int m = threadIdx.x; m < a.n_meshes; m += blockDim.x) {
     MeshState s = a.state[m];
     if (m == 0) s_flags = (s.constrained ? 1 : 0) | (s.connexity ? 2 : 0);
     if (s.phase == kPhaseSweep) {
+      atomicMax(&s_rounds, pass_rounds(s));
       ran = true;
       if (!a.item_mesh) {  // one mesh: the sliced counters are its own
         s.pass_props = s_props;
@@ -1116,7 +1160,11 @@ __global__ void k_controller(SweepArgs a) {
   if (ran) atomicOr(&any_sweep, 2);
   __syncthreads();
   if (threadIdx.x < kSlices)
-    for (int r = 0; r < kMatchRounds; ++r) a.ctrl->mods[r][threadIdx.x] = 0;
+    for (int r = 0; r < kMatchRounds; ++r) {
+      a.ctrl->mods[r][threadIdx.x] = 0;
+      a.ctrl->live[r][threadIdx.x] = 0;
+      a.ctrl->stale[r][threadIdx.x] = 0;
+    }
   if (threadIdx.x == 0 && (any_sweep & 2)) {
     const unsigned long long pass = a.ctrl->passes;
     int32_t* e = a.ctrl->pass_log[pass % kPassLogCap];
@@ -1131,6 +1179,18 @@ __global__ void k_controller(SweepArgs a) {
     a.ctrl->evaluated += (unsigned long long)s_work;
     a.ctrl->proposed += (unsigned long long)s_props;
     a.ctrl->accepted += (unsigned long long)s_mods;
+    if (s_flags & 1) a.ctrl->accepted_constrained += (unsigned long long)s_mods;
+    // accept rounds that walked the candidate list: round 0, and each later
+    // round whose predecessor applied a move (a round after one without moves
+    // exits at once); the re-evaluate kernels walked the list in all but one
+    if (any_sweep & 2) {
+      int walked = 1;
+      while (walked < s_rounds && s_round_mods[walked - 1] > 0) ++walked;
+      a.ctrl->rounds += (unsigned long long)walked;
+      a.ctrl->walked += (unsigned long long)s_props * (unsigned long long)(2 * walked - 1);
+      a.ctrl->reeval_live += (unsigned long long)s_live;
+      a.ctrl->reevaluated += (unsigned long long)s_stale;
+    }
     a.ctrl->any_sweep = any_sweep & 1;
     a.ctrl->seg_walk = s_props <= kSegWalkMax ? 1 : 0;
     a.ctrl->any_need_clean = any_clean;
@@ -1224,8 +1284,17 @@ __global__ void k_cc_best(SweepArgs a, const int32_t* __restrict__ mask, const i
   if (i >= a.n_items || !mask[mesh_of_item(a, i)]) return;
   const int32_t lab = a.labels[i];
   if (lab == kNull || parent[i] != i) return;
-  // larger component first, then the smaller first item
-  const unsigned long long key = (uint64_t(uint32_t(comp_size[i])) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(i));
+  // the component holding the cluster's fixed item first (its size counts 1e9
+  // in CleanClustering, vtkUniformClustering.h:441-447), then the larger
+  // component, then the smaller first item
+  uint32_t size = uint32_t(comp_size[i]);
+  const int32_t an = a.anchor ? a.anchor[lab] : -1;
+  if (an >= 0) {
+    int32_t r = an;
+    while (parent[r] != r) r = parent[r];  // read-only find (the hooks are done)
+    if (r == int32_t(i)) size |= 0x80000000u;
+  }
+  const unsigned long long key = (uint64_t(size) << 32) | uint64_t(0xFFFFFFFFu - uint32_t(i));
   atomicMax(&best[lab], key);
   atomicAdd(&ncomp[lab], 1);
 }
@@ -1292,8 +1361,14 @@ __global__ void k_cluster_points(SweepArgs a, int32_t mesh, double* pts) {
   const ClusterGeom g = geom_from_fixed(W, ld_i64(&a.acc_ws[4 * c + 1]), ld_i64(&a.acc_ws[4 * c + 2]),
                                         ld_i64(&a.acc_ws[4 * c + 3]), dm.scales);
   double q[3] = {g.cx, g.cy, g.cz};
+  const int32_t an = a.anchor && dm.metric == 1 ? a.anchor[c] : -1;
   if (!(g.W > 0.0)) {
     q[0] = q[1] = q[2] = __builtin_nan("");
+  } else if (an >= 0) {  // fixed cluster: the anchor (exact input coordinates are set by the host)
+    const float4 p = a.xw[an];
+    q[0] = p.x;
+    q[1] = p.y;
+    q[2] = p.z;
   } else if (dm.metric == 1 && a.state[mesh].constrained) {
     int64_t q10[10];
     for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);

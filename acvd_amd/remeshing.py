@@ -122,6 +122,11 @@ class Context:
             raise ValueError("initial clustering length must equal the number of vertices")
         check(N.lib().acvd_set_initial_labels(self._h, mesh_id, _p(lab)))
 
+    def set_fixed_clusters(self, mesh_id, items):
+        """SetFixedClusters + AnchorItem (cluster k fixed at item items[k])."""
+        ids = np.zeros(0, np.int32) if items is None else np.ascontiguousarray(items, np.int32).reshape(-1)
+        check(N.lib().acvd_set_fixed_clusters(self._h, mesh_id, len(ids), _p(ids) if len(ids) else None))
+
     def indicator(self, mesh_id=0):
         """The indicator the last build weighted the mesh with: the one set,
         else the computed curvature indicator (params curvature_indicator)."""
@@ -246,6 +251,8 @@ class _DiscreteRemeshing:
                         "gradation": 0.0}
         self._indicator = None
         self._initial = None
+        self._fixed = None
+        self._anchors = {}
         self._clustering = None
         self._ctx = None
         self._subsampling_threshold = 10
@@ -291,6 +298,18 @@ class _DiscreteRemeshing:
     def SetInitialClustering(self, labels):
         self._initial = None if labels is None else np.ascontiguousarray(labels, np.int32)
 
+    def SetFixedClusters(self, ids):
+        """vtkUniformClustering::SetFixedClusters (vtkUniformClustering.h:134):
+        cluster k starts at item ids[k] and keeps it (NumberOfClusters counts
+        these clusters).  With the QEM metric the item also becomes the
+        cluster's AnchorItem, as ACVDQ sets it (Examples/ACVDQ.cxx:327-334)."""
+        self._fixed = None if ids is None else np.ascontiguousarray(ids, np.int32).reshape(-1)
+        self._anchors = {}
+
+    def GetCluster(self, i):
+        """The cluster record, for its AnchorItem (the fixed item, or -1)."""
+        return _ClusterProxy(self, int(i))
+
     def SetSubsamplingThreshold(self, t):
         self._subsampling_threshold = int(t)
 
@@ -327,6 +346,7 @@ class _DiscreteRemeshing:
         ctx.set_params(0, curvature_indicator=int(curvature_indicator), **self._params)
         ctx.set_indicator(0, self._indicator)
         ctx.set_initial_labels(0, self._initial)
+        ctx.set_fixed_clusters(0, self._fixed)
         ctx.process()
         self._clustering = ctx.labels(0)
         return self._clustering
@@ -371,6 +391,26 @@ class _DiscreteRemeshing:
 
     def GetContext(self):
         return self._ctx
+
+
+class _ClusterProxy:
+    """GetCluster(i): its AnchorItem is the fixed item of cluster i (the
+    reference sets it to the same item, Examples/ACVDQ.cxx:333-334); another
+    value than that item is refused."""
+
+    def __init__(self, owner, i):
+        self._owner = owner
+        self._i = i
+
+    @property
+    def AnchorItem(self):
+        f = self._owner._fixed
+        return int(f[self._i]) if f is not None and self._i < len(f) else -1
+
+    @AnchorItem.setter
+    def AnchorItem(self, item):
+        if int(item) != self.AnchorItem:
+            raise AcvdError("AnchorItem must be the cluster's fixed item (SetFixedClusters)")
 
 
 class IsotropicDiscreteRemeshing(_DiscreteRemeshing):

@@ -8,23 +8,27 @@ initialisation (ACVDQ).  Synthetic, generated from seed 3.
 
 A step is one full clustering convergence (MinimizeEnergy: all passes, cleans
 and fills until every stopping rule is met) from the same initial labelling
-(the reference's region growing), with the mesh, CSR and items resident in
-HBM.  `value` is
-the median device time of the timed steps in ms (HIP events inside the
-library; 5 steps after 1 warm-up by default); `ms_per_step` the wall time per
-step around the call.
+(the reference's region growing, then the device fill), with the mesh, CSR
+and items resident in HBM.  `value` is the median device time of the timed
+steps in ms (HIP events inside the library; 5 steps after 1 warm-up by
+default); `ms_per_step` the wall time per step around the call; `init_ms`
+the wall time of the initial sampling (host region growing + device fill),
+paid once per mesh and outside `value`.
 
-Extra legs:
-  roofline     the dominant sweep kernel's algorithmic bytes per launch over its
-               average launch duration (HIP events, one extra timed step);
+Extra legs (all on by default):
+  roofline     per kernel group (scan, evaluate, accept rounds) the
+               algorithmic bytes of one clustering over its device time (HIP
+               events, one extra timed step); `roofline` is the group with the
+               most time;
   cpu_baseline the threaded restatement of the reference (ACVDQP,
                vtkThreadedClustering) on all host cores, from the same initial
-               labels, on the full workload (or a 2048 x 1024 torus);
-               --gap adds the serial restatement (ACVDQ) for the energy gap;
-  batched      (--batch) 384 star-shaped icospheres (levels 4-8, iso/QEM 50/50,
+               labels, on the full workload; the serial restatement (ACVDQ,
+               the parity reference) runs beside the GPU steps on one core and
+               gives the energy gap;
+  batched      384 star-shaped icospheres (levels 4-8, iso/QEM 50/50,
                K = V/15) packed in one context vs clustered one by one;
-  energy       (--energy) fixed-assignment energy forward and forward+backward
-               on 16 QEM tori 1024 x 1024 (K = 25,000 each) with an FD check.
+  energy       fixed-assignment energy forward and forward+backward on 16 QEM
+               tori 1024 x 1024 (K = 25,000 each) with an FD check.
 """
 import argparse
 import json
@@ -81,6 +85,54 @@ def eval_bytes_per_launch(E2, N, K, work_per_launch, proposed_per_launch):
     return work_per_launch * (72.0 + 4.0 * E2 / N) + 156.0 * touched + 28.0 * proposed_per_launch
 
 
+def accept_bytes(st, N, E2):
+    """Algorithmic bytes of the accept rounds of one clustering, from the
+    device counters (all passes):
+      4 B per candidate entry a round kernel reads (its target word; dead
+        entries included: ctx_walked);
+      24 B per live entry of an apply round (item and label words, the
+        priority words of both clusters): the proposals (round 0) and the
+        candidates the re-evaluate rounds registered again (ctx_reeval_live);
+      36 B per such registration (item, label and stamp words, the round tags
+        of both clusters, two 8-B keys);
+      per re-evaluated candidate an evaluation: 72 B of item record and row
+        bounds, 8 B per CSR entry (neighbour id and label), three cluster
+        records of 156 B;
+      per move 252 B (label, target and moved-list words; per cluster the
+        fixed-point W and S read and written, size, stamp and round tag read
+        and written, the derived centroid and W written) and, while the QEM
+        constraints are on, 480 B more (per cluster the ten quadric
+        accumulators read and written, T and the eigenvectors written)."""
+    deg = E2 / N
+    return (4.0 * st["ctx_walked"] + 24.0 * (st["ctx_proposed"] + st["ctx_reeval_live"])
+            + 36.0 * st["ctx_reeval_live"] + (72.0 + 8.0 * deg + 3 * 156.0) * st["ctx_reevaluated"]
+            + 252.0 * st["ctx_accepted"] + 480.0 * st["ctx_accepted_constrained"])
+
+
+def serial_in_background(xyz, tri, H, K, init):
+    """Oracle A (the serial restatement of ACVDQ) on one host core, in a
+    thread (the C call releases the GIL) while the GPU steps run.  It prints
+    one line per loop on stderr."""
+    import threading
+    from oracle import pyoracle as po
+    box = {}
+
+    def work():
+        try:
+            t = time.time()
+            m = po.RefMesh(xyz, tri)
+            box["mesh_s"] = time.time() - t
+            po.set_progress(True)
+            box["labels"], box["stats"] = m.serial_run(po.QEM, K, 1.5, H, 3, 1, init)
+            box["mesh"] = m
+        except Exception as e:  # reported by the caller
+            box["error"] = e
+
+    th = threading.Thread(target=work, daemon=True)
+    th.start()
+    return th, box
+
+
 def run_gpu_leg(args):
     import acvd_amd
     xyz, tri, H, K = torus_workload(4096, 2048, 100000)
@@ -90,9 +142,13 @@ def run_gpu_leg(args):
     t = time.time()
     ctx.build()
     log(f"device build (CSR + items + scales): {time.time() - t:.2f} s wall")
+    t = time.perf_counter()
     ctx.initial_sampling()
+    init_ms = (time.perf_counter() - t) * 1e3
     init = ctx.labels(0)
+    log(f"initial sampling (host region growing + device fill): {init_ms:.0f} ms wall")
     ctx.set_initial_labels(0, init)
+    serial = None if args.skip_cpu else serial_in_background(xyz, tri, H, K, init)
     for w in range(args.warmup):
         ctx.process()
         log(f"warmup {w}: {ctx.stats(0)['converge_ms']:.1f} ms")
@@ -115,87 +171,76 @@ def run_gpu_leg(args):
     ctx.set_timing(False)
     names = ["k_scan", "k_eval<QEM>", "accept rounds (k_apply, k_reeval, k_tile_update)", "k_controller"]
     launches = [max(1, v) for v in tst["kernel_launches"]]
-    avg_ms = [tst["kernel_ms"][k] / launches[k] for k in range(4)]
     work_per = tst["ctx_evaluated"] / launches[0]
     prop_per = tst["ctx_proposed"] / launches[0]
     E2 = tst["n_adjacency"]
-    byts = {0: scan_bytes_per_launch(N, E2, K, work_per), 1: eval_bytes_per_launch(E2, N, K, work_per, prop_per)}
+    # algorithmic bytes of one clustering per kernel group
+    byts = {0: scan_bytes_per_launch(N, E2, K, work_per) * launches[0],
+            1: eval_bytes_per_launch(E2, N, K, work_per, prop_per) * launches[0],
+            2: accept_bytes(tst, N, E2)}
+    groups = {}
     for k in range(4):
+        ms = tst["kernel_ms"][k]
+        g = {"ms": round(ms, 2), "launches": launches[k], "avg_launch_us": round(ms / launches[k] * 1e3, 2)}
         extra = ""
         if k in byts:
-            extra = f", {byts[k] / 1e6:.1f} MB/launch -> {byts[k] / (avg_ms[k] * 1e-3) / 1e9:.0f} GB/s"
-        log(f"{names[k]}: {launches[k]} launches, total {tst['kernel_ms'][k]:.1f} ms, avg {avg_ms[k] * 1e3:.1f} us"
-            + extra)
-    log(f"work items/pass {work_per:.0f}, proposals/pass {prop_per:.0f}, timed-step total {sum(tst['kernel_ms']):.1f} ms")
-    dom = max((0, 1), key=lambda k: tst["kernel_ms"][k])
-    achieved = byts[dom] / (avg_ms[dom] * 1e-3) / 1e9
-    # HBM traffic per launch of that kernel from the committed rocprofv3 --pmc
-    # summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-    traffic, traffic_src = None, None
-    pmc = os.path.join(ROOT, "profiles", "c3_pmc_traffic.json")
-    if os.path.exists(pmc):
-        with open(pmc) as fh:
-            tr = json.load(fh)
-        if names[dom] in tr.get("kernels", {}):
-            traffic = tr["kernels"][names[dom]]["bytes"]
-            traffic_src = "profiles/c3_pmc_traffic.json"
+            gbs = byts[k] / (ms * 1e-3) / 1e9
+            g.update({"bytes": round(byts[k]), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            extra = f", {byts[k] / 1e9:.2f} GB algorithmic -> {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS * 100:.2f} % of peak)"
+        groups[names[k]] = g
+        log(f"{names[k]}: {launches[k]} launches, total {ms:.1f} ms, avg {ms / launches[k] * 1e3:.1f} us" + extra)
+    log(f"work items/pass {work_per:.0f}, proposals/pass {prop_per:.0f}, accept rounds {tst['ctx_rounds']}, "
+        f"candidate entries walked {tst['ctx_walked']}, re-registered {tst['ctx_reeval_live']}, re-evaluated "
+        f"{tst['ctx_reevaluated']}, moves {tst['ctx_accepted']} ({tst['ctx_accepted_constrained']} constrained); "
+        f"timed-step total {sum(tst['kernel_ms']):.1f} ms")
+    dom = max((0, 1, 2), key=lambda k: tst["kernel_ms"][k])
+    gd = groups[names[dom]]
     return {
         "N": N, "K": K, "init": init, "labels": labels, "dev_ms": dev_ms, "wall_ms": wall_ms, "stats": st,
-        "energy_gpu": energy, "xyz": xyz, "tri": tri, "H": H,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": names[dom], "avg_launch_us": round(avg_ms[dom] * 1e3, 2),
-                     "bytes_per_launch": round(byts[dom]), "launches": launches[dom],
-                     "kernel_ms_per_step": {names[k]: round(tst["kernel_ms"][k], 2) for k in range(4)}},
+        "energy_gpu": energy, "xyz": xyz, "tri": tri, "H": H, "init_ms": init_ms, "serial": serial,
+        "roofline": {"bound": "hbm", "achieved": gd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gd["frac"], "traffic": None, "kernel": names[dom],
+                     "bytes_per_clustering": gd["bytes"], "ms_per_clustering": gd["ms"], "launches": gd["launches"],
+                     "groups": groups},
     }
 
 
 def run_cpu_baseline(args, g):
     """ACVDQP restatement (oracle B, vtkThreadedClustering) on all host cores,
-    from the GPU's initial labels.  Sample: the full workload when
-    --cpu-sample full (the default: about 10-30 s of CPU work on 16 cores),
-    or a 2048 x 1024 torus (K = 25,000, same items per cluster) otherwise."""
+    from the GPU's initial labels, on the full workload, after the serial
+    restatement (oracle A) started beside the GPU steps has finished; both
+    energies against the GPU's (unshifted FP64 energies of the oracle)."""
     from oracle import pyoracle as po
-    if args.cpu_sample == "full":
-        xyz, tri, H, K, init = g["xyz"], g["tri"], g["H"], g["K"], g["init"]
-        glab, gpu_ms = g["labels"], statistics.median(g["dev_ms"])
-        sample = "the full workload (torus 4096x2048, 8,388,608 vertices, K=100000)"
-    else:
-        xyz, tri, H, K = torus_workload(2048, 1024, 25000)
-        ctx = make_ctx(xyz, tri, H, K)
-        ctx.initial_sampling()
-        init = ctx.labels(0)
-        ctx.set_initial_labels(0, init)
-        ctx.process()
-        ctx.process()
-        gpu_ms = ctx.stats(0)["converge_ms"]
-        glab = ctx.labels(0)
-        sample = "torus 2048x1024 (2,097,152 vertices), K=25000"
-    threads = int(os.environ.get("ACVD_CPU_THREADS", "0")) or (os.cpu_count() or 1)
+    xyz, tri, H, K, init = g["xyz"], g["tri"], g["H"], g["K"], g["init"]
+    glab, gpu_ms = g["labels"], statistics.median(g["dev_ms"])
+    th, box = g["serial"]
     t = time.time()
-    m = po.RefMesh(xyz, tri)
-    log(f"cpu_baseline: reference mesh structure built in {time.time() - t:.1f} s")
+    th.join()
+    if "error" in box:
+        raise box["error"]
+    po.set_progress(False)
+    log(f"serial restatement done ({time.time() - t:.0f} s after the GPU leg): {box['stats']['seconds']:.1f} s, "
+        f"{box['stats']['loops']} loops")
+    m = box["mesh"]
+    threads = int(os.environ.get("ACVD_CPU_THREADS", "0")) or (os.cpu_count() or 1)
     lab, st = m.threaded_run(po.QEM, K, 1.5, H, 3, 1, init, threads=threads, pooling=5)
     cpu_ms = st["seconds"] * 1e3
     e_cpu = m.energy(po.QEM, K, lab, 1.5, H)[0]
     e_gpu = m.energy(po.QEM, K, glab, 1.5, H)[0]
+    e_ser = m.energy(po.QEM, K, box["labels"], 1.5, H)[0]
     out = {"value": round(cpu_ms, 1), "unit": "ms to converge", "cores": threads, "kind": "port",
-           "sample": sample + ", QEM, gradation 1.5, same initial labels as the GPU; full convergence of "
-                     "vtkThreadedClustering restated (PoolingRatio 5), timed around ProcessClustering",
+           "sample": "the full workload (torus 4096x2048, 8,388,608 vertices, K=100000), QEM, gradation 1.5, same "
+                     "initial labels as the GPU; full convergence of vtkThreadedClustering restated (PoolingRatio 5), "
+                     "timed around ProcessClustering",
            "gpu_same_sample_ms": round(gpu_ms, 2), "speedup_same_sample": round(cpu_ms / gpu_ms, 1),
+           "energy_gpu": e_gpu, "energy_acvdqp": e_cpu, "energy_serial": e_ser,
+           "energy_gap_gpu_vs_serial_pct": round((e_gpu - e_ser) / e_ser * 100, 4),
            "energy_gap_gpu_vs_acvdqp_pct": round((e_gpu - e_cpu) / e_cpu * 100, 4),
+           "serial_ms": round(box["stats"]["seconds"] * 1e3, 1), "serial_loops": box["stats"]["loops"],
            "cpu_loops": st["loops"], "cpu_convergences": st["convergences"]}
     log(f"cpu_baseline: ACVDQP restatement, {threads} threads, {cpu_ms:.0f} ms ({st['loops']} loops, "
-        f"{st['convergences']} convergences); GPU on the same sample {gpu_ms:.1f} ms; energy gap gpu vs ACVDQP "
-        f"{out['energy_gap_gpu_vs_acvdqp_pct']:+.3f}%")
-    if args.gap:
-        t = time.time()
-        slab, sst = m.serial_run(po.QEM, K, 1.5, H, 3, 1, init)
-        e_ser = m.energy(po.QEM, K, slab, 1.5, H)[0]
-        out["serial_ms"] = round(sst["seconds"] * 1e3, 1)
-        out["energy_gap_gpu_vs_serial_pct"] = round((e_gpu - e_ser) / e_ser * 100, 4)
-        log(f"gap: serial ACVDQ restatement {sst['seconds']:.1f} s ({time.time() - t:.0f} s wall), energy gap "
-            f"gpu vs serial {out['energy_gap_gpu_vs_serial_pct']:+.4f}%")
+        f"{st['convergences']} convergences); GPU {gpu_ms:.1f} ms ({cpu_ms / gpu_ms:.1f}x); energy gap gpu vs serial "
+        f"{out['energy_gap_gpu_vs_serial_pct']:+.4f}%, vs ACVDQP {out['energy_gap_gpu_vs_acvdqp_pct']:+.4f}%")
     return out
 
 
@@ -392,14 +437,11 @@ def main():
                     help="only run a 1 GiB device copy (reference for the HBM byte counters)")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample", choices=["full", "half"], default="full",
-                    help="CPU baseline on the full workload or on a 2048x1024 torus")
-    ap.add_argument("--gap", action="store_true", help="also run the serial ACVDQ restatement for the energy gap")
-    ap.add_argument("--batch", action="store_true", help="also run the 384-mesh batched leg")
+    ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU legs (serial gap and ACVDQP baseline)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 384-mesh batched leg")
     ap.add_argument("--batch-count", type=int, default=384)
     ap.add_argument("--batch-gap", type=int, default=0, help="serial-reference energy gap on this many batch meshes")
-    ap.add_argument("--energy", action="store_true", help="also run the differentiable-energy leg (16 tori)")
+    ap.add_argument("--no-energy", action="store_true", help="skip the differentiable-energy leg (16 tori)")
     ap.add_argument("--energy-meshes", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -429,15 +471,17 @@ def main():
         "passes": g["stats"]["loops"],
         "moves": g["stats"]["total_moves"],
         "item_moves_per_s": round(g["stats"]["total_moves"] / (statistics.median(g["dev_ms"]) * 1e-3)),
+        "init_ms": round(g["init_ms"], 1),
         "energy_gpu": g["energy_gpu"],
         "roofline": g["roofline"],
     }
+    # the GPU legs run while the serial restatement works on its core
+    if not args.no_batch:
+        out["batched"] = run_batch_leg(args)
+    if not args.no_energy:
+        out["energy"] = run_energy_leg(args)
     if not args.skip_cpu:
         out["cpu_baseline"] = run_cpu_baseline(args, g)
-    if args.batch:
-        out["batched"] = run_batch_leg(args)
-    if args.energy:
-        out["energy"] = run_energy_leg(args)
     print(json.dumps(out), flush=True)
 
 

@@ -11,6 +11,9 @@
 //   SetConsoleOutput (:95), SetUnconstrainedInitialization (:81),
 //   SetMaxNumberOfLoops (:103), SetMaxNumberOfConvergences (:99),
 //   SetInitialClustering (:111)                             same
+//   SetFixedClusters (:134) + GetCluster(i)->AnchorItem
+//     (ACVDQ.cxx:327-334)                                   SetFixedClusters(const std::vector<int32_t>*)
+//                                                           + GetAnchorItem(i)
 //   GetMetric()->SetGradation / SetQuadricsOptimizationLevel
 //     (vtkIsotropicMetricForClustering.h:73-76, vtkQEMetricForClustering.h:50-54)  same
 //   SetSubsamplingThreshold (vtkDiscreteRemeshing.h:71)     same
@@ -78,6 +81,13 @@ class DiscreteRemeshing {
   void SetMaxNumberOfLoops(int v) { params_.max_number_of_loops = v; }
   void SetMaxNumberOfConvergences(int v) { params_.max_number_of_convergences = v; }
   void SetInitialClustering(const Labels* l) { initial_ = l; }
+  // cluster i is fixed at item (*ids)[i]: it starts there, keeps it and, with
+  // the QEM metric, takes its position as the cluster point (the
+  // AnchorItem ACVDQ sets to the same item); NumberOfClusters counts them
+  void SetFixedClusters(const std::vector<int32_t>* ids) { fixed_ = ids; }
+  int64_t GetAnchorItem(int i) const {
+    return fixed_ && i >= 0 && size_t(i) < fixed_->size() ? (*fixed_)[size_t(i)] : -1;
+  }
   void SetSubsamplingThreshold(int t) { subsampling_threshold_ = t; }
   void SetSeed(uint32_t s) { params_.seed = s; }
   void SetDevice(int d) { device_ = d; }
@@ -102,6 +112,9 @@ class DiscreteRemeshing {
     if (!ok(acvd_set_params(ctx_, 0, &p))) return nullptr;
     if (indicator_ && !ok(acvd_set_indicator(ctx_, 0, indicator_->data()))) return nullptr;
     if (initial_ && !ok(acvd_set_initial_labels(ctx_, 0, initial_->data()))) return nullptr;
+    if (!ok(acvd_set_fixed_clusters(ctx_, 0, fixed_ ? int64_t(fixed_->size()) : 0,
+                                    fixed_ && !fixed_->empty() ? fixed_->data() : nullptr)))
+      return nullptr;
     if (!ok(acvd_process_clustering(ctx_))) return nullptr;
     clustering_.resize(size_t(nv));
     if (!ok(acvd_get_labels(ctx_, 0, clustering_.data()))) return nullptr;
@@ -163,6 +176,7 @@ class DiscreteRemeshing {
   MetricHandle metric_;
   const Mesh* input_ = nullptr;
   const Labels* initial_ = nullptr;
+  const std::vector<int32_t>* fixed_ = nullptr;
   const std::vector<double>* indicator_ = nullptr;
   int subsampling_threshold_ = 10;
   int device_ = 0;

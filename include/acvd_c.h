@@ -60,7 +60,8 @@ typedef struct acvd_ctx acvd_ctx;
  *                                  on the device and weights the items with it; Remesh() of the class API
  *                                  sets it, as the reference's Remesh runs SamplingPreProcessing
  *   first_phase_rounds          <- (no counterpart) accept rounds per pass until the first convergence,
- *                                  1..12 (0: the library default, 1); later phases run 12
+ *                                  1..12; 0 = the library's rule: 1 for meshes of at most 2^20 vertices
+ *                                  (lower final energy), 12 above (fewer passes); later phases run 12
  */
 typedef struct acvd_params {
   int32_t number_of_clusters;
@@ -98,6 +99,13 @@ typedef struct acvd_stats {
    * [3] controller; launches count one per pass for [2] */
   double kernel_ms[4];
   int64_t kernel_launches[4];
+  /* accept-round work, context-wide, all passes (the last two only with
+   * timing on, acvd_set_timing; 0 otherwise) */
+  int64_t ctx_rounds;       /* accept rounds that walked the candidate list */
+  int64_t ctx_walked;       /* candidate entries read by the round kernels (dead ones included) */
+  int64_t ctx_reeval_live;  /* live candidates registered again by re-evaluate rounds */
+  int64_t ctx_reevaluated;  /* candidates re-evaluated because one of their clusters moved */
+  int64_t ctx_accepted_constrained; /* moves applied while mesh 0 was constrained (QEM derives solve) */
 } acvd_stats;
 
 /* ---- library -------------------------------------------------------------- */
@@ -149,6 +157,16 @@ int acvd_get_indicator(acvd_ctx* ctx, int32_t mesh_id, double* out);
  * 1166-1169): labels in 0..K (K = null), one per vertex.  NULL clears (the
  * library then seeds its own initial clustering on the device). */
 int acvd_set_initial_labels(acvd_ctx* ctx, int32_t mesh_id, const int32_t* labels);
+
+/* SetFixedClusters(vtkIdList*) (vtkUniformClustering.h:134) together with
+ * GetCluster(i)->AnchorItem = items[i] for the QEM metric, as ACVDQ's -fv / -ft
+ * options do (DiscreteRemeshing/Examples/ACVDQ.cxx:327-334): cluster i
+ * (i < n) starts at item items[i] in the reference's initial sampling
+ * (vtkUniformClustering.h:1186-1191), keeps the component holding it when
+ * cleaned (:441-447), and with the QEM metric its point is that item's input
+ * position, which the item never leaves (vtkQEMetricForClustering.h:196-199,
+ * 259, 270-274).  number_of_clusters counts these clusters.  n = 0 clears. */
+int acvd_set_fixed_clusters(acvd_ctx* ctx, int32_t mesh_id, int64_t n, const int32_t* items);
 
 /* ---- run ------------------------------------------------------------------ */
 /* ProcessClustering() (vtkUniformClustering.h:654-714) for every mesh of the

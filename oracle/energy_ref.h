@@ -197,7 +197,7 @@ struct EnergyRef {
             best = aw[k];
             idx = k;
           }
-        if (!(mx > 0 && best / mx > 1e-3 && left > 0)) break;
+        if (!(mx > 0 && best * (1.0 / mx) > 1e-3 && left > 0)) break;
         used[idx] = true;
         --left;
         S vr = V[0][idx] * r[0] + V[1][idx] * r[1] + V[2][idx] * r[2];

@@ -43,6 +43,9 @@ struct OrcStats {
 
 // per-loop progress lines of the serial runs on stderr (orc_set_progress)
 int g_progress = 0;
+// fixed clusters of the next runs and samplings (orc_set_fixed): cluster k
+// holds item g_fixed[k]
+std::vector<int32_t> g_fixed;
 
 template <class M>
 int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int level, int unconstrained,
@@ -50,11 +53,15 @@ int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int
                std::vector<EdgeTrace>* etrace = nullptr, size_t ecap = 0, std::vector<LoopTrace>* ltrace = nullptr) {
   Clustering<M> cl(m);
   cl.progress = g_progress;
+  cl.fixed = g_fixed;
   cl.edge_trace = etrace;
   cl.edge_trace_cap = ecap;
   cl.loop_trace = ltrace;
   if constexpr (M::kQEM) cl.metric.level = level;
   cl.set_number_of_clusters(K);
+  // QEM: GetCluster(k)->AnchorItem = fixed item k (Examples/ACVDQ.cxx:333-334)
+  if constexpr (M::kQEM)
+    for (size_t k = 0; k < g_fixed.size() && k < size_t(K); ++k) cl.clusters[k].anchor = g_fixed[k];
   cl.unconstrained_init = unconstrained;
   cl.max_loops = max_loops;
   if (init) {
@@ -113,6 +120,8 @@ void energy(const RefMesh& m, const double* ind, double gradation, int level, in
   met.build(m, ind, gradation);
   std::vector<typename M::Cluster> cl(K);
   for (auto& c : cl) M::reset(&c);
+  if constexpr (M::kQEM)  // fixed clusters: the point of cluster k is its item's position
+    for (size_t k = 0; k < g_fixed.size() && k < size_t(K); ++k) cl[k].anchor = g_fixed[k];
   for (int64_t i = 0; i < m.nv; ++i)
     if (labels[i] >= 0 && labels[i] < K) met.add_item(i, &cl[labels[i]]);
   double shifted = 0.0;
@@ -220,6 +229,8 @@ int orc_serial_run(void* mh, int metric, int K, double gradation, const double* 
 
 void orc_set_progress(int on) { g_progress = on; }
 
+void orc_set_fixed(const int32_t* ids, int64_t n) { g_fixed.assign(ids, ids + (ids ? n : 0)); }
+
 // orc_serial_run with its trace: edge records (loop, decision, edge, i1, i2 in
 // edge_i[5k..]; try1..try3 in edge_d[3k..]) up to edge_cap, loop records
 // (loop, mods, connexity, constrained, convergence, disconnected in
@@ -280,11 +291,13 @@ int orc_initial_sampling(void* mh, int metric, int K, double gradation, const do
   if (metric == 0) {
     Clustering<IsoMetric> cl(m);
     cl.set_number_of_clusters(K);
+    cl.fixed = g_fixed;
     cl.init(ind, gradation);
     cl.initial_random_sampling(lab, K);
   } else {
     Clustering<QEMMetric> cl(m);
     cl.set_number_of_clusters(K);
+    cl.fixed = g_fixed;
     cl.init(ind, gradation);
     cl.initial_random_sampling(lab, K);
   }
@@ -310,6 +323,7 @@ int orc_clean(void* mh, int K, int32_t* labels) {
   const RefMesh& m = *static_cast<RefMesh*>(mh);
   Clustering<IsoMetric> cl(m);
   cl.set_number_of_clusters(K);
+  cl.fixed = g_fixed;
   cl.allocate();
   cl.labels.assign(labels, labels + m.nv);
   int n = cl.clean_clustering();
@@ -423,6 +437,10 @@ int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* c
                         int unconstrained, int max_loops, int max_conv, const int32_t* init, int32_t* out,
                         int64_t* stats6) {
   Replay r;
+  if (!g_fixed.empty()) {
+    r.anchor.assign(size_t(K), -1);
+    for (size_t k = 0; k < g_fixed.size() && k < size_t(K); ++k) r.anchor[k] = g_fixed[k];
+  }
   r.generic_qem = (path & 1) != 0;
   r.all_rounds = (path & 2) != 0;
   // bits 8..15: accept rounds per pass before the first convergence (0: default)

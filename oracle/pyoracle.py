@@ -54,6 +54,7 @@ def lib():
         L.orc_serial_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int, ctypes.c_int,
                                      P, ctypes.c_int, P, P]
         L.orc_set_progress.argtypes = [ctypes.c_int]
+        L.orc_set_fixed.argtypes = [P, ctypes.c_int64]
         L.orc_serial_trace.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int, ctypes.c_int,
                                        P, ctypes.c_int, P, P, ctypes.c_int64, P, P, ctypes.c_int64, P, P]
         L.orc_threaded_run.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, ctypes.c_int,
@@ -218,6 +219,14 @@ class RefMesh:
     def ring_problem(self, labels, item, cluster):
         lab = np.ascontiguousarray(labels, np.int32)
         return lib().orc_ring_problem(self.h, _p(lab), int(item), int(cluster))
+
+
+def set_fixed(items):
+    """Fixed clusters of the following oracle runs, samplings and replays
+    (cluster k holds item items[k]; the QEM clusters are anchored there);
+    None or [] clears."""
+    ids = np.zeros(0, np.int32) if items is None else np.ascontiguousarray(items, np.int32)
+    lib().orc_set_fixed(_p(ids) if len(ids) else None, len(ids))
 
 
 def set_progress(on):

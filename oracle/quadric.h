@@ -130,9 +130,11 @@ inline int compute_displacement(const double* Q, const double* P, double* D, int
         local = absw[j];
         idx = j;
       }
+    // |sigma| * invmaxW > thr as the reference computes it (:117, :134);
     // maxw == 0 gives 0 * inf = NaN in the reference: the test fails and the
     // component is dropped; handled explicitly here.
-    bool ok = (maxw > 0.0) && (absw[idx] / maxw > thr) && (max_sv > 0);
+    const double invmaxw = 1.0 / maxw;
+    bool ok = (maxw > 0.0) && (absw[idx] * invmaxw > thr) && (max_sv > 0);
     if (ok) keep[idx] = 1.0;
     else rank_def++;
     absw[idx] = -2.0;

@@ -34,13 +34,14 @@ struct Replay {
   int max_loops = 5000000, max_conv = 1000000000, unconstrained_init = 0;
   bool generic_qem = false;  // force the generic QEM proposal path (equivalence checks)
   bool all_rounds = false;   // run every round even after one without moves (equivalence checks)
-  int first_rounds = acvd::kDefaultFirstRounds;  // accept rounds per pass before the first convergence
+  int first_rounds = 0;  // accept rounds per pass before the first convergence (0: the library's rule)
   const int32_t* rp = nullptr;
   const int32_t* col = nullptr;
   const float* xw = nullptr;
   const float* q = nullptr;
 
   std::vector<int32_t> labels, sizes, stamp, cand_pass, pending, cl_round;
+  std::vector<int32_t> anchor;  // per cluster: its fixed item or -1 (empty: no fixed clusters)
   std::vector<unsigned long long> acc_ws, acc_q, ring_adj;
   std::vector<acvd::ClusterGeom> geom;
   std::vector<double> T, evec;
@@ -64,6 +65,7 @@ struct Replay {
     v.stamp = stamp.data();
     v.cand_pass = cand_pass.data();
     v.ring_adj = ring_adj.empty() ? nullptr : ring_adj.data();
+    v.anchor = anchor.empty() ? nullptr : anchor.data();
     return v;
   }
 
@@ -96,9 +98,15 @@ struct Replay {
       const int64_t* q10 = reinterpret_cast<const int64_t*>(&acc_q[10 * int64_t(c)]);
       double V0[9];
       for (int k = 0; k < 9; ++k) V0[k] = evec[9 * int64_t(c) + k];
-      T[c] = st.constrained ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr, warm ? V0 : nullptr,
-                                                     &evec[9 * int64_t(c)])
-                            : 0.0;
+      if (!anchor.empty() && anchor[c] >= 0) {
+        const double a[3] = {double(xw[4 * int64_t(anchor[c])]), double(xw[4 * int64_t(anchor[c]) + 1]),
+                             double(xw[4 * int64_t(anchor[c]) + 2])};
+        T[c] = acvd::anchored_T(geom[c], a);
+      } else {
+        T[c] = st.constrained ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr, warm ? V0 : nullptr,
+                                                       &evec[9 * int64_t(c)])
+                              : 0.0;
+      }
     }
   }
 
@@ -159,8 +167,9 @@ struct Replay {
     const acvd::SweepView v = view();
     const int32_t lab = labels[i];
     if (metric == 0) return acvd::propose_item<0>(v, i, lab, st.connexity, st.constrained, level, sc, target);
-    return generic_qem ? acvd::propose_item<1, false>(v, i, lab, st.connexity, st.constrained, level, sc, target)
-                       : acvd::propose_item<1>(v, i, lab, st.connexity, st.constrained, level, sc, target);
+    const int mode = acvd::qem_mode(st);
+    return generic_qem ? acvd::propose_item<1, false>(v, i, lab, st.connexity, mode, level, sc, target)
+                       : acvd::propose_item<1>(v, i, lab, st.connexity, mode, level, sc, target);
   }
 
   void pass() {
@@ -255,6 +264,11 @@ struct Replay {
       comp_size.push_back(size);
       comp_root.push_back(i);
     }
+    // the component holding a cluster's fixed item is kept (its size counts
+    // 1e9 in CleanClustering, vtkUniformClustering.h:441-447)
+    if (!anchor.empty())
+      for (int32_t c = 0; c < K; ++c)
+        if (anchor[c] >= 0 && labels[anchor[c]] == c) comp_size[comp[anchor[c]]] += int64_t(1) << 40;
     std::vector<int32_t> best(K, -1), ncomp(K, 0);
     for (size_t c = 0; c < comp_size.size(); ++c) {
       const int32_t lab = labels[comp_root[c]];
@@ -295,7 +309,8 @@ struct Replay {
     st.max_loops = std::min<int32_t>(max_loops, acvd::kMaxLoops - 2);
     st.max_convergences = max_conv;
     st.unconstrained_init = unconstrained_init;
-    st.first_rounds = first_rounds;
+    st.anchored = metric == 1 && !anchor.empty();
+    st.first_rounds = first_rounds > 0 ? first_rounds : acvd::auto_first_rounds(N);
     st.constrained = unconstrained_init ? 0 : 1;
     st.early_stop_items = recompute(true);
     while (st.phase == acvd::kPhaseSweep) {

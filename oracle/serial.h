@@ -78,6 +78,7 @@ struct Clustering {
   int unconstrained_init = 0;
   int initial_sampling_type = 1;
   const int32_t* initial_labels = nullptr;
+  std::vector<int32_t> fixed;  // FixedClusters: cluster k holds item fixed[k] (vtkUniformClustering.h:134)
   int console = 0;
   int progress = 0;  // 1: one line per loop on stderr (long runs)
   RunStats stats;
@@ -212,11 +213,13 @@ struct Clustering {
       int64_t size = 0;
       q.push_back(i);
       int32_t type = labels[i];
+      // is there a constraint on the item? (:436-447: the fixed item counts 1e9)
+      const int64_t fixed_item = type < int32_t(fixed.size()) ? fixed[type] : -1;
       while (!q.empty()) {
         int64_t i1 = q.front();
         q.pop_front();
         if (visited[i1]) continue;
-        size += 1;
+        size += i1 == fixed_item ? 1000000000 : 1;
         visited[i1] = 1;
         mesh.neighbours(i1, nb);
         for (int32_t i2 : nb)
@@ -315,11 +318,12 @@ struct Clustering {
   }
 
   // vtkUniformClustering::ComputeInitialRandomSampling, metric weights, the
-  // whole item range (List == 0), no fixed clusters.
+  // whole item range (List == 0); fixed item k starts in cluster k.
   void initial_random_sampling(std::vector<int32_t>& sampling, int nregions) {
     const int64_t n = n_items();
     sampling.assign(n, K);
     int offset = 0;
+    for (; offset < int(fixed.size()); ++offset) sampling[fixed[offset]] = offset;
     std::vector<int32_t> items(n);
     int64_t remaining_items = n;
     int remaining_regions = nregions - offset;
@@ -341,7 +345,7 @@ struct Clustering {
         if (sampling[item] == K) found = true;
         else first_item++;
       }
-      if (!found) continue;
+      if (!found) break;  // every item labelled (the reference would spin here)
       iq.clear();
       iq.push_back(item);
       sw = 0;

@@ -51,13 +51,22 @@ def rep_points(Q9, c, level=3, thr=1e-3):
     return q
 
 
-def cluster_energy(metric, W, S, Q9=None, level=3, constrained=True):
+def cluster_energy(metric, W, S, Q9=None, level=3, constrained=True, apos=None):
+    """apos: per row the anchor position of a fixed QEM cluster, NaN for a
+    free one (its point is the anchor in every phase,
+    vtkQEMetricForClustering.h:270-274)."""
     W = np.asarray(W, np.float64)
     S = np.asarray(S, np.float64).reshape(-1, 3)
     if metric == 0 or not constrained:
-        return -np.einsum("ni,ni->n", S, S) / W
-    q = rep_points(Q9, S / W[:, None], level)
-    return W * np.einsum("ni,ni->n", q, q) - 2.0 * np.einsum("ni,ni->n", q, S)
+        E = -np.einsum("ni,ni->n", S, S) / W
+    else:
+        q = rep_points(Q9, S / W[:, None], level)
+        E = W * np.einsum("ni,ni->n", q, q) - 2.0 * np.einsum("ni,ni->n", q, S)
+    if metric == 1 and apos is not None:
+        fixed = ~np.isnan(apos[:, 0])
+        a = apos[fixed]
+        E[fixed] = W[fixed] * np.einsum("ni,ni->n", a, a) - 2.0 * np.einsum("ni,ni->n", a, S[fixed])
+    return E
 
 
 def neighbour_sets(nv, a, b):
@@ -87,13 +96,18 @@ def ring_problem(nb, lab, i, c):
 
 
 def violations(metric, xyz, a, b, lab, K, w, q9=None, level=3, connexity=True, constrained=True, rel_tol=1e-9,
-               margin=None):
+               accept_eps=0.0, fixed=None):
     """Allowed single moves across cut edges that lower the energy by more
     than the tolerance: a list of (item, from, to, dE, tol).
 
     rel_tol scales |E_from| + |E_to| (FP64 rounding of the shifted energies);
-    margin(item, from, to) may add a move-dependent slack (the device's
-    acceptance threshold)."""
+    accept_eps adds accept_eps * M, M the sum of the magnitudes of the terms
+    of dE: w (t_a + t_b) with t_a = W_a / (W_a - w) |x - c_a|^2 and
+    t_b = W_b / (W_b + w) |x - c_b|^2 (the isotropic part, Hartigan form), plus
+    T = W |q - c|^2 of both clusters before and after (QEM) -- the device
+    takes a move only when dE < -2^-18 M.  fixed: item of fixed cluster k
+    (SetFixedClusters); that item never moves and, with the QEM metric, is
+    the cluster's point."""
     x = np.asarray(xyz, np.float64).reshape(-1, 3)
     lab = np.asarray(lab)
     w = np.asarray(w, np.float64)
@@ -107,13 +121,20 @@ def violations(metric, xyz, a, b, lab, K, w, q9=None, level=3, connexity=True, c
         Q = np.zeros((K + 1, 9))
         np.add.at(Q, lab, q9)
         Q = Q[:K]
-    E = cluster_energy(metric, W, S, Q, level, constrained)
+    apos = None
+    anchor_item = np.full(K, -1, np.int64)
+    if fixed is not None and len(fixed):
+        fixed = np.asarray(fixed, np.int64)
+        anchor_item[:len(fixed)] = fixed
+        apos = np.full((K, 3), np.nan)
+        apos[:len(fixed)] = x[fixed]
+    E = cluster_energy(metric, W, S, Q, level, constrained, apos)
     cut = (lab[a] != lab[b]) & (lab[a] < K) & (lab[b] < K)  # null items (label K) are not moved
     src = np.concatenate([a[cut], b[cut]])
     dst_item = np.concatenate([b[cut], a[cut]])
     ca = lab[src]
     cb = lab[dst_item]
-    ok = size[ca] > 1
+    ok = (size[ca] > 1) & (anchor_item[ca] != src)
     src, ca, cb = src[ok], ca[ok], cb[ok]
     wi = w[src][:, None]
     xi = x[src]
@@ -122,16 +143,28 @@ def violations(metric, xyz, a, b, lab, K, w, q9=None, level=3, connexity=True, c
     Qa = Qb = None
     if metric == 1:
         Qa, Qb = Q[ca] - q9[src], Q[cb] + q9[src]
-    dE = (cluster_energy(metric, Wa, Sa, Qa, level, constrained) + cluster_energy(metric, Wb, Sb, Qb, level, constrained)
-          - E[ca] - E[cb])
+    pa = pb = None
+    if apos is not None:
+        pa, pb = apos[ca], apos[cb]
+    dE = (cluster_energy(metric, Wa, Sa, Qa, level, constrained, pa)
+          + cluster_energy(metric, Wb, Sb, Qb, level, constrained, pb) - E[ca] - E[cb])
     tol = rel_tol * (np.abs(E[ca]) + np.abs(E[cb]))
+    if accept_eps:
+        ca_c, cb_c = S[ca] / W[ca][:, None], S[cb] / W[cb][:, None]
+        ta = W[ca] / Wa * np.einsum("ni,ni->n", xi - ca_c, xi - ca_c)
+        tb = W[cb] / Wb * np.einsum("ni,ni->n", xi - cb_c, xi - cb_c)
+        M = w[src] * (ta + tb)
+        if metric == 1 and (constrained or apos is not None):
+            def T(Wx, Sx, Qx, px):
+                return cluster_energy(1, Wx, Sx, Qx, level, constrained, px) - cluster_energy(0, Wx, Sx)
+            M = (M + T(W[ca], S[ca], Q[ca], pa) + T(Wa, Sa, Qa, pa) + T(W[cb], S[cb], Q[cb], pb)
+                 + T(Wb, Sb, Qb, pb))
+        tol = tol + accept_eps * M
     out = []
     nb = None
     for k in np.flatnonzero(dE < -tol):
         i, f, t = int(src[k]), int(ca[k]), int(cb[k])
-        slack = float(tol[k]) + (margin(i, f, t) if margin else 0.0)
-        if not dE[k] < -slack:
-            continue
+        slack = float(tol[k])
         if connexity:
             if nb is None:
                 nb = neighbour_sets(nv, a, b)

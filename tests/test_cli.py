@@ -130,3 +130,28 @@ def test_cli_output_equals_the_python_api(tmp_path, qem, gradation):
     np.testing.assert_array_equal(cx, out.points)
     np.testing.assert_array_equal(ct, out.triangles)
     assert len(cx) == K and len(ct) == 2 * K - 4  # closed genus-0 output
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["-fv", "-ft"])
+def test_cli_fixed_vertices_are_kept(gpu_ready, tmp_path, kind):
+    """ACVDQ's -fv / -ft (Examples/ACVDQ.cxx:246-281, 327-366): the fixed
+    vertices become anchored clusters and the program's own check (output
+    vertex i == input vertex fixed[i], coordinate for coordinate) passes."""
+    xyz, tri = meshgen.icosphere(5, 4, 0.02)
+    src = tmp_path / "in.ply"
+    write_ply(src, np.asarray(xyz, np.float64), tri, binary=False)
+    lst = tmp_path / "fixed.txt"
+    if kind == "-fv":
+        fixed = [17, 1200, 9000, 5]
+        lst.write_text("\n".join(map(str, fixed)) + "\n")
+    else:
+        faces = [3, 400, 2047]
+        lst.write_text(" ".join(map(str, faces)) + "\n")
+        fixed = sorted({int(v) for f in faces for v in tri[f]})
+    r = run_cli(src, 300, 0, "-qem", 1, kind, lst, "-o", tmp_path, "-of", "out.ply")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Constraints on vertices have been checked" in r.stdout
+    out, _ = read_ply_ascii(tmp_path / "out.ply")
+    assert len(out) == 300 + len(fixed)
+    np.testing.assert_array_equal(out[:len(fixed)], np.asarray(xyz, np.float64)[fixed])
