@@ -33,6 +33,7 @@ class Params(ctypes.Structure):
         ("initial_sampling", ctypes.c_int32),
         ("curvature_indicator", ctypes.c_int32),
         ("first_phase_rounds", ctypes.c_int32),
+        ("write_energy_log", ctypes.c_int32),
         ("gradation", ctypes.c_double),
     ]
 
@@ -78,7 +79,7 @@ EXPORTS = [
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
     "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
     "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
-    "acvd_set_fixed_clusters",
+    "acvd_set_fixed_clusters", "acvd_subdivide", "acvd_get_energy_log",
     "acvd_energy_forward_cvt_noalloc",
 ]
 
@@ -143,6 +144,8 @@ def lib():
     L.acvd_energy_forward.argtypes = [P, P, P, P, P, P]
     L.acvd_energy_backward.argtypes = [P, P, P, P, P, P, P, P]
     L.acvd_set_fixed_clusters.argtypes = [P, i32, i64, P]
+    L.acvd_subdivide.argtypes = [i64, P, i64, P, ctypes.POINTER(i64), ctypes.POINTER(i64), P, P, P]
+    L.acvd_get_energy_log.argtypes = [P, i32, i64, ctypes.POINTER(i64), P, P, P]
     L.acvd_energy_forward_cvt_noalloc.argtypes = [ctypes.c_int, i64, i32, P, P, P, P, P]
     _lib = L
     return L

@@ -8,7 +8,8 @@
 //                 0 = none, ACVD.cxx:66, 217-291); ACVDQ: quadrics
 //                 optimisation level of the metric (default 3)
 //   -s threshold  subsampling threshold (default 10); inputs with fewer than
-//                 threshold * nvertices vertices are refused (no subdivision)
+//                 threshold * nvertices vertices are subdivided at their edge
+//                 midpoints until they reach it (CheckSubsamplingRatio)
 //   -o directory  output directory;  -of file  output file (simplification.ply)
 //   -cd file      per-vertex density indicator, one value per line (the
 //                 reference reads a VTK image instead); without it and with
@@ -22,7 +23,10 @@
 //                 the free clusters), and the output is checked to hold every
 //                 fixed vertex exactly (:340-366)
 //   -ft file      ACVDQ: fixed triangles; their vertices are fixed (:255-281)
-//   -m -d -b -l -w -cmin -cmax -cf -np -p   accepted; not supported (warned)
+//   -w 0/1        energy log: energy.txt in the output directory, one line
+//                 "loop seconds energy" per loop and the final energy
+//                 (SetWriteToGlobalEnergyLog, ACVD.cxx:157-160)
+//   -m -d -b -l -cmin -cmax -cf -np -p   accepted; not supported (warned)
 //
 // Meshes are read from PLY (ascii or binary little-endian) or OBJ and written
 // as ASCII PLY.  Polygons are fanned into triangles, as vtkSurface::SetInput
@@ -231,8 +235,13 @@ bool read_values(const std::string& path, int64_t n, std::vector<double>& out, s
 template <class Remesher>
 int run(Remesher* remesh, const acvd::Mesh& mesh, int K, double gradation, int subsampling, int q_level,
         bool qem, const std::vector<double>* indicator, const std::vector<int32_t>* fixed,
-        const std::string& out_path, const std::string& smooth_path) {
+        const std::string& out_path, const std::string& smooth_path, int energy_log, const std::string& prefix) {
   remesh->SetInput(&mesh);
+  if (energy_log) {  // ACVD.cxx:157-160, ACVDQ.cxx:199-203; energy.txt beside the outputs
+    std::printf("Setting writing energy log file to %d\n", energy_log);
+    remesh->SetWriteToGlobalEnergyLog(energy_log);
+    remesh->SetOutputDirectory(prefix.c_str());
+  }
   if (fixed) {
     remesh->SetFixedClusters(fixed);
     remesh->SetNumberOfClusters(K + int(fixed->size()));
@@ -295,13 +304,13 @@ int main(int argc, char** argv) {
                 "nvertices is the desired number of vertices\n"
                 "gradation defines the influence of local curvature (0=uniform meshing)\n"
                 "options: -q level, -s threshold, -o directory, -of file, -cd indicator file, -qem 0/1,\n"
-                "         -fv fixed vertices file, -ft fixed triangles file (with -qem 1)\n");
+                "         -fv fixed vertices file, -ft fixed triangles file (with -qem 1), -w energy log 0/1\n");
     return 0;
   }
   const std::string input = argv[1];
   const int K = std::atoi(argv[2]);
   const double gradation = std::atof(argv[3]);
-  int subsampling = 10, q_level = -1, qem = 0;
+  int subsampling = 10, q_level = -1, qem = 0, energy_log = 0;
   std::string out_dir, out_file = "simplification.ply", density, fixed_file;
   bool fixed_triangles = false;
   for (int a = 4; a + 1 < argc; a += 2) {
@@ -312,6 +321,7 @@ int main(int argc, char** argv) {
     else if (key == "-of") out_file = value;
     else if (key == "-cd") density = value;
     else if (key == "-qem") qem = std::atoi(value.c_str());
+    else if (key == "-w") energy_log = std::atoi(value.c_str());
     else if (key == "-fv" || key == "-ft") {
       fixed_file = value;
       fixed_triangles = key == "-ft";
@@ -354,11 +364,12 @@ int main(int argc, char** argv) {
   if (qem) {
     auto* r = acvd::QIsotropicDiscreteRemeshing::New();
     rc = run(r, mesh, K, gradation, subsampling, q_level, true, ind, fixed_file.empty() ? nullptr : &fixed, out_path,
-             smooth_path);
+             smooth_path, energy_log, prefix);
     r->Delete();
   } else {
     auto* r = acvd::IsotropicDiscreteRemeshing::New();
-    rc = run(r, mesh, K, gradation, subsampling, q_level, false, ind, nullptr, out_path, smooth_path);
+    rc = run(r, mesh, K, gradation, subsampling, q_level, false, ind, nullptr, out_path, smooth_path, energy_log,
+             prefix);
     r->Delete();
   }
   return rc;

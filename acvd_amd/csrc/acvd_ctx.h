@@ -71,6 +71,13 @@ struct DevMesh {
 constexpr int kPassLogFields = 5 + kMatchRounds;
 constexpr int kPassLogCap = 64;  // > passes per chunk
 
+// One energy log line of a mesh: the energy after a pass and the device wall
+// clock when it was summed.
+struct EnergyLogEntry {
+  double energy;
+  unsigned long long clock;
+};
+
 // Small control block (device), read back by the host between pass chunks.
 struct Ctrl {
   int32_t any_sweep;
@@ -199,6 +206,12 @@ hipError_t sort_pairs_u32(hipStream_t s, const uint32_t* kin, uint32_t* kout, co
                           int64_t n, void** tmp, size_t* tmp_bytes);
 hipError_t launch_cluster_points(hipStream_t s, const SweepArgs& a, int32_t mesh, double* pts);
 hipError_t launch_item_energy(hipStream_t s, const SweepArgs& a, const double* pts_all, double* e_item);
+// energy log: cluster energies into e_cluster (n_clusters), then per
+// sweeping mesh entry `loops % ring` of its ring (ring entries per mesh), or
+// with final_e every mesh's sum into final_e[mesh]
+hipError_t launch_energy_log(hipStream_t s, const SweepArgs& a, double* e_cluster, EnergyLogEntry* log, int32_t ring,
+                             double* final_e);
+hipError_t launch_clock_stamp(hipStream_t s, unsigned long long* t);
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
 // output mesh
 hipError_t build_dual_faces(hipStream_t s, int64_t nf, const int32_t* tri, const int32_t* labels, int64_t item_off,

@@ -362,6 +362,22 @@ ACVD_HD double anchored_T(const ClusterGeom& g, const double* a) {
   return g.W * (dx * dx + dy * dy + dz * dz);
 }
 
+// Energy of one cluster as the energy log sums it (GetClusterEnergy:
+// -|S|^2/W isotropic, vtkIsotropicMetricForClustering.h:114-121; W|p|^2 - 2 p.S
+// QEM, vtkQEMetricForClustering.h:194-208, with p the anchor, the solved point
+// when constrained, else the centroid), written -W|c|^2 + T with
+// T = W|p - c|^2 (0 for isotropic clusters).  An empty cluster counts 0.
+ACVD_HD double log_cluster_energy(const ClusterGeom& g, double T) {
+  if (!(g.W > 0.0)) return 0.0;
+  return T - g.W * (g.cx * g.cx + g.cy * g.cy + g.cz * g.cz);
+}
+
+// Per-mesh sum of the energy log in a fixed order: kLogThreads partial sums
+// (partial t adds clusters t, t + kLogThreads, ... in turn, from 0), then a
+// halving tree (partial t += partial t + h for h = kLogThreads / 2, ..., 1).
+// The device reduction and the host replay take exactly these steps.
+constexpr int kLogThreads = 256;
+
 // T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item:
 // with the anchor position apos when the cluster is anchored, else (solve)
 // the rank-truncated solve warm-started from the cluster's own eigenvectors

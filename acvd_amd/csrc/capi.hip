@@ -20,6 +20,7 @@
 
 #include "../../include/acvd_c.h"
 #include "acvd_ctx.h"
+#include "host_mesh.h"
 #include "host_sampling.h"
 
 using namespace acvd;
@@ -79,12 +80,18 @@ struct HostMesh {
   std::vector<int32_t> out_tri;
   bool clustered = false;
   acvd_stats stats;
+  // energy log of the last run (write_energy_log): per loop, seconds since the
+  // start of the minimisation and the energy; the energy after the run
+  std::vector<double> log_seconds, log_energy;
+  double log_final = 0.0;
+  bool logged = false;
 };
 
 constexpr int kChunk = 16;        // passes per graph launch
 constexpr int kShortChunk = 4;    // passes per graph launch near the end of a phase
 static_assert(kPassLogCap > kChunk, "the pass log must hold a chunk of passes");
 constexpr int kReduceItems = 16384;  // items per block of the deterministic reductions
+constexpr int kEnergyLogRing = 2 * kChunk;  // energy log entries per mesh on the device, drained per chunk
 
 }  // namespace
 
@@ -120,6 +127,12 @@ struct acvd_ctx {
   // clean scratch
   DBuf<int32_t> d_parent, d_comp_size, d_ncomp;
   DBuf<unsigned long long> d_best;
+  // energy log (meshes with write_energy_log): cluster energies, a ring of
+  // kEnergyLogRing entries per mesh, the start clock, the final energies
+  bool energy_log = false;
+  DBuf<double> d_log_cluster, d_log_final;
+  DBuf<EnergyLogEntry> d_log_ring;
+  DBuf<unsigned long long> d_log_clock0;
   // pinned host mirrors
   Ctrl* h_ctrl = nullptr;
   MeshState* h_state = nullptr;
@@ -591,6 +604,8 @@ int enqueue_chunk(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass, int n_round
       HIP_TRY(launch_apply(s, a, r, !qem_unc));
     }
     if (tiles) HIP_TRY(launch_tile_update(s, a));
+    if (c->energy_log)
+      HIP_TRY(launch_energy_log(s, a, c->d_log_cluster.p, c->d_log_ring.p, kEnergyLogRing, nullptr));
     HIP_TRY(launch_controller(s, a));
   }
   return ACVD_OK;
@@ -615,6 +630,8 @@ int run_chunk_timed(acvd_ctx* c, bool qem_unc, bool tiles, int n_pass, int n_rou
     }
     if (tiles) HIP_TRY(launch_tile_update(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 3], s));
+    if (c->energy_log)  // counted with the controller
+      HIP_TRY(launch_energy_log(s, a, c->d_log_cluster.p, c->d_log_ring.p, kEnergyLogRing, nullptr));
     HIP_TRY(launch_controller(s, a));
     HIP_TRY(hipEventRecord(ev[5 * p + 4], s));
   }
@@ -706,11 +723,83 @@ void exact_anchor_points(const HostMesh& h, double* xyz) {
     for (int d = 0; d < 3; ++d) xyz[3 * k + d] = h.xyz[3 * size_t(h.fixed[k]) + d];
 }
 
+// Energy log (SetWriteToGlobalEnergyLog, vtkUniformClustering.h:123, 670-699,
+// 1383-1395): buffers for the meshes that asked for it, and the start clock
+// (StartTime, taken just before MinimizeEnergy, :689).
+int start_energy_log(acvd_ctx* c) {
+  c->energy_log = false;
+  for (auto& h : c->meshes) {
+    h.logged = h.params.write_energy_log != 0;
+    h.log_seconds.clear();
+    h.log_energy.clear();
+    h.log_final = 0.0;
+    c->energy_log |= h.logged;
+  }
+  if (!c->energy_log) return ACVD_OK;
+  const size_t M = c->meshes.size();
+  if (c->d_log_cluster.n < size_t(c->K)) HIP_TRY(c->d_log_cluster.alloc(size_t(c->K)));
+  if (c->d_log_ring.n < M * kEnergyLogRing) HIP_TRY(c->d_log_ring.alloc(M * kEnergyLogRing));
+  if (c->d_log_final.n < M) HIP_TRY(c->d_log_final.alloc(M));
+  if (!c->d_log_clock0.p) HIP_TRY(c->d_log_clock0.alloc(1));
+  HIP_TRY(launch_clock_stamp(c->stream, c->d_log_clock0.p));
+  return ACVD_OK;
+}
+
+double wall_clock_hz(int device) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0) return 0.0;
+  return double(khz) * 1e3;
+}
+
+// Moves the log entries of the passes run since the last call to the host
+// (each pass of a sweeping mesh wrote entry `loops % kEnergyLogRing`; a chunk
+// runs fewer passes than the ring holds).
+int drain_energy_log(acvd_ctx* c) {
+  if (!c->energy_log) return ACVD_OK;
+  hipStream_t s = c->stream;
+  const int M = int(c->meshes.size());
+  std::vector<EnergyLogEntry> ring(size_t(M) * kEnergyLogRing);
+  std::vector<MeshState> st(M);
+  unsigned long long clock0 = 0;
+  HIP_TRY(hipMemcpyAsync(ring.data(), c->d_log_ring.p, ring.size() * sizeof(EnergyLogEntry), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(st.data(), c->d_state.p, M * sizeof(MeshState), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&clock0, c->d_log_clock0.p, sizeof(clock0), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const double hz = wall_clock_hz(c->device);
+  for (int m = 0; m < M; ++m) {
+    HostMesh& h = c->meshes[m];
+    if (!h.logged) continue;
+    for (int64_t L = int64_t(h.log_energy.size()); L < st[m].loops; ++L) {
+      const EnergyLogEntry& x = ring[size_t(m) * kEnergyLogRing + size_t(L % kEnergyLogRing)];
+      h.log_energy.push_back(x.energy);
+      h.log_seconds.push_back(hz > 0.0 ? double(x.clock - clock0) / hz : 0.0);
+    }
+  }
+  return ACVD_OK;
+}
+
+// The energy after the run ("Final Energy :", vtkUniformClustering.h:695-699).
+int finish_energy_log(acvd_ctx* c) {
+  if (!c->energy_log) return ACVD_OK;
+  hipStream_t s = c->stream;
+  const int M = int(c->meshes.size());
+  HIP_TRY(launch_energy_log(s, c->args(), c->d_log_cluster.p, c->d_log_ring.p, kEnergyLogRing, c->d_log_final.p));
+  std::vector<double> fin(M);
+  HIP_TRY(hipMemcpyAsync(fin.data(), c->d_log_final.p, M * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int m = 0; m < M; ++m)
+    if (c->meshes[m].logged) c->meshes[m].log_final = fin[m];
+  return ACVD_OK;
+}
+
 int minimize_energy(acvd_ctx* c) {
   hipStream_t s = c->stream;
   const int M = int(c->meshes.size());
   SweepArgs a = c->args();
-  const bool use_graph = std::getenv("ACVD_NO_GRAPH") == nullptr;
+  int rc0 = start_energy_log(c);
+  if (rc0) return rc0;
+  // the energy log launches its kernels in every pass, outside the captured chunks
+  const bool use_graph = std::getenv("ACVD_NO_GRAPH") == nullptr && !c->energy_log;
   // FillHolesInClustering, then ReComputeStatistics + SetAllClustersToModified
   int rc = fill_until_stable(c, c->d_mask_all.p, 1 << 20);
   if (rc) return rc;
@@ -773,6 +862,8 @@ int minimize_energy(acvd_ctx* c) {
     if (rc2) return rc2;
     HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    rc = drain_energy_log(c);
+    if (rc) return rc;
     if (loop_output) print_pass_log(*c->h_ctrl, &printed);
     if (c->h_ctrl->any_need_clean) {
       HIP_TRY(hipMemcpyAsync(c->h_state, c->d_state.p, M * sizeof(MeshState), hipMemcpyDeviceToHost, s));
@@ -829,7 +920,7 @@ int minimize_energy(acvd_ctx* c) {
   HIP_TRY(hipMemcpyAsync(c->h_ctrl, c->d_ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   c->last_ctrl = *c->h_ctrl;
-  return ACVD_OK;
+  return finish_energy_log(c);
 }
 
 int initial_labels(acvd_ctx* c) {
@@ -902,6 +993,7 @@ void acvd_params_init(acvd_params* p) {
   p->initial_sampling = 0;
   p->curvature_indicator = 0;
   p->first_phase_rounds = 0;
+  p->write_energy_log = 0;
   p->gradation = 0.0;
 }
 
@@ -1001,6 +1093,23 @@ int acvd_set_initial_labels(acvd_ctx* ctx, int32_t mesh_id, const int32_t* label
   HostMesh& h = ctx->meshes[mesh_id];
   if (!labels) h.init_labels.clear();
   else h.init_labels.assign(labels, labels + h.nv);
+  return ACVD_OK;
+}
+
+int acvd_subdivide(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int64_t* out_nv, int64_t* out_nf,
+                   double* out_xyz, int32_t* out_tri, int32_t* parents) {
+  if (nv <= 0 || nf <= 0 || !xyz || !tri || !out_nv || !out_nf) return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv) return fail(ACVD_ERR_ARG, "triangle references a missing vertex");
+  std::vector<double> x;
+  std::vector<int32_t> t, par;
+  const int64_t ne = subdivide_mesh(nv, xyz, nf, tri, &x, &t, &par);
+  if (nv + ne >= (int64_t(1) << 31) - 1) return fail(ACVD_ERR_LIMIT, "subdivided mesh exceeds 2^31 vertices");
+  *out_nv = nv + ne;
+  *out_nf = int64_t(t.size() / 3);
+  if (out_xyz) std::memcpy(out_xyz, x.data(), x.size() * sizeof(double));
+  if (out_tri) std::memcpy(out_tri, t.data(), t.size() * sizeof(int32_t));
+  if (parents) std::memcpy(parents, par.data(), par.size() * sizeof(int32_t));
   return ACVD_OK;
 }
 
@@ -1241,6 +1350,21 @@ int acvd_get_energy(acvd_ctx* ctx, int32_t mesh_id, double* energy) {
     he.resize(half);
   }
   *energy = he.empty() ? 0.0 : he[0];
+  return ACVD_OK;
+}
+
+int acvd_get_energy_log(acvd_ctx* ctx, int32_t mesh_id, int64_t capacity, int64_t* n_loops, double* seconds,
+                        double* energy, double* final_energy) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (!n_loops || capacity < 0) return fail(ACVD_ERR_ARG, "null count or negative capacity");
+  const HostMesh& h = ctx->meshes[mesh_id];
+  if (!h.logged || !h.clustered) return fail(ACVD_ERR_STATE, "no energy log recorded (write_energy_log off)");
+  const int64_t n = int64_t(h.log_energy.size());
+  *n_loops = n;
+  const int64_t w = std::min(n, capacity);
+  if (seconds) std::copy(h.log_seconds.begin(), h.log_seconds.begin() + w, seconds);
+  if (energy) std::copy(h.log_energy.begin(), h.log_energy.begin() + w, energy);
+  if (final_energy) *final_energy = h.log_final;
   return ACVD_OK;
 }
 

@@ -18,8 +18,8 @@
 
 #include <algorithm>
 #include <cmath>
-#include <deque>
 #include <random>
+#include <thread>
 #include <vector>
 
 namespace acvd {
@@ -37,54 +37,60 @@ double face_area(const double* xyz, const int32_t* t) {
   return 0.25 * std::sqrt(std::fabs(4.0 * a * c - (a - b + c) * (a - b + c)));
 }
 
+// f(begin, end) over [0, n) split across the host's cores (the per-vertex
+// stages are independent; every result is the one a serial loop gives).
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int64_t T = std::min<int64_t>(16, std::max<int64_t>(1, hw));
+  if (n < (int64_t(1) << 16) || T == 1) {
+    f(int64_t(0), n);
+    return;
+  }
+  const int64_t chunk = (n + T - 1) / T;
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < T; ++t) {
+    const int64_t a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
+  }
+  for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
                                 const double* indicator, double gradation, int32_t K, int32_t* labels,
                                 int64_t n_fixed, const int32_t* fixed) {
   // vertex -> faces, increasing face id (faces with a repeated first edge
-  // vertex are inactive in the reference and skipped)
+  // vertex are inactive in the reference and skipped; a face listing a
+  // vertex twice is stored once for it)
+  auto slot_of = [](const int32_t* t, int j) { return !(j > 0 && t[j] == t[0]) && !(j > 1 && t[j] == t[1]); };
   std::vector<int64_t> vfptr(nv + 1, 0);
   for (int64_t f = 0; f < nf; ++f) {
     const int32_t* t = tri + 3 * f;
     if (t[0] == t[1]) continue;
-    for (int j = 0; j < 3; ++j) vfptr[t[j] + 1]++;
+    for (int j = 0; j < 3; ++j)
+      if (slot_of(t, j)) vfptr[t[j] + 1]++;
   }
   for (int64_t v = 0; v < nv; ++v) vfptr[v + 1] += vfptr[v];
-  std::vector<int32_t> vf(vfptr[nv]);
+  std::vector<int32_t> vf(size_t(vfptr[nv]));
   {
     std::vector<int64_t> fill(vfptr.begin(), vfptr.end() - 1);
     for (int64_t f = 0; f < nf; ++f) {
       const int32_t* t = tri + 3 * f;
       if (t[0] == t[1]) continue;
-      for (int j = 0; j < 3; ++j) {
-        // a face listing a vertex twice is stored once for it
-        if (j > 0 && t[j] == t[0]) continue;
-        if (j > 1 && t[j] == t[1]) continue;
-        vf[fill[t[j]]++] = int32_t(f);
-      }
+      for (int j = 0; j < 3; ++j)
+        if (slot_of(t, j)) vf[size_t(fill[t[j]]++)] = int32_t(f);
     }
-    // drop the slots of repeated vertices
-    std::vector<int64_t> np(nv + 1, 0);
-    std::vector<int32_t> nvf;
-    nvf.reserve(vf.size());
-    for (int64_t v = 0; v < nv; ++v) {
-      for (int64_t k = vfptr[v]; k < fill[v]; ++k) nvf.push_back(vf[k]);
-      np[v + 1] = int64_t(nvf.size());
-    }
-    vfptr.swap(np);
-    vf.swap(nvf);
   }
   // rings in edge-creation order: scanning the faces of v in increasing id and
   // each face's edges in order, the first sighting of neighbour u is the
-  // creation of edge (v,u), which is when it joins both rings
-  std::vector<int64_t> rptr(nv + 1, 0);
-  std::vector<int32_t> ring;
-  ring.reserve(size_t(vfptr[nv]) * 2);
-  for (int64_t v = 0; v < nv; ++v) {
-    const size_t start = ring.size();
+  // creation of edge (v,u), which is when it joins both rings.  Two passes:
+  // sizes, then the rings in place.
+  auto ring_of = [&](int64_t v, int32_t* out) {
+    int64_t cnt = 0;
     for (int64_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
-      const int32_t* t = tri + 3 * int64_t(vf[k]);
+      const int32_t* t = tri + 3 * int64_t(vf[size_t(k)]);
       for (int j = 0; j < 3; ++j) {
         const int32_t a = t[j], b = t[(j + 1) % 3];
         if (a == b) continue;
@@ -93,37 +99,59 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
         else if (b == v) u = a;
         if (u < 0) continue;
         bool seen = false;
-        for (size_t s = start; s < ring.size(); ++s)
-          if (ring[s] == u) {
+        for (int64_t q = 0; q < cnt; ++q)
+          if (out[q] == u) {
             seen = true;
             break;
           }
-        if (!seen) ring.push_back(u);
+        if (!seen) out[cnt++] = u;
       }
     }
-    rptr[v + 1] = int64_t(ring.size());
-  }
-  // metric weights
+    return cnt;
+  };
+  std::vector<int64_t> rptr(nv + 1, 0);
+  parallel_for(nv, [&](int64_t v0, int64_t v1) {
+    std::vector<int32_t> tmp;
+    for (int64_t v = v0; v < v1; ++v) {
+      tmp.resize(size_t(2 * (vfptr[v + 1] - vfptr[v])));
+      rptr[v + 1] = ring_of(v, tmp.data());
+    }
+  });
+  for (int64_t v = 0; v < nv; ++v) rptr[v + 1] += rptr[v];
+  std::vector<int32_t> ring(size_t(rptr[nv]));
+  parallel_for(nv, [&](int64_t v0, int64_t v1) {
+    for (int64_t v = v0; v < v1; ++v) ring_of(v, ring.data() + rptr[v]);
+  });
+  // metric weights: vertex area = sum, in GetVertexNeighbourFaces order (the
+  // faces of each ring neighbour in turn), of the face areas / 3
+  std::vector<double> farea(static_cast<size_t>(nf));
+  parallel_for(nf, [&](int64_t f0, int64_t f1) {
+    for (int64_t f = f0; f < f1; ++f) farea[size_t(f)] = face_area(xyz, tri + 3 * f) / 3.0;
+  });
   std::vector<double> w(nv, 0.0);
-  std::vector<int32_t> nfaces;
-  for (int64_t v = 0; v < nv; ++v) {
-    nfaces.clear();
-    for (int64_t r = rptr[v]; r < rptr[v + 1]; ++r) {
-      const int32_t u = ring[r];
-      for (int64_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
-        const int32_t f = vf[k];
-        const int32_t* t = tri + 3 * int64_t(f);
-        if (t[0] != u && t[1] != u && t[2] != u) continue;
-        if (std::find(nfaces.begin(), nfaces.end(), f) == nfaces.end()) nfaces.push_back(f);
+  parallel_for(nv, [&](int64_t v0, int64_t v1) {
+    std::vector<int32_t> nfaces;
+    for (int64_t v = v0; v < v1; ++v) {
+      nfaces.clear();
+      for (int64_t r = rptr[v]; r < rptr[v + 1]; ++r) {
+        const int32_t u = ring[size_t(r)];
+        for (int64_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
+          const int32_t f = vf[size_t(k)];
+          const int32_t* t = tri + 3 * int64_t(f);
+          if (t[0] != u && t[1] != u && t[2] != u) continue;
+          if (std::find(nfaces.begin(), nfaces.end(), f) == nfaces.end()) nfaces.push_back(f);
+        }
       }
+      double a = 0.0;
+      for (int32_t f : nfaces) a += farea[size_t(f)];
+      w[v] = a;
     }
-    double a = 0.0;
-    for (int32_t f : nfaces) a += face_area(xyz, tri + 3 * int64_t(f)) / 3.0;
-    w[v] = a;
-  }
+  });
   const bool qem = metric == 1;
   if (indicator && (!qem || gradation > 0))
-    for (int64_t v = 0; v < nv; ++v) w[v] *= std::pow(indicator[v], gradation);
+    parallel_for(nv, [&](int64_t v0, int64_t v1) {
+      for (int64_t v = v0; v < v1; ++v) w[v] *= std::pow(indicator[v], gradation);
+    });
   {
     double avg = 0.0;
     for (double x : w) avg += x;
@@ -151,7 +179,9 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
   int64_t remaining_items = n;
   int32_t remaining_regions = K - offset;
   int64_t first = 0;
-  std::deque<int32_t> queue;
+  // FIFO of one region as a flat array
+  std::vector<int32_t> queue;
+  queue.reserve(1024);
   while (remaining_items > 0 && remaining_regions > 0) {
     int32_t item = -1;
     while (first < n) {
@@ -164,16 +194,16 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
     if (item < 0) break;  // every item labelled
     queue.clear();
     queue.push_back(item);
+    size_t head = 0;
     sw = 0.0;
     --remaining_regions;
-    while (!queue.empty()) {
-      const int32_t it = queue.front();
-      queue.pop_front();
+    while (head < queue.size()) {
+      const int32_t it = queue[head++];
       if (labels[it] != K) continue;
       labels[it] = remaining_regions + offset;
       sw += w[it];
       --remaining_items;
-      for (int64_t r = rptr[it]; r < rptr[it + 1]; ++r) queue.push_back(ring[r]);
+      for (int64_t r = rptr[it]; r < rptr[it + 1]; ++r) queue.push_back(ring[size_t(r)]);
       if (sw > target) break;
     }
   }

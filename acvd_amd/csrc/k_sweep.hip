@@ -1395,6 +1395,66 @@ __global__ void k_item_energy(SweepArgs a, const double* __restrict__ pts_all, d
   e_item[i] = double(p.w) * (dx * dx + dy * dy + dz * dz);
 }
 
+// Energy log, step 1 (WriteToGlobalEnergyLog, vtkUniformClustering.h:1383-1395):
+// the energy of every cluster from its exact accumulators, the point solved
+// from scratch (no warm start, so the value depends on the labels only).
+__global__ void k_log_cluster_energy(SweepArgs a, double* __restrict__ e) {
+  int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= a.n_clusters) return;
+  const int m = mesh_of_cluster(a, c);
+  const DevMesh dm = a.meshes[m];
+  const int64_t W = ld_i64(&a.acc_ws[4 * c]);
+  const ClusterGeom g = geom_from_fixed(W, ld_i64(&a.acc_ws[4 * c + 1]), ld_i64(&a.acc_ws[4 * c + 2]),
+                                        ld_i64(&a.acc_ws[4 * c + 3]), dm.scales);
+  double T = 0.0;
+  if (dm.metric == 1) {
+    const int32_t an = a.anchor ? a.anchor[c] : -1;
+    if (an >= 0) {
+      const float4 p = a.xw[an];
+      const double ap[3] = {double(p.x), double(p.y), double(p.z)};
+      T = anchored_T(g, ap);
+    } else if (a.state[m].constrained) {
+      int64_t q10[10];
+      for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
+      T = qem_T_from_fixed(q10, g, dm.scales, dm.level, nullptr);
+    }
+  }
+  e[c] = log_cluster_energy(g, T);
+}
+
+// Energy log, step 2: one block per mesh sums its clusters in the order of
+// kLogThreads (acvd_math.h).  In a pass (final_e null) a sweeping mesh writes
+// its energy and the device wall clock to entry `loops % ring` of its ring of
+// log entries, before the controller counts the pass; the final call writes
+// every mesh's energy to final_e.
+__global__ void __launch_bounds__(kLogThreads) k_log_reduce(SweepArgs a, const double* __restrict__ e,
+                                                            EnergyLogEntry* log, int32_t ring, double* final_e) {
+  const int m = blockIdx.x;
+  const int32_t phase = a.state[m].phase, loop = a.state[m].loops;
+  if (!final_e && (phase != kPhaseSweep || loop < 0)) return;  // uniform over the block
+  const DevMesh dm = a.meshes[m];
+  __shared__ double part[kLogThreads];
+  double s = 0.0;
+  for (int64_t r = threadIdx.x; r < dm.K; r += kLogThreads) s += e[dm.clus_off + r];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kLogThreads / 2; h > 0; h >>= 1) {
+    if (int(threadIdx.x) < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (final_e) {
+      final_e[m] = part[0];
+    } else {
+      EnergyLogEntry& x = log[int64_t(m) * ring + loop % ring];
+      x.energy = part[0];
+      x.clock = (unsigned long long)wall_clock64();
+    }
+  }
+}
+
+__global__ void k_clock_stamp(unsigned long long* t) { *t = (unsigned long long)wall_clock64(); }
+
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1541,6 +1601,20 @@ hipError_t launch_cluster_points(hipStream_t s, const SweepArgs& a, int32_t mesh
 hipError_t launch_item_energy(hipStream_t s, const SweepArgs& a, const double* pts_all, double* e_item) {
   if (a.n_items == 0) return hipSuccess;
   hipLaunchKernelGGL(k_item_energy, dim3(grid_for(a.n_items)), dim3(kBlock), 0, s, a, pts_all, e_item);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_log(hipStream_t s, const SweepArgs& a, double* e_cluster, EnergyLogEntry* log, int32_t ring,
+                             double* final_e) {
+  if (a.n_clusters == 0 || a.n_meshes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_log_cluster_energy, dim3(grid_for(a.n_clusters)), dim3(kBlock), 0, s, a, e_cluster);
+  hipLaunchKernelGGL(k_log_reduce, dim3(unsigned(a.n_meshes)), dim3(kLogThreads), 0, s, a, e_cluster, log, ring,
+                     final_e);
+  return hipGetLastError();
+}
+
+hipError_t launch_clock_stamp(hipStream_t s, unsigned long long* t) {
+  hipLaunchKernelGGL(k_clock_stamp, dim3(1), dim3(1), 0, s, t);
   return hipGetLastError();
 }
 

@@ -169,6 +169,17 @@ class Context:
         check(N.lib().acvd_get_energy(self._h, mesh_id, _p(e)))
         return float(e[0])
 
+    def energy_log(self, mesh_id=0):
+        """(seconds, energy, final_energy) of the last run with
+        write_energy_log set: one entry per loop (acvd_get_energy_log)."""
+        n = ctypes.c_int64()
+        check(N.lib().acvd_get_energy_log(self._h, mesh_id, 0, ctypes.byref(n), None, None, None))
+        sec = np.empty(n.value, np.float64)
+        en = np.empty(n.value, np.float64)
+        fin = np.zeros(1, np.float64)
+        check(N.lib().acvd_get_energy_log(self._h, mesh_id, n.value, ctypes.byref(n), _p(sec), _p(en), _p(fin)))
+        return sec, en, float(fin[0])
+
     def stats(self, mesh_id=0):
         st = N.Stats()
         check(N.lib().acvd_get_stats(self._h, mesh_id, ctypes.byref(st)))
@@ -218,6 +229,24 @@ def reference_initial_sampling(xyz, tri, n_clusters, metric=N.METRIC_ISOTROPIC, 
     return out
 
 
+def subdivide(points, triangles):
+    """One level of midpoint subdivision (vtkSurface::Subdivide,
+    Common/vtkSurface.cxx:605-677) on the host: (points, triangles, parents)
+    with the input vertices first, then one midpoint per edge in the
+    reference's edge order; parents[k] = the two ends of new vertex nv + k."""
+    x = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    onv, onf = ctypes.c_int64(), ctypes.c_int64()
+    check(N.lib().acvd_subdivide(len(x), _p(x), len(t), _p(t), ctypes.byref(onv), ctypes.byref(onf), None, None,
+                                 None))
+    ox = np.empty((onv.value, 3), np.float64)
+    ot = np.empty((onf.value, 3), np.int32)
+    par = np.empty((onv.value - len(x), 2), np.int32)
+    check(N.lib().acvd_subdivide(len(x), _p(x), len(t), _p(t), ctypes.byref(onv), ctypes.byref(onf), _p(ox), _p(ot),
+                                 _p(par)))
+    return ox, ot, par
+
+
 class _MetricProxy:
     """What GetMetric() returns: the metric-dependent options."""
 
@@ -258,6 +287,7 @@ class _DiscreteRemeshing:
         self._subsampling_threshold = 10
         self._output = None
         self._metric = _MetricProxy(self)
+        self._output_directory = None
 
     @classmethod
     def New(cls):
@@ -310,6 +340,18 @@ class _DiscreteRemeshing:
         """The cluster record, for its AnchorItem (the fixed item, or -1)."""
         return _ClusterProxy(self, int(i))
 
+    def SetWriteToGlobalEnergyLog(self, v):
+        """vtkUniformClustering::SetWriteToGlobalEnergyLog (vtkUniformClustering.h:123):
+        ProcessClustering then writes energy.txt, prefixed by the output
+        directory as given (no separator is added, :670-675): one line
+        "loop seconds energy" per loop and "Final Energy :<energy>" after the
+        run (:695-699, 1383-1395)."""
+        self._params["write_energy_log"] = int(v)
+
+    def SetOutputDirectory(self, d):
+        """vtkUniformClustering::SetOutputDirectory (vtkUniformClustering.h:126)."""
+        self._output_directory = None if d is None else str(d)
+
     def SetSubsamplingThreshold(self, t):
         self._subsampling_threshold = int(t)
 
@@ -349,24 +391,73 @@ class _DiscreteRemeshing:
         ctx.set_fixed_clusters(0, self._fixed)
         ctx.process()
         self._clustering = ctx.labels(0)
+        if self._params.get("write_energy_log"):
+            self._write_energy_log()
         return self._clustering
+
+    def _write_energy_log(self):
+        """The file of WriteToGlobalEnergyLog: loop number, seconds since the
+        start of the minimisation (6 significant digits), energy (15); then the
+        final energy.  Like the reference's ofstream, a file that cannot be
+        opened is skipped silently."""
+        sec, en, fin = self._ctx.energy_log(0)
+        name = (self._output_directory or "") + "energy.txt"
+        lines = ["%d %.6g %.15g\n" % (i, t, e) for i, (t, e) in enumerate(zip(sec, en))]
+        lines.append("Final Energy :%.15g\n" % fin)
+        try:
+            with open(name, "w") as f:
+                f.writelines(lines)
+        except OSError:
+            pass
 
     def GetClustering(self):
         return self._clustering
 
     def Remesh(self):
         """vtkDiscreteRemeshing::Remesh (vtkDiscreteRemeshing.h:877-954):
+        CheckSubsamplingRatio (:840-875: while the input has fewer than
+        SubsamplingThreshold * K vertices it is replaced by its midpoint
+        subdivision, and GetInput() returns the subdivided mesh),
         SamplingPreProcessing (with a positive gradation and no indicator set,
         the items are weighted by the reference's curvature indicator, computed
-        on the device, :583-653), ProcessClustering, then the output
-        triangulation.  Subdivision of meshes with fewer than
-        SubsamplingThreshold * K vertices, ForceManifold and boundary fixing
-        are not part of this path."""
-        n = self.GetNumberOfItems()
-        if n < self._subsampling_threshold * self._params["number_of_clusters"]:
-            raise AcvdError(f"input has {n} vertices, fewer than SubsamplingThreshold * NumberOfClusters; "
-                            "subdivide the input first")
-        if self.ProcessClustering(curvature_indicator=1) is None:
+        on the device on the original input and interpolated to the midpoints
+        as the mean of their two parents, :583-653, 700-745),
+        ProcessClustering, then the output triangulation.  ForceManifold and
+        boundary fixing are not part of this path."""
+        K = self._params["number_of_clusters"]
+        if K <= 0:
+            return
+        original = self._input
+        mesh = original
+        chain = []
+        while mesh.GetNumberOfPoints() < self._subsampling_threshold * K:
+            if self._params["console_output"]:
+                print("Subdividing mesh")
+            x, t, par = subdivide(mesh.points, mesh.triangles)
+            mesh = Mesh(x, t)
+            chain.append(par)
+        curvature = 1
+        if chain:
+            if self._initial is not None:
+                raise AcvdError("SetInitialClustering labels the input's vertices; the input needs subdivision")
+            ind = self._indicator
+            if ind is None and self._params["gradation"] > 0:
+                c = Context()
+                c.set_input(original.points, original.triangles)
+                c.set_params(0, number_of_clusters=1, metric=self.METRIC, gradation=self._params["gradation"],
+                             curvature_indicator=1)
+                c.build()
+                ind = c.indicator(0)
+                c.close()
+            if ind is not None:
+                ind = np.asarray(ind, np.float64)
+                for par in chain:
+                    ind = np.concatenate([ind, 0.5 * (ind[par[:, 0]] + ind[par[:, 1]])])
+            self._original_input = original
+            self._input = mesh
+            self._indicator = ind
+            curvature = 0
+        if self.ProcessClustering(curvature_indicator=curvature) is None:
             return
         xyz, tri = self._ctx.output(0, 0)
         self._output = Mesh(xyz, tri)

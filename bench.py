@@ -45,6 +45,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(group):
+    """HBM bytes per launch of a kernel group from the committed PMC summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload, copy-calibrated),
+    or None when the summary does not hold the group."""
+    path = os.path.join(ROOT, "profiles", "c3_pmc_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][group]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -199,7 +211,9 @@ def run_gpu_leg(args):
         "N": N, "K": K, "init": init, "labels": labels, "dev_ms": dev_ms, "wall_ms": wall_ms, "stats": st,
         "energy_gpu": energy, "xyz": xyz, "tri": tri, "H": H, "init_ms": init_ms, "serial": serial,
         "roofline": {"bound": "hbm", "achieved": gd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gd["frac"], "traffic": None, "kernel": names[dom],
+                     "frac": gd["frac"], "traffic": measured_traffic(names[dom]),
+                     "traffic_source": "profiles/c3_pmc_traffic.json", "kernel": names[dom],
+                     "bytes_per_launch": round(gd["bytes"] / gd["launches"]),
                      "bytes_per_clustering": gd["bytes"], "ms_per_clustering": gd["ms"], "launches": gd["launches"],
                      "groups": groups},
     }

@@ -17,10 +17,12 @@
 //   GetMetric()->SetGradation / SetQuadricsOptimizationLevel
 //     (vtkIsotropicMetricForClustering.h:73-76, vtkQEMetricForClustering.h:50-54)  same
 //   SetSubsamplingThreshold (vtkDiscreteRemeshing.h:71)     same
+//   SetWriteToGlobalEnergyLog (vtkUniformClustering.h:123),
+//   SetOutputDirectory (:126)                               same (energy.txt written by ProcessClustering)
 //   ProcessClustering() -> vtkIntArray* (vtkUniformClustering.h:75)
 //                                                           ProcessClustering() -> const Labels*
 //   GetClustering() (:78)                                   same (borrowed, owned by the object)
-//   Remesh() (vtkDiscreteRemeshing.h:64, 877-954)           same (no subdivision: fails instead)
+//   Remesh() (vtkDiscreteRemeshing.h:64, 877-954)           same (subdivides coarse inputs; no ForceManifold)
 //   GetOutput() -> vtkSurface* (:61)                        GetOutput() -> const Mesh* (borrowed)
 //
 // Error behaviour follows the reference: no exceptions; a failing call
@@ -89,6 +91,11 @@ class DiscreteRemeshing {
     return fixed_ && i >= 0 && size_t(i) < fixed_->size() ? (*fixed_)[size_t(i)] : -1;
   }
   void SetSubsamplingThreshold(int t) { subsampling_threshold_ = t; }
+  // energy.txt in the output directory (prefixed as given, no separator
+  // added, vtkUniformClustering.h:670-675): "loop seconds energy" per loop,
+  // then "Final Energy :<energy>" (:695-699, 1383-1395)
+  void SetWriteToGlobalEnergyLog(int v) { params_.write_energy_log = v; }
+  void SetOutputDirectory(const char* d) { output_directory_ = d ? d : ""; }
   void SetSeed(uint32_t s) { params_.seed = s; }
   void SetDevice(int d) { device_ = d; }
   MetricHandle* GetMetric() { return &metric_; }
@@ -119,6 +126,7 @@ class DiscreteRemeshing {
     clustering_.resize(size_t(nv));
     if (!ok(acvd_get_labels(ctx_, 0, clustering_.data()))) return nullptr;
     clustered_ = true;
+    if (params_.write_energy_log && !write_energy_log()) return nullptr;
     return &clustering_;
   }
   const Labels* GetClustering() const { return clustered_ ? &clustering_ : nullptr; }
@@ -127,12 +135,58 @@ class DiscreteRemeshing {
   // curvature indicator when the gradation is positive and none was set,
   // :583-653), clustering, then the output mesh.  Returns 1 on success, 0 on
   // failure.
+  // CheckSubsamplingRatio (vtkDiscreteRemeshing.h:840-875): while the input
+  // has fewer than SubsamplingThreshold * K vertices it is replaced by its
+  // midpoint subdivision (acvd_subdivide), which GetInput() then returns; the
+  // curvature indicator is computed on the original input and interpolated
+  // to the midpoints as the mean of their parents (:700-745).
   int Remesh() {
-    if (input_ && GetNumberOfItems() < int64_t(subsampling_threshold_) * params_.number_of_clusters)
-      return fail("input has fewer than SubsamplingThreshold * NumberOfClusters vertices; subdivide it first"), 0;
-    if (!ProcessClustering(1)) return 0;
+    if (!input_) return fail("no input"), 0;
+    const Mesh* mesh = input_;
+    std::vector<std::vector<int32_t>> chain;
+    while (mesh->GetNumberOfPoints() < int64_t(subsampling_threshold_) * params_.number_of_clusters) {
+      if (params_.console_output) std::printf("Subdividing mesh\n");
+      Mesh next;
+      std::vector<int32_t> parents;
+      if (!subdivide(*mesh, &next, &parents)) return 0;
+      subdivided_ = std::move(next);
+      mesh = &subdivided_;
+      chain.push_back(std::move(parents));
+    }
+    if (chain.empty()) {
+      if (!ProcessClustering(1)) return 0;
+      return BuildOutput(0);
+    }
+    if (initial_) return fail("the initial clustering labels the input's vertices; the input needs subdivision"), 0;
+    std::vector<double> ind;
+    if (indicator_) {
+      ind = *indicator_;
+    } else if (params_.gradation > 0) {  // curvature of the original input
+      acvd_ctx* c = nullptr;
+      if (!ok(acvd_create(&c, device_))) return 0;
+      acvd_params p = params_;
+      p.metric = METRIC;
+      p.number_of_clusters = 1;
+      p.curvature_indicator = 1;
+      ind.resize(size_t(input_->GetNumberOfPoints()));
+      const bool good = ok(acvd_set_input_f64(c, input_->GetNumberOfPoints(), input_->points.data(),
+                                              input_->GetNumberOfCells(), input_->triangles.data())) &&
+                        ok(acvd_set_params(c, 0, &p)) && ok(acvd_build(c)) && ok(acvd_get_indicator(c, 0, ind.data()));
+      acvd_destroy(c);
+      if (!good) return 0;
+    }
+    if (!ind.empty())
+      for (const auto& par : chain)
+        for (size_t k = 0; k < par.size() / 2; ++k) ind.push_back(0.5 * (ind[size_t(par[2 * k])] + ind[size_t(par[2 * k + 1])]));
+    original_ = input_;
+    input_ = &subdivided_;
+    interpolated_ = std::move(ind);
+    if (!interpolated_.empty()) indicator_ = &interpolated_;
+    if (!ProcessClustering(0)) return 0;
     return BuildOutput(0);
   }
+  // the input Remesh() replaced by its subdivision, or nullptr
+  const Mesh* GetOriginalInput() const { return original_; }
   // Output mesh with vertices moved by the quadric placement of
   // Examples/ACVD.cxx:217-291 at `level` (0 = cluster points).
   int BuildOutput(int level) {
@@ -158,6 +212,34 @@ class DiscreteRemeshing {
   DiscreteRemeshing& operator=(const DiscreteRemeshing&) = delete;
 
  private:
+  bool subdivide(const Mesh& in, Mesh* out, std::vector<int32_t>* parents) {
+    int64_t nv = 0, nf = 0;
+    if (!ok(acvd_subdivide(in.GetNumberOfPoints(), in.points.data(), in.GetNumberOfCells(), in.triangles.data(), &nv,
+                           &nf, nullptr, nullptr, nullptr)))
+      return false;
+    out->points.resize(size_t(3 * nv));
+    out->triangles.resize(size_t(3 * nf));
+    parents->resize(size_t(2 * (nv - in.GetNumberOfPoints())));
+    return ok(acvd_subdivide(in.GetNumberOfPoints(), in.points.data(), in.GetNumberOfCells(), in.triangles.data(), &nv,
+                             &nf, out->points.data(), out->triangles.data(), parents->data()));
+  }
+  // The file of WriteToGlobalEnergyLog: loop, seconds (6 significant
+  // digits), energy (15); a file that cannot be opened is skipped, as the
+  // reference's ofstream does.
+  bool write_energy_log() {
+    int64_t n = 0;
+    double fin = 0.0;
+    if (!ok(acvd_get_energy_log(ctx_, 0, 0, &n, nullptr, nullptr, nullptr))) return false;
+    std::vector<double> sec(static_cast<size_t>(n)), en(static_cast<size_t>(n));
+    if (!ok(acvd_get_energy_log(ctx_, 0, n, &n, sec.data(), en.data(), &fin))) return false;
+    const std::string name = output_directory_ + "energy.txt";
+    std::FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) return true;
+    for (int64_t i = 0; i < n; ++i) std::fprintf(f, "%lld %.6g %.15g\n", (long long)i, sec[size_t(i)], en[size_t(i)]);
+    std::fprintf(f, "Final Energy :%.15g\n", fin);
+    std::fclose(f);
+    return true;
+  }
   bool ensure_ctx() {
     if (ctx_) return true;
     return ok(acvd_create(&ctx_, device_));
@@ -177,8 +259,12 @@ class DiscreteRemeshing {
   const Mesh* input_ = nullptr;
   const Labels* initial_ = nullptr;
   const std::vector<int32_t>* fixed_ = nullptr;
+  const Mesh* original_ = nullptr;
+  Mesh subdivided_;
+  std::vector<double> interpolated_;
   const std::vector<double>* indicator_ = nullptr;
   int subsampling_threshold_ = 10;
+  std::string output_directory_;
   int device_ = 0;
   acvd_ctx* ctx_ = nullptr;
   Labels clustering_;

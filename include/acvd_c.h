@@ -62,6 +62,8 @@ typedef struct acvd_ctx acvd_ctx;
  *   first_phase_rounds          <- (no counterpart) accept rounds per pass until the first convergence,
  *                                  1..12; 0 = the library's rule: 1 for meshes of at most 2^20 vertices
  *                                  (lower final energy), 12 above (fewer passes); later phases run 12
+ *   write_energy_log            <- SetWriteToGlobalEnergyLog (vtkUniformClustering.h:123): nonzero = record the
+ *                                  energy after every loop and after the run (acvd_get_energy_log)
  */
 typedef struct acvd_params {
   int32_t number_of_clusters;
@@ -75,6 +77,7 @@ typedef struct acvd_params {
   int32_t initial_sampling;
   int32_t curvature_indicator;
   int32_t first_phase_rounds;
+  int32_t write_energy_log;
   double gradation;
 } acvd_params;
 
@@ -201,6 +204,20 @@ int acvd_get_cluster_points(acvd_ctx* ctx, int32_t mesh_id, double* xyz);
 /* Unshifted clustering energy sum_i w_i |x_i - p(c_i)|^2 of the current
  * labels, accumulated in FP64 on the device (diagnostic). */
 int acvd_get_energy(acvd_ctx* ctx, int32_t mesh_id, double* energy);
+/* Energy log of the last acvd_process_clustering() of a mesh whose params
+ * had write_energy_log set: what WriteToGlobalEnergyLog writes after each loop
+ * (vtkUniformClustering.h:1383-1395; the loop number is the entry's index) and
+ * the "Final Energy :" line after the run (:695-699).  Each energy is the sum
+ * of the cluster energies GetClusterEnergy returns (-|S|^2/W isotropic,
+ * W|p|^2 - 2 p.S QEM; vtkIsotropicMetricForClustering.h:114-121,
+ * vtkQEMetricForClustering.h:194-208), summed in FP64 in a fixed order (the
+ * reference: long double, ComputeGlobalEnergy :1318-1346); seconds is the
+ * device wall clock since the start of the minimisation (Time - StartTime).
+ * *n_loops receives the number of entries; min(n_loops, capacity) of them are
+ * written to seconds / energy (either may be null); final_energy may be null.
+ * ACVD_ERR_STATE when the last run did not record a log for this mesh. */
+int acvd_get_energy_log(acvd_ctx* ctx, int32_t mesh_id, int64_t capacity, int64_t* n_loops, double* seconds,
+                        double* energy, double* final_energy);
 int acvd_get_stats(acvd_ctx* ctx, int32_t mesh_id, acvd_stats* st);
 
 /* The reference's default initial sampling (InitSamples type 1,
@@ -210,6 +227,21 @@ int acvd_get_stats(acvd_ctx* ctx, int32_t mesh_id, acvd_stats* st);
  * be NULL.  acvd_initial_sampling() uses it when initial_sampling == 0. */
 int acvd_reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int32_t metric,
                                     const double* indicator, double gradation, int32_t n_clusters, int32_t* labels);
+
+/* vtkSurface::Subdivide (Common/vtkSurface.cxx:605-677), the step Remesh()
+ * repeats while the input has fewer than SubsamplingThreshold * K vertices
+ * (vtkDiscreteRemeshing::CheckSubsamplingRatio, vtkDiscreteRemeshing.h:840-875):
+ * one level of midpoint subdivision on the host.  The output holds the nv
+ * input vertices, then one vertex per edge at its midpoint, edges in the order
+ * the reference creates them (face by face, edges (v0,v1), (v1,v2), (v2,v0),
+ * Common/vtkSurfaceBase.cxx:1166-1221), and four faces per input face:
+ * (v1,m12,m31), (m12,v2,m23), (m23,v3,m31), (m12,m23,m31).
+ * parents[2k], parents[2k+1] receive the ends of the edge of new vertex nv+k
+ * (VerticesParent1/2).  Pass NULL output buffers to get the sizes first
+ * (*out_nv = nv + edges, *out_nf = 4 nf); out_xyz[3*out_nv], out_tri[3*out_nf],
+ * parents[2*(out_nv - nv)]. */
+int acvd_subdivide(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int64_t* out_nv, int64_t* out_nf,
+                   double* out_xyz, int32_t* out_tri, int32_t* parents);
 
 /* ---- output mesh ---------------------------------------------------------- */
 /* BuildDelaunayTriangulation (vtkDiscreteRemeshing.h:1002-1152, vertex

@@ -46,6 +46,10 @@ int g_progress = 0;
 // fixed clusters of the next runs and samplings (orc_set_fixed): cluster k
 // holds item g_fixed[k]
 std::vector<int32_t> g_fixed;
+// energy log of the serial runs (orc_set_energy_log / orc_serial_energy_log)
+int g_energy_log = 0;
+std::vector<long double> g_serial_log;
+long double g_serial_final = 0.0L;
 
 template <class M>
 int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int level, int unconstrained,
@@ -57,6 +61,8 @@ int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int
   cl.edge_trace = etrace;
   cl.edge_trace_cap = ecap;
   cl.loop_trace = ltrace;
+  g_serial_log.clear();
+  if (g_energy_log) cl.energy_log = &g_serial_log;
   if constexpr (M::kQEM) cl.metric.level = level;
   cl.set_number_of_clusters(K);
   // QEM: GetCluster(k)->AnchorItem = fixed item k (Examples/ACVDQ.cxx:333-334)
@@ -69,6 +75,7 @@ int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int
     cl.initial_labels = init;
   }
   cl.process(ind, gradation);
+  g_serial_final = cl.final_energy;
   for (int64_t i = 0; i < m.nv; ++i) out[i] = cl.labels[i];
   if (st) {
     st->loops = cl.stats.loops;
@@ -230,6 +237,16 @@ int orc_serial_run(void* mh, int metric, int K, double gradation, const double* 
 void orc_set_progress(int on) { g_progress = on; }
 
 void orc_set_fixed(const int32_t* ids, int64_t n) { g_fixed.assign(ids, ids + (ids ? n : 0)); }
+
+void orc_set_energy_log(int on) { g_energy_log = on; }
+
+// The energy log of the last serial run (long double values as double).
+int64_t orc_serial_energy_log(int64_t cap, double* energy, double* final_energy) {
+  const int64_t n = int64_t(g_serial_log.size());
+  for (int64_t i = 0; i < n && i < cap; ++i) energy[i] = double(g_serial_log[size_t(i)]);
+  if (final_energy) *final_energy = double(g_serial_final);
+  return n;
+}
 
 // orc_serial_run with its trace: edge records (loop, decision, edge, i1, i2 in
 // edge_i[5k..]; try1..try3 in edge_d[3k..]) up to edge_cap, loop records
@@ -431,7 +448,19 @@ uint32_t orc_mt19937_10000(void) {
 // Replay C.  params: K, metric, level, seed, unconstrained, max_loops,
 // max_convergences.  scales3 = {sw, ss, sq}.  init/out in the reference
 // convention (K = null).
-// path: bit 0 generic QEM rule, bit 1 every round (no end after an empty round)
+// Energy log of the last replay run with path bit 16 set.
+static std::vector<double> g_replay_log;
+static double g_replay_final = 0.0;
+
+int64_t orc_replay_energy_log(int64_t cap, double* energy, double* final_energy) {
+  const int64_t n = int64_t(g_replay_log.size());
+  for (int64_t i = 0; i < n && i < cap; ++i) energy[i] = g_replay_log[size_t(i)];
+  if (final_energy) *final_energy = g_replay_final;
+  return n;
+}
+
+// path: bit 0 generic QEM rule, bit 1 every round (no end after an empty round),
+// bit 16 record the energy log
 int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* col, const float* xw,
                         const float* q, const double* scales3, int K, int metric, int level, uint32_t seed,
                         int unconstrained, int max_loops, int max_conv, const int32_t* init, int32_t* out,
@@ -463,7 +492,10 @@ int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* c
   r.sc.isw = 1.0 / r.sc.sw;
   r.sc.iss = 1.0 / r.sc.ss;
   r.sc.isq = 1.0 / r.sc.sq;
+  r.log_energy = (path >> 16) & 1;
   r.run(init);
+  g_replay_log = r.energy_log;
+  g_replay_final = r.final_energy;
   for (int64_t i = 0; i < n; ++i) out[i] = r.labels[i] == acvd::kNull ? K : r.labels[i];
   if (stats6) {
     stats6[0] = r.stats.loops;

@@ -75,6 +75,11 @@ def lib():
         L.orc_replay_run.argtypes = [ctypes.c_int64, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
         L.orc_replay_run_path.argtypes = [ctypes.c_int] + L.orc_replay_run.argtypes
+        L.orc_replay_energy_log.restype = ctypes.c_int64
+        L.orc_replay_energy_log.argtypes = [ctypes.c_int64, P, P]
+        L.orc_set_energy_log.argtypes = [ctypes.c_int]
+        L.orc_serial_energy_log.restype = ctypes.c_int64
+        L.orc_serial_energy_log.argtypes = [ctypes.c_int64, P, P]
         L.orc_devinputs_create.restype = P
         L.orc_devinputs_create.argtypes = [ctypes.c_int64, P, ctypes.c_int64, P, ctypes.c_int, P, ctypes.c_double]
         L.orc_devinputs_ncol.restype = ctypes.c_int64
@@ -229,6 +234,21 @@ def set_fixed(items):
     lib().orc_set_fixed(_p(ids) if len(ids) else None, len(ids))
 
 
+def set_energy_log(on):
+    """Serial runs (oracle A) record ComputeGlobalEnergy after each loop and
+    after the run (SetWriteToGlobalEnergyLog); read with serial_energy_log()."""
+    lib().orc_set_energy_log(1 if on else 0)
+
+
+def serial_energy_log():
+    """(energy per loop, final energy) of the last serial run."""
+    n = lib().orc_serial_energy_log(0, None, None)
+    e = np.empty(n, np.float64)
+    fin = np.zeros(1, np.float64)
+    lib().orc_serial_energy_log(n, _p(e), _p(fin))
+    return e, float(fin[0])
+
+
 def set_progress(on):
     """per-loop progress lines of the serial runs on stderr"""
     lib().orc_set_progress(int(bool(on)))
@@ -286,7 +306,9 @@ def hdr_accept_key(seed, loop, item, round=0):
 
 
 def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, unconstrained=0, max_loops=5000000,
-               max_conv=1000000000, generic_qem=False, all_rounds=False, first_rounds=0):
+               max_conv=1000000000, generic_qem=False, all_rounds=False, first_rounds=0, energy_log=False):
+    """Replay C.  With energy_log the stats also hold "energy_log" (energy
+    after each pass) and "final_energy", summed as the library sums them."""
     n = len(row_ptr) - 1
     out = np.empty(n, np.int32)
     st = np.zeros(6, np.int64)
@@ -296,10 +318,19 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
     qq = None if q is None else np.ascontiguousarray(q, np.float32)
     sc = np.ascontiguousarray(scales, np.float64)
     ini = np.ascontiguousarray(init, np.int32)
-    lib().orc_replay_run_path(int(generic_qem) | (int(all_rounds) << 1) | (int(first_rounds) << 8), n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
+    path = int(generic_qem) | (int(all_rounds) << 1) | (int(first_rounds) << 8) | (int(bool(energy_log)) << 16)
+    lib().orc_replay_run_path(path, n, _p(rp), _p(cl), _p(x), _p(qq), _p(sc), K, metric, level, seed,
                               unconstrained, max_loops, max_conv, _p(ini), _p(out), _p(st))
     keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
-    return out, dict(zip(keys, (int(v) for v in st)))
+    stats = dict(zip(keys, (int(v) for v in st)))
+    if energy_log:
+        m = lib().orc_replay_energy_log(0, None, None)
+        e = np.empty(m, np.float64)
+        fin = np.zeros(1, np.float64)
+        lib().orc_replay_energy_log(m, _p(e), _p(fin))
+        stats["energy_log"] = e
+        stats["final_energy"] = float(fin[0])
+    return out, stats
 
 
 def device_inputs(xyz, tri, metric, indicator=None, gradation=0.0):

@@ -48,6 +48,38 @@ struct Replay {
   std::vector<unsigned long long> prio;
   acvd::MeshState st{};
   ReplayStats stats;
+  // energy log (the library's write_energy_log): the energy after each pass
+  // and after the run, summed as the device sums it
+  bool log_energy = false;
+  std::vector<double> energy_log;
+  double final_energy = 0.0;
+
+  // Energy of the current labels: each cluster's log_cluster_energy from its
+  // accumulators with a cold solve, summed in the fixed order of kLogThreads
+  // (acvd_math.h).
+  double log_sum() const {
+    double part[acvd::kLogThreads];
+    for (int t = 0; t < acvd::kLogThreads; ++t) part[t] = 0.0;
+    for (int32_t c = 0; c < K; ++c) {
+      const int64_t* ws = reinterpret_cast<const int64_t*>(&acc_ws[4 * int64_t(c)]);
+      const acvd::ClusterGeom g = acvd::geom_from_fixed(ws[0], ws[1], ws[2], ws[3], sc);
+      double Tc = 0.0;
+      if (metric == 1) {
+        if (!anchor.empty() && anchor[c] >= 0) {
+          const double a[3] = {double(xw[4 * int64_t(anchor[c])]), double(xw[4 * int64_t(anchor[c]) + 1]),
+                               double(xw[4 * int64_t(anchor[c]) + 2])};
+          Tc = acvd::anchored_T(g, a);
+        } else if (st.constrained) {
+          Tc = acvd::qem_T_from_fixed(reinterpret_cast<const int64_t*>(&acc_q[10 * int64_t(c)]), g, sc, level,
+                                      nullptr);
+        }
+      }
+      part[c % acvd::kLogThreads] += acvd::log_cluster_energy(g, Tc);
+    }
+    for (int h = acvd::kLogThreads / 2; h > 0; h >>= 1)
+      for (int t = 0; t < h; ++t) part[t] += part[t + h];
+    return part[0];
+  }
 
   acvd::SweepView view() const {
     acvd::SweepView v;
@@ -313,14 +345,17 @@ struct Replay {
     st.first_rounds = first_rounds > 0 ? first_rounds : acvd::auto_first_rounds(N);
     st.constrained = unconstrained_init ? 0 : 1;
     st.early_stop_items = recompute(true);
+    energy_log.clear();
     while (st.phase == acvd::kPhaseSweep) {
       pass();
+      if (log_energy) energy_log.push_back(log_sum());
       if (!acvd::controller_step(st)) continue;
       int nd = clean();
       fill_until_stable();
       st.phase = acvd::finish_after_clean(st, nd) ? acvd::kPhaseDone : acvd::kPhaseSweep;
       recompute(true);
     }
+    if (log_energy) final_energy = log_sum();
     stats.loops = st.loops;
     stats.convergences = st.convergences;
     stats.total_moves = st.total_mods;

@@ -16,7 +16,7 @@
 //   InitSamples              :1123-1175 (type 1 and 2; type 0 uses VTK's RNG,
 //                                        restated with our own stream)
 //   ComputeInitialRandomSampling :1177-1316
-//   ComputeGlobalEnergy      :1318-1346
+//   ComputeGlobalEnergy      :1318-1346 (global_energy), WriteToGlobalEnergyLog :1383-1395
 //   Init / Allocate          :1348-1381, defaults :1397-1419
 // and the vertex adaptor DiscreteRemeshing/vtkVerticesProcessing.h:
 //   AddItemRingToProcess :142-148, GetEdgeItems :150-153,
@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdint>
 #include <deque>
@@ -85,6 +86,28 @@ struct Clustering {
   std::vector<EdgeTrace>* edge_trace = nullptr;
   size_t edge_trace_cap = 0;
   std::vector<LoopTrace>* loop_trace = nullptr;
+  // energy log (SetWriteToGlobalEnergyLog): ComputeGlobalEnergy after each
+  // loop (WriteToGlobalEnergyLog, :1383-1395) and after the run (:695-699)
+  std::vector<long double>* energy_log = nullptr;
+  long double final_energy = 0.0L;
+
+  // ComputeGlobalEnergy (:1318-1346): the cluster energies summed in long
+  // double over (int)sqrt(K) bins.  BinCount is never reset, so bin 0 takes
+  // the first (int)sqrt(K) clusters and bin 1 every later one (:1333-1338).
+  long double global_energy() const {
+    const int nb = int(std::sqrt(double(K)));
+    if (nb <= 0) return 0.0L;
+    std::vector<long double> bins(size_t(nb), 0.0L);
+    int bin = 0, count = 0;
+    for (int i = 0; i < K; ++i) {
+      bins[size_t(bin)] += (long double)M::energy(&clusters[i]);
+      count++;
+      if (count == nb && bin < nb - 1) bin++;
+    }
+    long double e = 0.0L;
+    for (long double b : bins) e += b;
+    return e;
+  }
   int constrained_now = 1;  // the metric's ActiveConstraints (QEM), for the trace
   // scratch for the ring check
   std::vector<int32_t> vlist;
@@ -531,6 +554,7 @@ struct Clustering {
       swap_queues();
       const int loop_connexity = connexity;
       int mods = process_one_loop();
+      if (energy_log) energy_log->push_back(global_energy());
       stats.total_modifications += mods;
       if (progress) std::fprintf(stderr, "%sLoop %d: %d Modifications\n", connexity ? "*" : "", loops, mods);
       LoopTrace lt{loops, mods, loop_connexity, constrained_now, 0, 0};
@@ -583,6 +607,10 @@ struct Clustering {
     minimize_energy();
     auto t1 = std::chrono::steady_clock::now();
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (energy_log) {
+      recompute_statistics();
+      final_energy = global_energy();
+    }
     double e = 0;
     for (int i = 0; i < K; ++i) e += M::energy(&clusters[i]);
     stats.shifted_energy = e;

@@ -1,0 +1,109 @@
+"""Midpoint subdivision of Remesh() on coarse inputs (acvd_subdivide;
+vtkSurface::Subdivide, Common/vtkSurface.cxx:605-677, called by
+vtkDiscreteRemeshing::CheckSubsamplingRatio, vtkDiscreteRemeshing.h:840-875).
+Pins: a numpy restatement built from the reference's edge-creation order
+(faces in order, edges (v0,v1), (v1,v2), (v2,v0), first sighting creates the
+edge, Common/vtkSurfaceBase.cxx:1166-1221) gives the same vertices, faces and
+parents, degenerate faces included; and, independently, one subdivision of
+the level-L icosphere projected onto the sphere is the level-(L+1)
+icosphere of the mesh generator, vertex for vertex (its midpoints are made in
+edge-creation order too).  On the GPU, Remesh() of a coarse mesh equals the
+clustering of its subdivision, with the curvature indicator of the original
+input interpolated to the midpoints."""
+import numpy as np
+import pytest
+
+import acvd_amd
+from acvd_amd import meshgen
+from acvd_amd.remeshing import subdivide
+
+
+def subdivide_restated(x, t):
+    x = np.asarray(x, np.float64)
+    t = np.asarray(t, np.int64)
+    active = t[:, 0] != t[:, 1]
+    ta = t[active]
+    a = ta.reshape(-1)
+    b = np.roll(ta, -1, axis=1).reshape(-1)  # (v0,v1), (v1,v2), (v2,v0) of each face
+    valid = a != b
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    key = lo * len(x) + hi
+    key_v = key[valid]
+    uniq, first = np.unique(key_v, return_index=True)
+    order = np.argsort(first, kind="stable")  # creation order
+    ekeys = uniq[order]
+    firsts = np.flatnonzero(valid)[first[order]]
+    e1, e2 = a[firsts], b[firsts]
+    eid = {int(k): i for i, k in enumerate(ekeys)}
+    mid = 0.5 * (x[e1] + x[e2])
+    nv = len(x)
+    child = np.array([nv + eid[int(k)] for k in key]).reshape(-1, 3)
+    v1, v2, v3 = ta[:, 0], ta[:, 1], ta[:, 2]
+    v4, v5, v6 = child[:, 0], child[:, 1], child[:, 2]
+    faces = np.stack([v1, v4, v6, v4, v2, v5, v5, v3, v6, v4, v5, v6], 1).reshape(-1, 3)
+    return np.concatenate([x, mid]), faces.astype(np.int32), np.stack([e1, e2], 1).astype(np.int32)
+
+
+def meshes():
+    xi, ti = meshgen.icosphere(2, 3, 0.05)
+    xc, tc = meshgen.cube(3, 2, 1e-4)
+    rng = np.random.default_rng(1)
+    tp = ti[rng.permutation(len(ti))]
+    td = np.concatenate([ti, [[4, 4, 9]]]).astype(np.int32)  # an inactive face (first two vertices equal)
+    return {"ico2": (xi, ti), "cube3": (xc, tc), "ico2-shuffled-faces": (xi, tp), "ico2-degenerate": (xi, td)}
+
+
+@pytest.mark.parametrize("name", list(meshes()))
+def test_subdivision_equals_restatement(name):
+    x, t = meshes()[name]
+    ox, ot, par = subdivide(x, t)
+    rx, rt, rpar = subdivide_restated(x, t)
+    np.testing.assert_array_equal(ox, rx)
+    np.testing.assert_array_equal(ot, rt)
+    np.testing.assert_array_equal(par, rpar)
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_icosphere_subdivision_is_the_next_level(level):
+    x, t = meshgen.icosphere(level, 1, 0.0)
+    ox, _, par = subdivide(x, t)
+    nxt, _ = meshgen.icosphere(level + 1, 1, 0.0)
+    proj = ox / np.linalg.norm(ox, axis=1)[:, None]
+    np.testing.assert_allclose(proj, nxt, atol=1e-15)
+    np.testing.assert_array_equal(ox[len(x):], 0.5 * (ox[par[:, 0]] + ox[par[:, 1]]))
+
+
+@pytest.mark.gpu
+def test_remesh_subdivides_a_coarse_input(gpu_ready):
+    xyz, tri = meshgen.icosphere(2, 4, 0.05)  # 162 vertices, K = 100 needs 1,000: two subdivisions
+    K = 100
+    r = acvd_amd.QIsotropicDiscreteRemeshing.New()
+    r.SetInput(acvd_amd.Mesh(xyz, tri))
+    r.SetNumberOfClusters(K)
+    r.GetMetric().SetGradation(1.0)
+    r.SetUnconstrainedInitialization(1)
+    r.SetSeed(3)
+    r.Remesh()
+    sub = r.GetInput()
+    assert sub.GetNumberOfPoints() == 2562
+    # the same clustering by hand: subdivide twice, curvature of the original
+    # interpolated to the midpoints, ProcessClustering on the subdivision
+    x1, t1, p1 = subdivide(xyz, tri)
+    x2, t2, p2 = subdivide(x1, t1)
+    np.testing.assert_array_equal(sub.points, x2)
+    c = acvd_amd.Context()
+    c.set_input(xyz, tri)
+    c.set_params(0, number_of_clusters=1, metric=acvd_amd.METRIC_QEM, gradation=1.0, curvature_indicator=1)
+    c.build()
+    ind = c.indicator(0)
+    for par in (p1, p2):
+        ind = np.concatenate([ind, 0.5 * (ind[par[:, 0]] + ind[par[:, 1]])])
+    ctx = acvd_amd.Context()
+    ctx.set_input(x2, t2)
+    ctx.set_params(0, number_of_clusters=K, metric=acvd_amd.METRIC_QEM, gradation=1.0,
+                   unconstrained_initialization=1, seed=3)
+    ctx.set_indicator(0, ind)
+    ctx.process()
+    np.testing.assert_array_equal(r.GetClustering(), ctx.labels(0))
+    assert r.GetOutput().GetNumberOfPoints() == K
+    r.Delete()
