@@ -38,6 +38,15 @@ class Params(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("sweep_segments", ctypes.c_int32),
+        ("tile_on_divisor", ctypes.c_int32),
+        ("chunk_passes", ctypes.c_int32),
+        ("short_chunk_passes", ctypes.c_int32),
+    ]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("loops", ctypes.c_int64),
@@ -79,7 +88,9 @@ EXPORTS = [
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
     "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
     "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
-    "acvd_set_fixed_clusters", "acvd_subdivide", "acvd_get_energy_log",
+    "acvd_set_fixed_clusters", "acvd_subdivide", "acvd_get_energy_log", "acvd_force_manifold",
+    "acvd_set_frozen_clusters", "acvd_vertex_manifold", "acvd_force_manifold_step", "acvd_fix_mesh_boundaries",
+    "acvd_mesh_boundary_fix", "acvd_set_tuning", "acvd_get_tuning", "acvd_autotune",
     "acvd_energy_forward_cvt_noalloc",
 ]
 
@@ -146,6 +157,16 @@ def lib():
     L.acvd_set_fixed_clusters.argtypes = [P, i32, i64, P]
     L.acvd_subdivide.argtypes = [i64, P, i64, P, ctypes.POINTER(i64), ctypes.POINTER(i64), P, P, P]
     L.acvd_get_energy_log.argtypes = [P, i32, i64, ctypes.POINTER(i64), P, P, P]
+    L.acvd_force_manifold.argtypes = [P, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i64)]
+    L.acvd_set_frozen_clusters.argtypes = [P, i32, i64, P]
+    L.acvd_fix_mesh_boundaries.argtypes = [P, i32, ctypes.POINTER(i32)]
+    L.acvd_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    L.acvd_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
+    L.acvd_autotune.argtypes = [P, i32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double)]
+    L.acvd_mesh_boundary_fix.argtypes = [i64, P, i64, P, P, i32, ctypes.POINTER(i64), ctypes.POINTER(i64), P, P]
+    L.acvd_vertex_manifold.argtypes = [i64, i64, P, P]
+    L.acvd_force_manifold_step.argtypes = [i64, i64, P, i64, P, ctypes.POINTER(i32), P, P, i64,
+                                           ctypes.POINTER(i64)]
     L.acvd_energy_forward_cvt_noalloc.argtypes = [ctypes.c_int, i64, i32, P, P, P, P, P]
     _lib = L
     return L

@@ -26,7 +26,11 @@
 //   -w 0/1        energy log: energy.txt in the output directory, one line
 //                 "loop seconds energy" per loop and the final energy
 //                 (SetWriteToGlobalEnergyLog, ACVD.cxx:157-160)
-//   -m -d -b -l -cmin -cmax -cf -np -p   accepted; not supported (warned)
+//   -m 0/1        ForceManifold: restart the minimisation with added clusters
+//                 while the output has non-manifold vertices (at most 100 times)
+//   -b 0/1        boundary fixing: the output is closed along the input's
+//                 boundaries (FixMeshBoundaries)
+//   -d -l -cmin -cmax -cf -np -p   accepted; not supported (warned)
 //
 // Meshes are read from PLY (ascii or binary little-endian) or OBJ and written
 // as ASCII PLY.  Polygons are fanned into triangles, as vtkSurface::SetInput
@@ -235,8 +239,17 @@ bool read_values(const std::string& path, int64_t n, std::vector<double>& out, s
 template <class Remesher>
 int run(Remesher* remesh, const acvd::Mesh& mesh, int K, double gradation, int subsampling, int q_level,
         bool qem, const std::vector<double>* indicator, const std::vector<int32_t>* fixed,
-        const std::string& out_path, const std::string& smooth_path, int energy_log, const std::string& prefix) {
+        const std::string& out_path, const std::string& smooth_path, int energy_log, const std::string& prefix,
+        int force_manifold, int boundary_fixing) {
   remesh->SetInput(&mesh);
+  if (boundary_fixing) {  // ACVD.cxx:187-190, ACVDQ.cxx:241-245
+    std::printf("Setting boundary fixing to %d\n", boundary_fixing);
+    remesh->SetBoundaryFixing(boundary_fixing);
+  }
+  if (force_manifold) {  // ACVD.cxx:118-120, ACVDQ.cxx:159-162
+    std::printf("Force Manifold=%d\n", force_manifold);
+    remesh->SetForceManifold(true);
+  }
   if (energy_log) {  // ACVD.cxx:157-160, ACVDQ.cxx:199-203; energy.txt beside the outputs
     std::printf("Setting writing energy log file to %d\n", energy_log);
     remesh->SetWriteToGlobalEnergyLog(energy_log);
@@ -304,13 +317,14 @@ int main(int argc, char** argv) {
                 "nvertices is the desired number of vertices\n"
                 "gradation defines the influence of local curvature (0=uniform meshing)\n"
                 "options: -q level, -s threshold, -o directory, -of file, -cd indicator file, -qem 0/1,\n"
-                "         -fv fixed vertices file, -ft fixed triangles file (with -qem 1), -w energy log 0/1\n");
+                "         -fv fixed vertices file, -ft fixed triangles file (with -qem 1), -w energy log 0/1,\n"
+                "         -m force manifold 0/1, -b boundary fixing 0/1\n");
     return 0;
   }
   const std::string input = argv[1];
   const int K = std::atoi(argv[2]);
   const double gradation = std::atof(argv[3]);
-  int subsampling = 10, q_level = -1, qem = 0, energy_log = 0;
+  int subsampling = 10, q_level = -1, qem = 0, energy_log = 0, force_manifold = 0, boundary_fixing = 0;
   std::string out_dir, out_file = "simplification.ply", density, fixed_file;
   bool fixed_triangles = false;
   for (int a = 4; a + 1 < argc; a += 2) {
@@ -322,6 +336,8 @@ int main(int argc, char** argv) {
     else if (key == "-cd") density = value;
     else if (key == "-qem") qem = std::atoi(value.c_str());
     else if (key == "-w") energy_log = std::atoi(value.c_str());
+    else if (key == "-m") force_manifold = std::atoi(value.c_str());
+    else if (key == "-b") boundary_fixing = std::atoi(value.c_str());
     else if (key == "-fv" || key == "-ft") {
       fixed_file = value;
       fixed_triangles = key == "-ft";
@@ -364,12 +380,12 @@ int main(int argc, char** argv) {
   if (qem) {
     auto* r = acvd::QIsotropicDiscreteRemeshing::New();
     rc = run(r, mesh, K, gradation, subsampling, q_level, true, ind, fixed_file.empty() ? nullptr : &fixed, out_path,
-             smooth_path, energy_log, prefix);
+             smooth_path, energy_log, prefix, force_manifold, boundary_fixing);
     r->Delete();
   } else {
     auto* r = acvd::IsotropicDiscreteRemeshing::New();
     rc = run(r, mesh, K, gradation, subsampling, q_level, false, ind, nullptr, out_path, smooth_path, energy_log,
-             prefix);
+             prefix, force_manifold, boundary_fixing);
     r->Delete();
   }
   return rc;

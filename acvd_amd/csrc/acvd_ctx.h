@@ -115,6 +115,7 @@ struct SweepArgs {
   const int32_t* col;        // global item ids
   const unsigned long long* ring_adj;  // ring adjacency per item (ring_adjacency of acvd_math.h)
   const int32_t* anchor;     // per cluster: global id of its fixed item or -1; null when no mesh has any
+  const int32_t* frozen;     // per cluster: nonzero when frozen (ForceManifold restarts); null when none is
   int32_t count_rounds;      // count the live and re-evaluated candidates of each round (timed runs)
   const float4* xw;          // position + weight
   const float* q;            // 10 floats per item (QEM meshes), may be null

@@ -546,7 +546,14 @@ struct SweepView {
   const int32_t* cand_pass;
   const unsigned long long* ring_adj;  // ring adjacency per item (rows <= kRingAdjMax), or null
   const int32_t* anchor;             // per cluster: its anchor item (fixed clusters) or -1; null if none
+  const int32_t* frozen;             // per cluster: nonzero when frozen; null if none is
 };
+
+// Frozen clusters (IsClusterFreezed, vtkUniformClustering.h:910-919, set by
+// the ForceManifold restart, DiscreteRemeshing/vtkDiscreteRemeshing.h:
+// 165-250): an item moves only between two clusters that are not frozen; a
+// null item still joins any neighbouring cluster (:881-907 come first).
+ACVD_HD bool is_frozen(const SweepView& v, int32_t c) { return v.frozen && v.frozen[c]; }
 
 // Anchor position of cluster c (FP32 item position, as the sweep's items), if any.
 ACVD_HD bool anchor_pos(const SweepView& v, int32_t c, double* p) {
@@ -628,6 +635,7 @@ ACVD_HD void qem_item_terms(const SweepView& v, int64_t i, int32_t lab, int cons
 // a target with LB_b >= 0 can neither pass the acceptance test nor be the
 // best of a set that contains an acceptable move, so it is skipped.
 ACVD_HD bool qem_open(const SweepView& v, const QemItem& s, int32_t b, int constrained) {
+  if (is_frozen(v, b)) return false;
   const float tb = join_term(s.px, s.py, s.pz, s.pw, v.geom[b]);
   const double Tb = constrained ? v.T[b] : 0.0;
   return double(s.pw * (tb - s.ta)) + (-(s.Ta + Tb)) < 0.0;
@@ -655,7 +663,7 @@ ACVD_HD int32_t qem_next_open(const SweepView& v, int32_t lab, const QemItem& s,
 // Returns true when at least one target is open (solves and choice follow).
 ACVD_HD bool qem_screen(const SweepView& v, int64_t i, int32_t lab, int connexity, int constrained, QemItem* s) {
   if (v.sizes[lab] <= 1) return false;
-  if (is_anchor(v, i, lab)) return false;
+  if (is_anchor(v, i, lab) || is_frozen(v, lab)) return false;
   if (connexity && ring_check(v, i, lab)) return false;
   if (!can_leave(v.xw[4 * i + 3], v.geom[lab])) return false;
   qem_item_terms(v, i, lab, constrained, s);
@@ -751,7 +759,7 @@ ACVD_HD bool qem_screen_short_row(const SweepView& v, int64_t i, int32_t lab, in
   for (int p = 0; p < kRingCache; ++p) nb[p] = p < deg ? v.labels[v.col[r0 + p]] : kNull;
   *open_out = 0;
   if (size_a <= 1) return false;
-  if (is_anchor(v, i, lab)) return false;
+  if (is_anchor(v, i, lab) || is_frozen(v, lab)) return false;
   if (connexity) {
     const bool split = (v.ring_adj && deg <= kRingAdjMax)
                            ? ring_problem_mask(adj, nb, deg, lab)
@@ -852,14 +860,14 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
         v, lab, s, constrained, Ta_new, [&](int32_t, int32_t b) { return qem_join_T(v, i, b, sc, level, constrained); },
         target_out);
   }
-  if (size_a <= 1) return false;
+  if (size_a <= 1 || is_frozen(v, lab)) return false;
   if (connexity && ring_check(v, i, lab)) return false;
   if (!can_leave(pw, ga)) return false;
   const float ta = leave_term(px, py, pz, pw, ga);
   float best = 3.0e38f;
   for (int32_t k = r0; k < r1; ++k) {
     const int32_t b = v.labels[v.col[k]];
-    if (b == kNull || b == lab) continue;
+    if (b == kNull || b == lab || is_frozen(v, b)) continue;
     const float tb = join_term(px, py, pz, pw, v.geom[b]);
     if (tb < best || (tb == best && b < target)) {
       best = tb;

@@ -72,6 +72,7 @@ struct HostMesh {
   std::vector<double> curvature;  // computed by the last build (curvature_indicator)
   std::vector<int32_t> init_labels;
   std::vector<int32_t> fixed;  // SetFixedClusters: cluster k < fixed.size() holds item fixed[k]
+  std::vector<int32_t> frozen;  // IsClusterFreezed per cluster (ForceManifold restarts); empty: none
   acvd_params params;
   // layout in the packed context
   int64_t item_off = 0, clus_off = 0, face_off = 0, n_adj = 0, adj_off = 0;
@@ -115,10 +116,14 @@ struct acvd_ctx {
   DBuf<ClusterGeom> d_geom;
   DBuf<int32_t> d_anchor;  // per cluster: global id of its fixed item or -1
   bool any_fixed = false;
+  DBuf<int32_t> d_frozen;  // per cluster: nonzero when frozen
+  bool any_frozen = false;
   DBuf<double> d_T, d_evec;
   DBuf<int32_t> d_cand_tgt, d_cand_lab, d_cand_since, d_cl_round;
   DBuf<int32_t> d_work, d_work_count, d_cand, d_seg_cand, d_moved, d_seg_moved;
   int32_t work_cap = 0, n_seg = 0, cand_cap = 0;
+  // launch-shape knobs (acvd_set_tuning); none changes the labels
+  acvd_tuning tuning = {kSweepSegs, kTileOnDivisor, kChunk, kShortChunk};
   // scan tiles
   int32_t n_tiles = 0;
   DBuf<int32_t> d_tile_items, d_item_tile, d_tile_mesh, d_tile_cnt, d_tile_set, d_tile_prop;
@@ -149,6 +154,7 @@ struct acvd_ctx {
   Ctrl last_ctrl{};
   bool timing = false;
   bool labels_seeded = false;  // device labels hold the initial sampling
+  bool keep_frozen = false;    // the ForceManifold restarts keep the frozen flags
   double kernel_ms[4] = {0, 0, 0, 0};
   int64_t kernel_launches[4] = {0, 0, 0, 0};
 
@@ -180,6 +186,7 @@ struct acvd_ctx {
     a.acc_q = any_qem ? d_acc_q.p : nullptr;
     a.geom = d_geom.p;
     a.anchor = any_fixed ? d_anchor.p : nullptr;
+    a.frozen = any_frozen ? d_frozen.p : nullptr;
     a.count_rounds = timing ? 1 : 0;
     a.T = any_qem ? d_T.p : nullptr;
     a.evec = any_qem ? d_evec.p : nullptr;
@@ -267,7 +274,9 @@ bool morton_tiles() {
 }
 
 // eval grid (blocks of kBlock threads, grid-stride over the work list)
-int64_t sweep_grid_for(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kSweepSegs)); }
+int64_t sweep_grid_for(int64_t n, int32_t cap) {
+  return std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, cap));
+}
 
 // ------------------------------------------------------------------- build
 int build(acvd_ctx* c) {
@@ -514,6 +523,9 @@ int build(acvd_ctx* c) {
   c->any_fixed = false;
   for (auto& h : c->meshes) c->any_fixed |= !h.fixed.empty();
   if (c->any_fixed) HIP_TRY(c->d_anchor.alloc(K));
+  c->any_frozen = false;
+  for (auto& h : c->meshes) c->any_frozen |= !h.frozen.empty();
+  if (c->any_frozen) HIP_TRY(c->d_frozen.alloc(K));
   HIP_TRY(c->d_cl_round.alloc(K));
   {
     // work slices: scan tiles are dealt round-robin to the slices
@@ -525,7 +537,7 @@ int build(acvd_ctx* c) {
     // b * kBlock + j * n_seg * kBlock (at most N of them) and proposes at most
     // once per item; apply block b appends one moved item per candidate
     // (once per pass)
-    c->n_seg = int32_t(sweep_grid_for(N));
+    c->n_seg = int32_t(sweep_grid_for(N, c->tuning.sweep_segments));
     const int64_t steps = (N + int64_t(c->n_seg) * kBlock - 1) / (int64_t(c->n_seg) * kBlock);
     const int64_t cap = std::max<int64_t>(1, steps) * kBlock;
     if (cap * c->n_seg > (int64_t(1) << 31) - 1) return fail(ACVD_ERR_ARG, "too many items for one context");
@@ -659,7 +671,7 @@ int chunk_rounds(const acvd_ctx* c) {
 
 int run_chunk(acvd_ctx* c, bool use_graph, bool qem_unc, bool tiles, bool short_chunk) {
   hipStream_t s = c->stream;
-  const int n_pass = short_chunk ? kShortChunk : kChunk;
+  const int n_pass = short_chunk ? c->tuning.short_chunk_passes : c->tuning.chunk_passes;
   const int n_rounds = chunk_rounds(c);
   if (c->timing) return run_chunk_timed(c, qem_unc, tiles, n_pass, n_rounds);
   if (use_graph) {
@@ -707,6 +719,12 @@ void print_pass_log(const Ctrl& ctrl, unsigned long long* printed) {
 
 // Fixed items per cluster on the device (global ids; -1 for free clusters).
 int upload_anchors(acvd_ctx* c) {
+  if (c->any_frozen) {
+    std::vector<int32_t> fr(size_t(c->K), 0);
+    for (auto& h : c->meshes)
+      for (size_t k = 0; k < h.frozen.size(); ++k) fr[size_t(h.clus_off) + k] = h.frozen[k];
+    HIP_TRY(hipMemcpy(c->d_frozen.p, fr.data(), size_t(c->K) * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   if (!c->any_fixed) return ACVD_OK;
   std::vector<int32_t> an(size_t(c->K), -1);
   for (auto& h : c->meshes)
@@ -910,7 +928,7 @@ int minimize_energy(acvd_ctx* c) {
     // tile sets on once the last pass moved at most K / kTileOnDivisor items:
     // most tiles then hold no changed cluster and the scan skips them
     if (!tiles && c->h_ctrl->passes > 0 &&
-        int64_t(c->h_ctrl->pass_log[(c->h_ctrl->passes - 1) % kPassLogCap][3]) * kTileOnDivisor <= c->K) {
+        int64_t(c->h_ctrl->pass_log[(c->h_ctrl->passes - 1) % kPassLogCap][3]) * c->tuning.tile_on_divisor <= c->K) {
       HIP_TRY(launch_tile_sets(s, a, nullptr));
       tiles = true;
     }
@@ -1137,12 +1155,22 @@ int acvd_build(acvd_ctx* ctx) {
 int acvd_process_clustering(acvd_ctx* ctx) {
   if (int rc = check_ctx(ctx)) return rc;
   if (ctx->meshes.empty()) return fail(ACVD_ERR_STATE, "no input mesh");
+  // Init() clears IsClusterFreezed (vtkUniformClustering.h:1370-1375): only
+  // the ForceManifold restarts run with frozen clusters
+  if (!ctx->keep_frozen)
+    for (auto& h : ctx->meshes)
+      if (!h.frozen.empty()) {
+        h.frozen.clear();
+        ctx->invalidate();
+      }
   for (auto& h : ctx->meshes) {
     if (h.params.number_of_clusters <= 0) return fail(ACVD_ERR_ARG, "must set NumberOfClusters to more than zero");
     if (int64_t(h.fixed.size()) > h.params.number_of_clusters)
       return fail(ACVD_ERR_ARG, "more fixed clusters than clusters");
     if (!h.fixed.empty() && h.params.initial_sampling == 1 && h.init_labels.empty())
       return fail(ACVD_ERR_ARG, "fixed clusters need the reference's initial sampling (0) or initial labels");
+    if (!h.frozen.empty() && int64_t(h.frozen.size()) != h.params.number_of_clusters)
+      return fail(ACVD_ERR_ARG, "frozen flags do not match the number of clusters");
   }
   int rc = build(ctx);
   if (rc) return rc;
@@ -1271,6 +1299,211 @@ int acvd_build_output(acvd_ctx* ctx, int32_t mesh_id, int32_t quadrics_level) {
     HIP_TRY(hipMemcpyAsync(h.out_tri.data(), otri.p, 3 * size_t(nf_out) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   exact_anchor_points(h, h.out_xyz.data());
+  return ACVD_OK;
+}
+
+int acvd_set_tuning(acvd_ctx* ctx, const acvd_tuning* t) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!t) return fail(ACVD_ERR_ARG, "null tuning");
+  if (t->sweep_segments < 1 || t->sweep_segments > kMaxSeg) return fail(ACVD_ERR_ARG, "sweep_segments must be 1..2048");
+  if (t->tile_on_divisor < 1 || t->tile_on_divisor > 1024) return fail(ACVD_ERR_ARG, "tile_on_divisor must be 1..1024");
+  if (t->chunk_passes < 1 || t->chunk_passes > kChunk) return fail(ACVD_ERR_ARG, "chunk_passes must be 1..16");
+  if (t->short_chunk_passes < 1 || t->short_chunk_passes > t->chunk_passes)
+    return fail(ACVD_ERR_ARG, "short_chunk_passes must be 1..chunk_passes");
+  const bool rebuild = t->sweep_segments != ctx->tuning.sweep_segments;
+  ctx->tuning = *t;
+  if (rebuild) ctx->invalidate();
+  else ctx->drop_graphs();
+  return ACVD_OK;
+}
+
+int acvd_get_tuning(acvd_ctx* ctx, acvd_tuning* t) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (!t) return fail(ACVD_ERR_ARG, "null tuning");
+  *t = ctx->tuning;
+  return ACVD_OK;
+}
+
+int acvd_autotune(acvd_ctx* ctx, int32_t repeats, acvd_tuning* best_out, double* best_ms_out) {
+  if (int rc = check_ctx(ctx)) return rc;
+  if (ctx->meshes.empty()) return fail(ACVD_ERR_STATE, "no input mesh");
+  if (repeats < 1 || repeats > 16) return fail(ACVD_ERR_ARG, "repeats must be 1..16");
+  // one timed point: the best of `repeats` clusterings, and the labels
+  auto measure = [&](const acvd_tuning& t, double* ms, std::vector<int32_t>* lab) -> int {
+    if (int rc = acvd_set_tuning(ctx, &t)) return rc;
+    double best = 1e300;
+    for (int r = 0; r < repeats; ++r) {
+      if (int rc = acvd_process_clustering(ctx)) return rc;
+      best = std::min(best, ctx->converge_ms);
+    }
+    lab->resize(size_t(ctx->N));
+    HIP_TRY(hipMemcpy(lab->data(), ctx->d_labels.p, size_t(ctx->N) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    *ms = best;
+    return ACVD_OK;
+  };
+  const acvd_tuning start = ctx->tuning;
+  acvd_tuning cur = start;
+  double cur_ms = 0.0;
+  std::vector<int32_t> ref, lab;
+  if (int rc = measure(cur, &cur_ms, &ref)) return rc;
+  static const int32_t kSegs[] = {256, 512, 768, 1024, 1152, 1280, 1536, 2048};
+  static const int32_t kDivs[] = {4, 8, 16, 32, 64};
+  static const int32_t kChunks[] = {4, 8, 12, 16};
+  static const int32_t kShorts[] = {1, 2, 4, 8};
+  const int32_t* cand[4] = {kSegs, kDivs, kChunks, kShorts};
+  const int ncand[4] = {8, 5, 4, 4};
+  for (int knob = 0; knob < 4; ++knob)
+    for (int j = 0; j < ncand[knob]; ++j) {
+      acvd_tuning t = cur;
+      int32_t* f[4] = {&t.sweep_segments, &t.tile_on_divisor, &t.chunk_passes, &t.short_chunk_passes};
+      if (*f[knob] == cand[knob][j]) continue;
+      *f[knob] = cand[knob][j];
+      if (t.short_chunk_passes > t.chunk_passes) continue;
+      double ms = 0.0;
+      if (int rc = measure(t, &ms, &lab)) return rc;
+      if (lab != ref) {
+        (void)acvd_set_tuning(ctx, &start);
+        return fail(ACVD_ERR_STATE, "the labels changed with the launch shape");
+      }
+      if (ms < 0.99 * cur_ms) {
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+  if (int rc = measure(cur, &cur_ms, &lab)) return rc;
+  if (best_out) *best_out = cur;
+  if (best_ms_out) *best_ms_out = cur_ms;
+  return ACVD_OK;
+}
+
+int acvd_set_frozen_clusters(acvd_ctx* ctx, int32_t mesh_id, int64_t n, const int32_t* flags) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  HostMesh& h = ctx->meshes[mesh_id];
+  if (n != 0 && (!flags || n != h.params.number_of_clusters))
+    return fail(ACVD_ERR_ARG, "frozen flags: one per cluster (or none)");
+  h.frozen.assign(flags, flags + n);
+  ctx->keep_frozen = n != 0;
+  ctx->invalidate();
+  return ACVD_OK;
+}
+
+int acvd_vertex_manifold(int64_t nv, int64_t nf, const int32_t* tri, uint8_t* manifold) {
+  if (nv <= 0 || nf < 0 || (nf && !tri) || !manifold) return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv) return fail(ACVD_ERR_ARG, "triangle references a missing vertex");
+  std::vector<uint8_t> ok;
+  vertex_manifold(nv, nf, tri, &ok);
+  std::memcpy(manifold, ok.data(), size_t(nv));
+  return ACVD_OK;
+}
+
+int acvd_force_manifold_step(int64_t nv, int64_t nf, const int32_t* tri, int64_t n_out, const int32_t* out_tri,
+                             int32_t* n_clusters, int32_t* labels, int32_t* frozen, int64_t frozen_capacity,
+                             int64_t* issues) {
+  if (nv <= 0 || nf <= 0 || !tri || n_out < 0 || (n_out && !out_tri) || !n_clusters || !labels || !issues)
+    return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  const int32_t K = *n_clusters;
+  if (K <= 0) return fail(ACVD_ERR_ARG, "no clusters");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv) return fail(ACVD_ERR_ARG, "triangle references a missing vertex");
+  for (int64_t k = 0; k < 3 * n_out; ++k)
+    if (out_tri[k] < 0 || out_tri[k] >= K) return fail(ACVD_ERR_ARG, "output triangle references a missing cluster");
+  std::vector<uint8_t> in_ok;
+  vertex_manifold(nv, nf, tri, &in_ok);
+  VertexTopology topo;
+  vertex_topology(nv, nf, tri, &topo);
+  std::vector<int32_t> fr;
+  int32_t Kn = K;
+  std::vector<int32_t> lab(labels, labels + nv);
+  const int64_t n = force_manifold_step(nv, topo, in_ok, n_out, out_tri, &Kn, lab.data(), &fr, false);
+  if (frozen && int64_t(fr.size()) > frozen_capacity) return fail(ACVD_ERR_ARG, "frozen buffer too small");
+  std::memcpy(labels, lab.data(), size_t(nv) * sizeof(int32_t));
+  if (frozen) std::memcpy(frozen, fr.data(), fr.size() * sizeof(int32_t));
+  *n_clusters = Kn;
+  *issues = n;
+  return ACVD_OK;
+}
+
+int acvd_force_manifold(acvd_ctx* ctx, int32_t mesh_id, int32_t max_restarts, int32_t* restarts,
+                        int64_t* remaining_issues) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (ctx->meshes.size() != 1) return fail(ACVD_ERR_ARG, "ForceManifold runs on a context holding one mesh");
+  if (max_restarts < 0) return fail(ACVD_ERR_ARG, "negative restart limit");
+  HostMesh& h = ctx->meshes[0];
+  if (!ctx->built || !h.clustered) return fail(ACVD_ERR_STATE, "no clustering computed");
+  const bool verbose = h.params.console_output > 0;
+  std::vector<uint8_t> in_ok;
+  vertex_manifold(h.nv, h.nf, h.tri.data(), &in_ok);
+  VertexTopology topo;
+  vertex_topology(h.nv, h.nf, h.tri.data(), &topo);
+  const acvd_params user = h.params;
+  std::vector<int32_t> user_init;
+  user_init.swap(h.init_labels);
+  std::vector<int32_t> lab(size_t(h.nv));
+  int32_t n = 0;
+  int64_t issues = 0;
+  int rc = ACVD_OK;
+  for (;;) {
+    rc = acvd_build_output(ctx, 0, 0);
+    if (rc) break;
+    rc = acvd_get_labels(ctx, 0, lab.data());
+    if (rc) break;
+    int32_t K = h.params.number_of_clusters;
+    for (auto& l : lab)
+      if (l == K) l = -1;  // null: out of range whatever K becomes
+    std::vector<int32_t> frozen;
+    issues = force_manifold_step(h.nv, topo, in_ok, int64_t(h.out_tri.size() / 3), h.out_tri.data(), &K,
+                                 lab.data(), &frozen, verbose);
+    if (issues == 0 || n >= max_restarts) break;
+    if (verbose) std::printf("%lld topology issues, restarting minimization\n", (long long)issues);
+    // MinimizeEnergy again from the current labels with ConnexityConstraint
+    // = 0 (:941-943): the metric stays constrained, clusters added
+    h.params.number_of_clusters = K;
+    h.params.unconstrained_initialization = 0;
+    h.params.write_energy_log = 0;
+    h.init_labels = lab;
+    h.frozen.swap(frozen);
+    ctx->keep_frozen = true;
+    ctx->invalidate();
+    rc = acvd_process_clustering(ctx);
+    ctx->keep_frozen = false;
+    if (rc) break;
+    ++n;
+  }
+  h.params.unconstrained_initialization = user.unconstrained_initialization;
+  h.params.write_energy_log = user.write_energy_log;
+  h.init_labels.swap(user_init);
+  if (restarts) *restarts = n;
+  if (remaining_issues) *remaining_issues = issues;
+  return rc;
+}
+
+int acvd_mesh_boundary_fix(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, const int32_t* labels,
+                           int32_t n_clusters, int64_t* nv_added, int64_t* nf_added, double* added_xyz,
+                           int32_t* added_tri) {
+  if (nv <= 0 || nf <= 0 || !xyz || !tri || !labels || !nv_added || !nf_added || n_clusters <= 0)
+    return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  for (int64_t k = 0; k < 3 * nf; ++k)
+    if (tri[k] < 0 || tri[k] >= nv) return fail(ACVD_ERR_ARG, "triangle references a missing vertex");
+  std::vector<double> ox(3 * size_t(n_clusters), 0.0);
+  std::vector<int32_t> ot;
+  fix_mesh_boundaries(nv, xyz, nf, tri, labels, n_clusters, &ox, &ot);
+  *nv_added = int64_t(ox.size() / 3) - n_clusters;
+  *nf_added = int64_t(ot.size() / 3);
+  if (added_xyz) std::memcpy(added_xyz, ox.data() + 3 * size_t(n_clusters), size_t(*nv_added) * 3 * sizeof(double));
+  if (added_tri) std::memcpy(added_tri, ot.data(), ot.size() * sizeof(int32_t));
+  return ACVD_OK;
+}
+
+int acvd_fix_mesh_boundaries(acvd_ctx* ctx, int32_t mesh_id, int32_t* changed) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  HostMesh& h = ctx->meshes[mesh_id];
+  if (h.out_xyz.empty()) return fail(ACVD_ERR_STATE, "output mesh not built (acvd_build_output)");
+  std::vector<int32_t> lab(size_t(h.nv));
+  if (int rc = acvd_get_labels(ctx, mesh_id, lab.data())) return rc;
+  const bool c = fix_mesh_boundaries(h.nv, h.xyz.data(), h.nf, h.tri.data(), lab.data(), h.params.number_of_clusters,
+                                     &h.out_xyz, &h.out_tri);
+  if (changed) *changed = c ? 1 : 0;
   return ACVD_OK;
 }
 

@@ -16,10 +16,11 @@
 // `rng() % n` index (vtkUniformClustering.h:1202-1207).
 #include "host_sampling.h"
 
+#include "host_parallel.h"
+
 #include <algorithm>
 #include <cmath>
 #include <random>
-#include <thread>
 #include <vector>
 
 namespace acvd {
@@ -37,59 +38,41 @@ double face_area(const double* xyz, const int32_t* t) {
   return 0.25 * std::sqrt(std::fabs(4.0 * a * c - (a - b + c) * (a - b + c)));
 }
 
-// f(begin, end) over [0, n) split across the host's cores (the per-vertex
-// stages are independent; every result is the one a serial loop gives).
-template <class F>
-void parallel_for(int64_t n, F&& f) {
-  const unsigned hw = std::thread::hardware_concurrency();
-  const int64_t T = std::min<int64_t>(16, std::max<int64_t>(1, hw));
-  if (n < (int64_t(1) << 16) || T == 1) {
-    f(int64_t(0), n);
-    return;
-  }
-  const int64_t chunk = (n + T - 1) / T;
-  std::vector<std::thread> th;
-  for (int64_t t = 0; t < T; ++t) {
-    const int64_t a = t * chunk, b = std::min(n, a + chunk);
-    if (a < b) th.emplace_back([&f, a, b] { f(a, b); });
-  }
-  for (auto& x : th) x.join();
-}
-
 }  // namespace
 
-bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
-                                const double* indicator, double gradation, int32_t K, int32_t* labels,
-                                int64_t n_fixed, const int32_t* fixed) {
+void vertex_topology(int64_t nv, int64_t nf, const int32_t* tri, VertexTopology* out) {
+  VertexTopology& T = *out;
   // vertex -> faces, increasing face id (faces with a repeated first edge
   // vertex are inactive in the reference and skipped; a face listing a
   // vertex twice is stored once for it)
   auto slot_of = [](const int32_t* t, int j) { return !(j > 0 && t[j] == t[0]) && !(j > 1 && t[j] == t[1]); };
-  std::vector<int64_t> vfptr(nv + 1, 0);
+  T.vfptr.assign(size_t(nv + 1), 0);
   for (int64_t f = 0; f < nf; ++f) {
     const int32_t* t = tri + 3 * f;
     if (t[0] == t[1]) continue;
     for (int j = 0; j < 3; ++j)
-      if (slot_of(t, j)) vfptr[t[j] + 1]++;
+      if (slot_of(t, j)) T.vfptr[size_t(t[j] + 1)]++;
   }
-  for (int64_t v = 0; v < nv; ++v) vfptr[v + 1] += vfptr[v];
-  std::vector<int32_t> vf(size_t(vfptr[nv]));
+  for (int64_t v = 0; v < nv; ++v) T.vfptr[size_t(v + 1)] += T.vfptr[size_t(v)];
+  T.vf.assign(size_t(T.vfptr[size_t(nv)]), 0);
   {
-    std::vector<int64_t> fill(vfptr.begin(), vfptr.end() - 1);
+    std::vector<int64_t> fill(T.vfptr.begin(), T.vfptr.end() - 1);
     for (int64_t f = 0; f < nf; ++f) {
       const int32_t* t = tri + 3 * f;
       if (t[0] == t[1]) continue;
       for (int j = 0; j < 3; ++j)
-        if (slot_of(t, j)) vf[size_t(fill[t[j]]++)] = int32_t(f);
+        if (slot_of(t, j)) T.vf[size_t(fill[size_t(t[j])]++)] = int32_t(f);
     }
   }
+  const std::vector<int64_t>& vfptr = T.vfptr;
+  const std::vector<int32_t>& vf = T.vf;
   // rings in edge-creation order: scanning the faces of v in increasing id and
   // each face's edges in order, the first sighting of neighbour u is the
   // creation of edge (v,u), which is when it joins both rings.  Two passes:
   // sizes, then the rings in place.
-  auto ring_of = [&](int64_t v, int32_t* out) {
+  auto ring_of = [&](int64_t v, int32_t* dst) {
     int64_t cnt = 0;
-    for (int64_t k = vfptr[v]; k < vfptr[v + 1]; ++k) {
+    for (int64_t k = vfptr[size_t(v)]; k < vfptr[size_t(v + 1)]; ++k) {
       const int32_t* t = tri + 3 * int64_t(vf[size_t(k)]);
       for (int j = 0; j < 3; ++j) {
         const int32_t a = t[j], b = t[(j + 1) % 3];
@@ -100,28 +83,39 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
         if (u < 0) continue;
         bool seen = false;
         for (int64_t q = 0; q < cnt; ++q)
-          if (out[q] == u) {
+          if (dst[q] == u) {
             seen = true;
             break;
           }
-        if (!seen) out[cnt++] = u;
+        if (!seen) dst[cnt++] = u;
       }
     }
     return cnt;
   };
-  std::vector<int64_t> rptr(nv + 1, 0);
+  T.rptr.assign(size_t(nv + 1), 0);
   parallel_for(nv, [&](int64_t v0, int64_t v1) {
     std::vector<int32_t> tmp;
     for (int64_t v = v0; v < v1; ++v) {
-      tmp.resize(size_t(2 * (vfptr[v + 1] - vfptr[v])));
-      rptr[v + 1] = ring_of(v, tmp.data());
+      tmp.resize(size_t(2 * (vfptr[size_t(v + 1)] - vfptr[size_t(v)])));
+      T.rptr[size_t(v + 1)] = ring_of(v, tmp.data());
     }
   });
-  for (int64_t v = 0; v < nv; ++v) rptr[v + 1] += rptr[v];
-  std::vector<int32_t> ring(size_t(rptr[nv]));
+  for (int64_t v = 0; v < nv; ++v) T.rptr[size_t(v + 1)] += T.rptr[size_t(v)];
+  T.ring.assign(size_t(T.rptr[size_t(nv)]), 0);
   parallel_for(nv, [&](int64_t v0, int64_t v1) {
-    for (int64_t v = v0; v < v1; ++v) ring_of(v, ring.data() + rptr[v]);
+    for (int64_t v = v0; v < v1; ++v) ring_of(v, T.ring.data() + T.rptr[size_t(v)]);
   });
+}
+
+bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
+                                const double* indicator, double gradation, int32_t K, int32_t* labels,
+                                int64_t n_fixed, const int32_t* fixed) {
+  VertexTopology topo;
+  vertex_topology(nv, nf, tri, &topo);
+  const std::vector<int64_t>& vfptr = topo.vfptr;
+  const std::vector<int32_t>& vf = topo.vf;
+  const std::vector<int64_t>& rptr = topo.rptr;
+  const std::vector<int32_t>& ring = topo.ring;
   // metric weights: vertex area = sum, in GetVertexNeighbourFaces order (the
   // faces of each ring neighbour in turn), of the face areas / 3
   std::vector<double> farea(static_cast<size_t>(nf));

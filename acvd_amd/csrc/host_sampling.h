@@ -2,8 +2,20 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 namespace acvd {
+
+// Vertex topology in the reference's order: the faces of each vertex in
+// increasing face id (GetVertexNeighbourFaces order of the item weights,
+// Common/vtkSurfaceBase.cxx:964-984) and the ring of each vertex in
+// edge-creation order (GetVertexNeighbours, :1057-1068: edges (v0,v1),
+// (v1,v2), (v2,v0) of each face in turn, :1166-1221, 1407-1468).
+struct VertexTopology {
+  std::vector<int64_t> vfptr, rptr;  // nv + 1 offsets
+  std::vector<int32_t> vf, ring;
+};
+void vertex_topology(int64_t nv, int64_t nf, const int32_t* tri, VertexTopology* out);
 
 // InitSamples with sampling type 1 (vtkUniformClustering.h:1123-1175, default
 // type :1409) = ComputeInitialRandomSampling (:1177-1316) over one mesh with

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(SweepArgs a, const int32_
     if (dm.metric == 1)
       for (int k = 0; k < 10; ++k) atomicAdd(&v[4 + k], (unsigned long long)fix_q(a.q[10 * int64_t(i) + k], sc));
     atomicAdd(&v[14], 1ull);
-    atomicAdd(&s_count, 1ull);
+    // early-stop items: those of clusters that are not frozen (:733-736)
+    if (!(a.frozen && a.frozen[c])) atomicAdd(&s_count, 1ull);
   }
   __syncthreads();
   const int32_t key = keys[threadIdx.x];
@@ -159,6 +160,7 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   v.cand_pass = a.cand_pass;
   v.ring_adj = a.ring_adj;
   v.anchor = a.anchor;
+  v.frozen = a.frozen;
   return v;
 }
 

@@ -169,6 +169,60 @@ class Context:
         check(N.lib().acvd_get_energy(self._h, mesh_id, _p(e)))
         return float(e[0])
 
+    def fix_mesh_boundaries(self, mesh_id=0):
+        """FixMeshBoundaries on the built output (acvd_fix_mesh_boundaries):
+        True when the output changed."""
+        changed = ctypes.c_int32()
+        check(N.lib().acvd_fix_mesh_boundaries(self._h, mesh_id, ctypes.byref(changed)))
+        return bool(changed.value)
+
+    def set_tuning(self, **kw):
+        """Launch shape (acvd_set_tuning): sweep_segments, tile_on_divisor,
+        chunk_passes, short_chunk_passes; unnamed knobs keep their value."""
+        t = N.Tuning()
+        check(N.lib().acvd_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise AttributeError(f"unknown tuning knob {k}")
+            setattr(t, k, int(v))
+        check(N.lib().acvd_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = N.Tuning()
+        check(N.lib().acvd_get_tuning(self._h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def autotune(self, repeats=2):
+        """acvd_autotune: (best launch shape, device ms per clustering); the
+        shape stays set."""
+        t = N.Tuning()
+        ms = ctypes.c_double()
+        check(N.lib().acvd_autotune(self._h, int(repeats), ctypes.byref(t), ctypes.byref(ms)))
+        return {k: getattr(t, k) for k, _ in t._fields_}, ms.value
+
+    def sizes(self):
+        """(items, clusters, QEM meshes) over the context's meshes."""
+        n = sum(self._nv)
+        k = q = 0
+        for m in range(len(self._nv)):
+            p = self.get_params(m)
+            k += p["number_of_clusters"]
+            q += p["metric"] == N.METRIC_QEM
+        return n, k, q
+
+    def set_frozen_clusters(self, mesh_id, flags):
+        """IsClusterFreezed per cluster for the following runs (None clears)."""
+        f = np.zeros(0, np.int32) if flags is None else np.ascontiguousarray(flags, np.int32).reshape(-1)
+        check(N.lib().acvd_set_frozen_clusters(self._h, mesh_id, len(f), _p(f) if len(f) else None))
+
+    def force_manifold(self, mesh_id=0, max_restarts=100):
+        """Remesh's ForceManifold loop (acvd_force_manifold): returns
+        (restarts run, issues left)."""
+        r = ctypes.c_int32()
+        left = ctypes.c_int64()
+        check(N.lib().acvd_force_manifold(self._h, mesh_id, int(max_restarts), ctypes.byref(r), ctypes.byref(left)))
+        return r.value, left.value
+
     def energy_log(self, mesh_id=0):
         """(seconds, energy, final_energy) of the last run with
         write_energy_log set: one entry per loop (acvd_get_energy_log)."""
@@ -200,9 +254,11 @@ class Context:
         check(N.lib().acvd_get_items(self._h, mesh_id, _p(xw), _p(q)))
         return xw, q
 
-    def output(self, mesh_id=0, quadrics_level=0):
-        """Output mesh (cluster points, triangles) of the last clustering."""
-        check(N.lib().acvd_build_output(self._h, mesh_id, quadrics_level))
+    def output(self, mesh_id=0, quadrics_level=0, build=True):
+        """Output mesh (cluster points, triangles) of the last clustering
+        (build=False: the output as last built and fixed, without rebuilding)."""
+        if build:
+            check(N.lib().acvd_build_output(self._h, mesh_id, quadrics_level))
         nv, nf = ctypes.c_int64(), ctypes.c_int64()
         check(N.lib().acvd_get_output_sizes(self._h, mesh_id, ctypes.byref(nv), ctypes.byref(nf)))
         xyz = np.empty((nv.value, 3), np.float64)
@@ -227,6 +283,44 @@ def reference_initial_sampling(xyz, tri, n_clusters, metric=N.METRIC_ISOTROPIC, 
     check(N.lib().acvd_reference_initial_sampling(len(x), _p(x), len(t), _p(t), int(metric), _p(ind),
                                                   float(gradation), int(n_clusters), _p(out)))
     return out
+
+
+def vertex_manifold(n_vertices, triangles):
+    """IsVertexManifold of every vertex (acvd_vertex_manifold): bool array."""
+    tri = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    out = np.zeros(int(n_vertices), np.uint8)
+    check(N.lib().acvd_vertex_manifold(int(n_vertices), len(tri), _p(tri), _p(out)))
+    return out.astype(bool)
+
+
+def force_manifold_step(n_vertices, triangles, labels, n_clusters, out_triangles):
+    """One DetectNonManifoldOutputVertices step (acvd_force_manifold_step):
+    (issues, labels, n_clusters, frozen flags)."""
+    tri = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    otri = np.ascontiguousarray(out_triangles, np.int32).reshape(-1, 3)
+    lab = np.array(labels, np.int32)
+    k = ctypes.c_int32(int(n_clusters))
+    frozen = np.zeros(2 * int(n_clusters) + 1, np.int32)
+    n = ctypes.c_int64()
+    check(N.lib().acvd_force_manifold_step(int(n_vertices), len(tri), _p(tri), len(otri), _p(otri), ctypes.byref(k),
+                                           _p(lab), _p(frozen), len(frozen), ctypes.byref(n)))
+    return n.value, lab, k.value, frozen[:k.value]
+
+
+def mesh_boundary_fix(points, triangles, labels, n_clusters):
+    """FixMeshBoundaries on given labels (acvd_mesh_boundary_fix): the output
+    vertices it adds (ids n_clusters, ...) and the triangles it appends."""
+    x = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    tri = np.ascontiguousarray(triangles, np.int32).reshape(-1, 3)
+    lab = np.ascontiguousarray(labels, np.int32)
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    check(N.lib().acvd_mesh_boundary_fix(len(x), _p(x), len(tri), _p(tri), _p(lab), int(n_clusters), ctypes.byref(a),
+                                         ctypes.byref(b), None, None))
+    ax = np.empty((a.value, 3), np.float64)
+    at = np.empty((b.value, 3), np.int32)
+    check(N.lib().acvd_mesh_boundary_fix(len(x), _p(x), len(tri), _p(tri), _p(lab), int(n_clusters), ctypes.byref(a),
+                                         ctypes.byref(b), _p(ax), _p(at)))
+    return ax, at
 
 
 def subdivide(points, triangles):
@@ -288,6 +382,8 @@ class _DiscreteRemeshing:
         self._output = None
         self._metric = _MetricProxy(self)
         self._output_directory = None
+        self._force_manifold = False
+        self._boundary_fixing = 0
 
     @classmethod
     def New(cls):
@@ -351,6 +447,18 @@ class _DiscreteRemeshing:
     def SetOutputDirectory(self, d):
         """vtkUniformClustering::SetOutputDirectory (vtkUniformClustering.h:126)."""
         self._output_directory = None if d is None else str(d)
+
+    def SetBoundaryFixing(self, v):
+        """vtkDiscreteRemeshing::SetBoundaryFixing (vtkDiscreteRemeshing.h:77):
+        Remesh closes the output along the input's boundaries
+        (FixMeshBoundaries, :425-579)."""
+        self._boundary_fixing = int(v)
+
+    def SetForceManifold(self, v):
+        """vtkDiscreteRemeshing::SetForceManifold (vtkDiscreteRemeshing.h:79):
+        Remesh restarts the minimisation with added clusters while the output
+        has non-manifold vertices (:937-946), at most 100 times."""
+        self._force_manifold = bool(v)
 
     def SetSubsamplingThreshold(self, t):
         self._subsampling_threshold = int(t)
@@ -422,8 +530,9 @@ class _DiscreteRemeshing:
         the items are weighted by the reference's curvature indicator, computed
         on the device on the original input and interpolated to the midpoints
         as the mean of their two parents, :583-653, 700-745),
-        ProcessClustering, then the output triangulation.  ForceManifold and
-        boundary fixing are not part of this path."""
+        ProcessClustering, the ForceManifold restarts when set (:937-946),
+        the output triangulation, then FixMeshBoundaries when boundary
+        fixing is set (:952)."""
         K = self._params["number_of_clusters"]
         if K <= 0:
             return
@@ -459,15 +568,25 @@ class _DiscreteRemeshing:
             curvature = 0
         if self.ProcessClustering(curvature_indicator=curvature) is None:
             return
+        if self._force_manifold:
+            self._restarts = self._ctx.force_manifold(0)
+            self._params["number_of_clusters"] = self._ctx.get_params(0)["number_of_clusters"]
+            self._clustering = self._ctx.labels(0)
         xyz, tri = self._ctx.output(0, 0)
+        if self._boundary_fixing and self._ctx.fix_mesh_boundaries(0):
+            xyz, tri = self._ctx.output(0, build=False)
         self._output = Mesh(xyz, tri)
 
     def GetOutput(self):
         return self._output
 
     def QuadricPostProcessing(self, level=1):
-        """Quadric placement of the output vertices (Examples/ACVD.cxx:217-291)."""
+        """Quadric placement of the output vertices (Examples/ACVD.cxx:217-291);
+        with boundary fixing on, the boundary vertices and triangles are kept
+        (the placement moves only the cluster points)."""
         xyz, tri = self._ctx.output(0, level)
+        if self._boundary_fixing and self._ctx.fix_mesh_boundaries(0):
+            xyz, tri = self._ctx.output(0, build=False)
         self._output = Mesh(xyz, tri)
         return self._output
 

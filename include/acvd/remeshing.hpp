@@ -22,7 +22,9 @@
 //   ProcessClustering() -> vtkIntArray* (vtkUniformClustering.h:75)
 //                                                           ProcessClustering() -> const Labels*
 //   GetClustering() (:78)                                   same (borrowed, owned by the object)
-//   Remesh() (vtkDiscreteRemeshing.h:64, 877-954)           same (subdivides coarse inputs; no ForceManifold)
+//   Remesh() (vtkDiscreteRemeshing.h:64, 877-954)           same (subdivides coarse inputs)
+//   SetForceManifold (vtkDiscreteRemeshing.h:79)             same (at most 100 restarts)
+//   SetBoundaryFixing (vtkDiscreteRemeshing.h:77)            same (FixMeshBoundaries after the output)
 //   GetOutput() -> vtkSurface* (:61)                        GetOutput() -> const Mesh* (borrowed)
 //
 // Error behaviour follows the reference: no exceptions; a failing call
@@ -155,7 +157,7 @@ class DiscreteRemeshing {
     }
     if (chain.empty()) {
       if (!ProcessClustering(1)) return 0;
-      return BuildOutput(0);
+      return Finish();
     }
     if (initial_) return fail("the initial clustering labels the input's vertices; the input needs subdivision"), 0;
     std::vector<double> ind;
@@ -183,15 +185,32 @@ class DiscreteRemeshing {
     interpolated_ = std::move(ind);
     if (!interpolated_.empty()) indicator_ = &interpolated_;
     if (!ProcessClustering(0)) return 0;
-    return BuildOutput(0);
+    return Finish();
   }
+  // ForceManifold (vtkDiscreteRemeshing.h:937-946): restarts with added
+  // clusters while the output has non-manifold vertices (acvd_force_manifold)
+  void SetForceManifold(bool v) { force_manifold_ = v; }
+  // FixMeshBoundaries (vtkDiscreteRemeshing.h:425-579) after the output
+  void SetBoundaryFixing(int v) { boundary_fixing_ = v; }
+  int GetNumberOfRestarts() const { return restarts_; }
   // the input Remesh() replaced by its subdivision, or nullptr
   const Mesh* GetOriginalInput() const { return original_; }
   // Output mesh with vertices moved by the quadric placement of
   // Examples/ACVD.cxx:217-291 at `level` (0 = cluster points).
+  // With boundary fixing on, the boundary vertices and triangles are added
+  // after the cluster points (the reference's post-placement moves only the
+  // cluster points, Examples/ACVD.cxx:270-275, and keeps them).
   int BuildOutput(int level) {
     if (!clustered_) return fail("no clustering computed"), 0;
     if (!ok(acvd_build_output(ctx_, 0, level))) return 0;
+    if (boundary_fixing_) {
+      int32_t changed = 0;
+      if (!ok(acvd_fix_mesh_boundaries(ctx_, 0, &changed))) return 0;
+    }
+    return FetchOutput();
+  }
+  // the output as last built (and fixed along the boundaries)
+  int FetchOutput() {
     int64_t nv = 0, nf = 0;
     if (!ok(acvd_get_output_sizes(ctx_, 0, &nv, &nf))) return 0;
     output_.points.resize(size_t(3 * nv));
@@ -240,6 +259,19 @@ class DiscreteRemeshing {
     std::fclose(f);
     return true;
   }
+  int Finish() {
+    restarts_ = 0;
+    if (force_manifold_) {
+      int64_t left = 0;
+      if (!ok(acvd_force_manifold(ctx_, 0, 100, &restarts_, &left))) return 0;
+      acvd_params p;
+      if (!ok(acvd_get_params(ctx_, 0, &p))) return 0;
+      params_.number_of_clusters = p.number_of_clusters;
+      clustering_.resize(size_t(input_->GetNumberOfPoints()));
+      if (!ok(acvd_get_labels(ctx_, 0, clustering_.data()))) return 0;
+    }
+    return BuildOutput(0);
+  }
   bool ensure_ctx() {
     if (ctx_) return true;
     return ok(acvd_create(&ctx_, device_));
@@ -265,6 +297,9 @@ class DiscreteRemeshing {
   const std::vector<double>* indicator_ = nullptr;
   int subsampling_threshold_ = 10;
   std::string output_directory_;
+  bool force_manifold_ = false;
+  int boundary_fixing_ = 0;
+  int32_t restarts_ = 0;
   int device_ = 0;
   acvd_ctx* ctx_ = nullptr;
   Labels clustering_;

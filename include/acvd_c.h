@@ -81,6 +81,22 @@ typedef struct acvd_params {
   double gradation;
 } acvd_params;
 
+/* Launch shape of a context (no reference counterpart: the reference's
+ * nearest knob is vtkThreadedClustering's PoolingRatio / mesh slices,
+ * DiscreteRemeshing/vtkThreadedClustering.h:58-74, 891-987).  None of the
+ * knobs changes the labels; they trade launches, occupancy and host syncs:
+ *   sweep_segments      evaluate / accept-round grid (blocks) cap, 1..2048 (default 1152)
+ *   tile_on_divisor     the scan's tile sets are kept once a pass moves at most K / divisor
+ *                       items, 1..1024 (default 16)
+ *   chunk_passes        passes per captured graph launch, 1..16 (default 16)
+ *   short_chunk_passes  passes per launch near the end of a phase, 1..chunk_passes (default 4) */
+typedef struct acvd_tuning {
+  int32_t sweep_segments;
+  int32_t tile_on_divisor;
+  int32_t chunk_passes;
+  int32_t short_chunk_passes;
+} acvd_tuning;
+
 /* Statistics of the last acvd_process_clustering() for one mesh. */
 typedef struct acvd_stats {
   int64_t loops;            /* passes run (NumberOfLoops) */
@@ -269,6 +285,73 @@ int acvd_get_items(acvd_ctx* ctx, int32_t mesh_id, float* xw, float* q);
 int acvd_get_scales(acvd_ctx* ctx, int32_t mesh_id, double* scales3);
 /* Runs the device build (CSR, items, scales) without clustering. */
 int acvd_build(acvd_ctx* ctx);
+
+/* ---- launch-shape tuning -------------------------------------------------------
+ * acvd_set_tuning / acvd_get_tuning: the context's launch shape (acvd_tuning);
+ * a new segment count rebuilds the device state at the next run.
+ * acvd_autotune: picks the launch shape for the context's current meshes,
+ * parameters and initial labels by timing full clusterings (the best of
+ * `repeats` runs each; coordinate descent over the knobs, a change kept when
+ * it saves more than 1 %), checks that every shape gives the same labels
+ * (ACVD_ERR_STATE otherwise), leaves the best shape set and the clustering of
+ * the last run in place.  best / best_ms (device ms per clustering) may be
+ * null.  Runs about 20 clusterings times `repeats`. */
+int acvd_set_tuning(acvd_ctx* ctx, const acvd_tuning* t);
+int acvd_get_tuning(acvd_ctx* ctx, acvd_tuning* t);
+int acvd_autotune(acvd_ctx* ctx, int32_t repeats, acvd_tuning* best, double* best_ms);
+
+/* ---- ForceManifold (Remesh with SetForceManifold(1)) ------------------------
+ * acvd_force_manifold: the restart loop of vtkDiscreteRemeshing::Remesh
+ * (vtkDiscreteRemeshing.h:937-946) on a context holding one clustered mesh.
+ * While the output triangulation, with the wire edges ForceManifold adds
+ * between clusters adjacent in the input but joined by no output edge
+ * (:1114-1131), has a non-manifold vertex whose cluster holds no non-manifold
+ * input vertex (DetectNonManifoldOutputVertices, :165-383): a new cluster takes
+ * an item of each such cluster, every cluster not next to one is frozen, and
+ * MinimizeEnergy runs again from the current labels (ConnexityConstraint 0).
+ * The number of clusters grows by the clusters added (acvd_get_params shows
+ * it).  The reference loops without bound; here at most max_restarts restarts
+ * run.  *restarts receives the restarts run, *remaining_issues the issues the
+ * last detection found (0 when the output passed).  The output mesh of the
+ * final clustering is built (acvd_get_output). */
+int acvd_force_manifold(acvd_ctx* ctx, int32_t mesh_id, int32_t max_restarts, int32_t* restarts,
+                        int64_t* remaining_issues);
+/* FixMeshBoundaries (vtkDiscreteRemeshing.h:425-579; Remesh runs it last when
+ * SetBoundaryFixing is on, :952) on the output built by acvd_build_output:
+ * along the input's boundary, every edge whose ends lie in two clusters adds
+ * an output vertex at its midpoint and the triangle (that vertex, the two
+ * clusters); each run of boundary edges inside one cluster adds (cluster,
+ * the first two such vertices met); a vertex between two cut edges adds
+ * (cluster, their two vertices) when the reference's GetNumberOfBoundaries
+ * test passes (Common/vtkSurfaceBase.cxx:939-949, which reads the mesh's first
+ * edges rather than the vertex's).  The output grows in place
+ * (acvd_get_output_sizes / acvd_get_output); *changed = 1 when it did. */
+int acvd_fix_mesh_boundaries(acvd_ctx* ctx, int32_t mesh_id, int32_t* changed);
+/* Host-only form of the same rule on given labels (0..K-1, others: none) of
+ * an input mesh: *nv_added vertices (output ids K, K + 1, ...) and *nf_added
+ * triangles that FixMeshBoundaries appends; call with null buffers for the
+ * counts. */
+int acvd_mesh_boundary_fix(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, const int32_t* labels,
+                           int32_t n_clusters, int64_t* nv_added, int64_t* nf_added, double* added_xyz,
+                           int32_t* added_tri);
+/* IsClusterFreezed per cluster (vtkUniformClustering.h:329, 910-919; no public
+ * setter in the reference: DetectNonManifoldOutputVertices sets it).  While
+ * set (n = number_of_clusters flags, nonzero = frozen; n = 0 clears), the
+ * following runs move items only between two clusters that are not frozen
+ * and count only the items of the others for the early stop (:733-736). */
+int acvd_set_frozen_clusters(acvd_ctx* ctx, int32_t mesh_id, int64_t n, const int32_t* flags);
+/* Host-only helpers of the loop (no device needed):
+ * IsVertexManifold (Common/vtkSurfaceBase.cxx:259-317) of every vertex:
+ * manifold[v] = 1 when v has at least two edges, each in exactly two faces,
+ * and one fan of faces through all of them (a boundary vertex is not). */
+int acvd_vertex_manifold(int64_t nv, int64_t nf, const int32_t* tri, uint8_t* manifold);
+/* One detection step (DetectNonManifoldOutputVertices) on given labels
+ * (0..K-1, other values: none) and output triangles: updates labels and
+ * *n_clusters with the added clusters, writes the frozen flags of the
+ * resulting clusters (capacity frozen_capacity; may be null) and *issues. */
+int acvd_force_manifold_step(int64_t nv, int64_t nf, const int32_t* tri, int64_t n_out, const int32_t* out_tri,
+                             int32_t* n_clusters, int32_t* labels, int32_t* frozen, int64_t frozen_capacity,
+                             int64_t* issues);
 
 /* ---- fixed-assignment energy and its adjoint -------------------------------
  * No reference counterpart: the reference minimises the clustering energy but

@@ -46,6 +46,8 @@ int g_progress = 0;
 // fixed clusters of the next runs and samplings (orc_set_fixed): cluster k
 // holds item g_fixed[k]
 std::vector<int32_t> g_fixed;
+// frozen clusters of the next serial runs and replays (orc_set_frozen)
+std::vector<int32_t> g_frozen;
 // energy log of the serial runs (orc_set_energy_log / orc_serial_energy_log)
 int g_energy_log = 0;
 std::vector<long double> g_serial_log;
@@ -58,6 +60,7 @@ int run_serial(const RefMesh& m, int K, double gradation, const double* ind, int
   Clustering<M> cl(m);
   cl.progress = g_progress;
   cl.fixed = g_fixed;
+  if (g_frozen.size() == size_t(K)) cl.frozen = g_frozen;
   cl.edge_trace = etrace;
   cl.edge_trace_cap = ecap;
   cl.loop_trace = ltrace;
@@ -239,6 +242,8 @@ void orc_set_progress(int on) { g_progress = on; }
 void orc_set_fixed(const int32_t* ids, int64_t n) { g_fixed.assign(ids, ids + (ids ? n : 0)); }
 
 void orc_set_energy_log(int on) { g_energy_log = on; }
+
+void orc_set_frozen(const int32_t* flags, int64_t n) { g_frozen.assign(flags, flags + (flags ? n : 0)); }
 
 // The energy log of the last serial run (long double values as double).
 int64_t orc_serial_energy_log(int64_t cap, double* energy, double* final_energy) {
@@ -470,6 +475,7 @@ int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* c
     r.anchor.assign(size_t(K), -1);
     for (size_t k = 0; k < g_fixed.size() && k < size_t(K); ++k) r.anchor[k] = g_fixed[k];
   }
+  if (g_frozen.size() == size_t(K)) r.frozen = g_frozen;
   r.generic_qem = (path & 1) != 0;
   r.all_rounds = (path & 2) != 0;
   // bits 8..15: accept rounds per pass before the first convergence (0: default)

@@ -78,6 +78,7 @@ def lib():
         L.orc_replay_energy_log.restype = ctypes.c_int64
         L.orc_replay_energy_log.argtypes = [ctypes.c_int64, P, P]
         L.orc_set_energy_log.argtypes = [ctypes.c_int]
+        L.orc_set_frozen.argtypes = [P, ctypes.c_int64]
         L.orc_serial_energy_log.restype = ctypes.c_int64
         L.orc_serial_energy_log.argtypes = [ctypes.c_int64, P, P]
         L.orc_devinputs_create.restype = P
@@ -232,6 +233,13 @@ def set_fixed(items):
     None or [] clears."""
     ids = np.zeros(0, np.int32) if items is None else np.ascontiguousarray(items, np.int32)
     lib().orc_set_fixed(_p(ids) if len(ids) else None, len(ids))
+
+
+def set_frozen(flags):
+    """Frozen clusters (IsClusterFreezed) of the following serial runs and
+    replays whose cluster count equals len(flags); None clears."""
+    f = np.zeros(0, np.int32) if flags is None else np.ascontiguousarray(flags, np.int32)
+    lib().orc_set_frozen(_p(f) if len(f) else None, len(f))
 
 
 def set_energy_log(on):

@@ -42,6 +42,7 @@ struct Replay {
 
   std::vector<int32_t> labels, sizes, stamp, cand_pass, pending, cl_round;
   std::vector<int32_t> anchor;  // per cluster: its fixed item or -1 (empty: no fixed clusters)
+  std::vector<int32_t> frozen;  // per cluster: nonzero when frozen (empty: none)
   std::vector<unsigned long long> acc_ws, acc_q, ring_adj;
   std::vector<acvd::ClusterGeom> geom;
   std::vector<double> T, evec;
@@ -98,6 +99,7 @@ struct Replay {
     v.cand_pass = cand_pass.data();
     v.ring_adj = ring_adj.empty() ? nullptr : ring_adj.data();
     v.anchor = anchor.empty() ? nullptr : anchor.data();
+    v.frozen = frozen.empty() ? nullptr : frozen.data();
     return v;
   }
 
@@ -157,7 +159,7 @@ struct Replay {
       if (metric == 1)
         for (int k = 0; k < 10; ++k) acc_q[10 * int64_t(c) + k] += (unsigned long long)acvd::fix_q(q[10 * i + k], sc);
       sizes[c]++;
-      count++;
+      if (frozen.empty() || !frozen[size_t(c)]) count++;  // early-stop items
     }
     for (int32_t c = 0; c < K; ++c) {
       derive(c, false);

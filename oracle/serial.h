@@ -80,6 +80,11 @@ struct Clustering {
   int initial_sampling_type = 1;
   const int32_t* initial_labels = nullptr;
   std::vector<int32_t> fixed;  // FixedClusters: cluster k holds item fixed[k] (vtkUniformClustering.h:134)
+  // IsClusterFreezed (vtkUniformClustering.h:329; set by the ForceManifold
+  // restarts): edges touching a frozen cluster are skipped (:910-919) and
+  // its items do not count for the early stop (:733-736); empty: none frozen
+  std::vector<int32_t> frozen;
+  bool is_frozen(int32_t c) const { return !frozen.empty() && frozen[size_t(c)] != 0; }
   int console = 0;
   int progress = 0;  // 1: one line per loop on stderr (long runs)
   RunStats stats;
@@ -476,7 +481,7 @@ struct Clustering {
         last_mod[v1] = loops;
         continue;
       }
-      if (last_mod[v1] < loops - 1 && last_mod[v2] < loops - 1) {
+      if ((last_mod[v1] < loops - 1 && last_mod[v2] < loops - 1) || is_frozen(v1) || is_frozen(v2)) {
         queue.push_back(edge);
         continue;
       }
@@ -549,7 +554,8 @@ struct Clustering {
     set_all_modified();
     int convergences = 0;
     int64_t early_stop_items = 0;
-    for (int i = 0; i < K; ++i) early_stop_items += sizes[i];
+    for (int i = 0; i < K; ++i)
+      if (!is_frozen(i)) early_stop_items += sizes[i];
     while (true) {
       swap_queues();
       const int loop_connexity = connexity;
