@@ -90,7 +90,7 @@ EXPORTS = [
     "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
     "acvd_set_fixed_clusters", "acvd_subdivide", "acvd_get_energy_log", "acvd_force_manifold",
     "acvd_set_frozen_clusters", "acvd_vertex_manifold", "acvd_force_manifold_step", "acvd_fix_mesh_boundaries",
-    "acvd_mesh_boundary_fix", "acvd_set_tuning", "acvd_get_tuning", "acvd_autotune",
+    "acvd_mesh_boundary_fix", "acvd_reference_seed_sampling", "acvd_set_tuning", "acvd_get_tuning", "acvd_autotune",
     "acvd_energy_forward_cvt_noalloc",
 ]
 
@@ -160,6 +160,7 @@ def lib():
     L.acvd_force_manifold.argtypes = [P, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i64)]
     L.acvd_set_frozen_clusters.argtypes = [P, i32, i64, P]
     L.acvd_fix_mesh_boundaries.argtypes = [P, i32, ctypes.POINTER(i32)]
+    L.acvd_reference_seed_sampling.argtypes = [i64, i32, P]
     L.acvd_set_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
     L.acvd_get_tuning.argtypes = [P, ctypes.POINTER(Tuning)]
     L.acvd_autotune.argtypes = [P, i32, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double)]

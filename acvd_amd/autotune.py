@@ -54,8 +54,14 @@ def apply_cached(ctx):
 def tune(ctx, repeats=2, note="", save=True):
     """Runs the library's autotuner on ctx and records the result."""
     t0 = time.time()
+    start = ctx.get_tuning()
+    runs = []
+    for _ in range(repeats):
+        ctx.process()
+        runs.append(ctx.stats(0)["converge_ms"])
     best, ms = ctx.autotune(repeats)
-    entry = {"tuning": best, "ms": round(ms, 2), "tuned_s": round(time.time() - t0, 1), "workload": note}
+    entry = {"tuning": best, "ms": round(ms, 2), "start": start, "start_ms": round(min(runs), 2),
+             "tuned_s": round(time.time() - t0, 1), "workload": note}
     if save:
         cache = load()
         cache[size_class(ctx)] = entry

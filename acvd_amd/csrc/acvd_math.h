@@ -356,9 +356,16 @@ ACVD_HD bool qem_accept(double delta, double mag) { return delta < -double(kAcce
 // item's position in every phase, constrained or not, so its energy is
 // -|S|^2/W + T with T = W |a - c|^2 and no solve.  The anchor item itself
 // never leaves (the reference prices that at 1e100).
-ACVD_HD double anchored_T(const ClusterGeom& g, const double* a) {
+// Anchor position of a cluster, passed by value (valid = false: the cluster
+// has none) so that no private array holds it.
+struct AnchorPos {
+  double x, y, z;
+  bool valid;
+};
+
+ACVD_HD double anchored_T(const ClusterGeom& g, const AnchorPos& a) {
   if (!(g.W > 0.0)) return 0.0;
-  const double dx = a[0] - g.cx, dy = a[1] - g.cy, dz = a[2] - g.cz;
+  const double dx = a.x - g.cx, dy = a.y - g.cy, dz = a.z - g.cz;
   return g.W * (dx * dx + dy * dy + dz * dz);
 }
 
@@ -379,16 +386,16 @@ ACVD_HD double log_cluster_energy(const ClusterGeom& g, double T) {
 constexpr int kLogThreads = 256;
 
 // T of a trial cluster = accumulators (ws: W,Sx,Sy,Sz; q10) + sign * item:
-// with the anchor position apos when the cluster is anchored, else (solve)
+// with the anchor position an when the cluster is anchored, else (solve)
 // the rank-truncated solve warm-started from the cluster's own eigenvectors
 // V0 when given, else 0 (the cluster's point is its centroid).
 ACVD_HD double qem_T_trial(const int64_t* ws, const int64_t* q10, int64_t iw, const int64_t* is,
                            const int64_t* iq, int sign, const Scales& s, int level, const double* V0 = nullptr,
-                           const double* apos = nullptr, bool solve = true) {
+                           AnchorPos an = AnchorPos{0.0, 0.0, 0.0, false}, bool solve = true) {
   int64_t W = ws[0] + sign * iw;
   int64_t Sx = ws[1] + sign * is[0], Sy = ws[2] + sign * is[1], Sz = ws[3] + sign * is[2];
   ClusterGeom g = geom_from_fixed(W, Sx, Sy, Sz, s);
-  if (apos) return anchored_T(g, apos);
+  if (an.valid) return anchored_T(g, an);
   if (!solve) return 0.0;
   int64_t Q[10];
   for (int k = 0; k < 10; ++k) Q[k] = q10[k] + sign * iq[k];
@@ -556,14 +563,16 @@ struct SweepView {
 ACVD_HD bool is_frozen(const SweepView& v, int32_t c) { return v.frozen && v.frozen[c]; }
 
 // Anchor position of cluster c (FP32 item position, as the sweep's items), if any.
-ACVD_HD bool anchor_pos(const SweepView& v, int32_t c, double* p) {
-  if (!v.anchor) return false;
+ACVD_HD AnchorPos anchor_pos(const SweepView& v, int32_t c) {
+  AnchorPos p{0.0, 0.0, 0.0, false};
+  if (!v.anchor) return p;
   const int32_t it = v.anchor[c];
-  if (it < 0) return false;
-  p[0] = double(v.xw[4 * int64_t(it)]);
-  p[1] = double(v.xw[4 * int64_t(it) + 1]);
-  p[2] = double(v.xw[4 * int64_t(it) + 2]);
-  return true;
+  if (it < 0) return p;
+  p.x = double(v.xw[4 * int64_t(it)]);
+  p.y = double(v.xw[4 * int64_t(it) + 1]);
+  p.z = double(v.xw[4 * int64_t(it) + 2]);
+  p.valid = true;
+  return p;
 }
 
 // Item i is the anchor of its cluster lab: it never moves.
@@ -687,24 +696,20 @@ ACVD_HD void item_fixed(const SweepView& v, int64_t i, const Scales& sc, int64_t
 // mode (qem_mode): 1 every cluster's point is solved, 2 only anchored
 // clusters carry a T (the others keep their centroid), 0 no T at all.
 ACVD_HD double qem_leave_T(const SweepView& v, int64_t i, int32_t lab, const Scales& sc, int level, int mode) {
-  double a[3];
-  const bool an = anchor_pos(v, lab, a);
-  if (!an && mode != 1) return 0.0;
+  const AnchorPos an = anchor_pos(v, lab);
+  if (!an.valid && mode != 1) return 0.0;
   int64_t iw, is[3], iq[10], ws[4], q10[10];
   item_fixed(v, i, sc, &iw, is, iq);
   load_acc(v, lab, ws, q10);
-  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level, v.evec ? v.evec + 9 * int64_t(lab) : nullptr,
-                     an ? a : nullptr);
+  return qem_T_trial(ws, q10, iw, is, iq, -1, sc, level, v.evec ? v.evec + 9 * int64_t(lab) : nullptr, an);
 }
 ACVD_HD double qem_join_T(const SweepView& v, int64_t i, int32_t b, const Scales& sc, int level, int mode) {
-  double a[3];
-  const bool an = anchor_pos(v, b, a);
-  if (!an && mode != 1) return 0.0;
+  const AnchorPos an = anchor_pos(v, b);
+  if (!an.valid && mode != 1) return 0.0;
   int64_t iw, is[3], iq[10], ws[4], q10[10];
   item_fixed(v, i, sc, &iw, is, iq);
   load_acc(v, b, ws, q10);
-  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level, v.evec ? v.evec + 9 * int64_t(b) : nullptr,
-                     an ? a : nullptr);
+  return qem_T_trial(ws, q10, iw, is, iq, +1, sc, level, v.evec ? v.evec + 9 * int64_t(b) : nullptr, an);
 }
 
 // Choice among the open targets (CSR order; ties: lowest cluster id) and the

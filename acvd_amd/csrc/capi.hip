@@ -946,9 +946,19 @@ int initial_labels(acvd_ctx* c) {
   SweepArgs a = c->args();
   for (auto& h : c->meshes) {
     const int32_t K = h.params.number_of_clusters;
-    if (!h.init_labels.empty() || h.params.initial_sampling == 0) {
+    if (!h.init_labels.empty() || h.params.initial_sampling != 1) {
       std::vector<int32_t> lab(h.nv);
-      if (h.init_labels.empty()) {
+      if (h.init_labels.empty() && h.params.initial_sampling >= 2) {
+        // the reference's sampling types 0 (one random item per cluster) and
+        // -1 (item k for cluster k)
+        if (int64_t(K) > h.nv) return fail(ACVD_ERR_ARG, "more clusters than items");
+        if (h.params.initial_sampling == 2) {
+          reference_seed_sampling(h.nv, K, lab.data());
+        } else {
+          for (int64_t i = 0; i < h.nv; ++i) lab[i] = i < K ? int32_t(i) : K;
+        }
+        for (int64_t i = 0; i < h.nv; ++i) lab[i] = lab[i] == K ? kNull : int32_t(lab[i] + h.clus_off);
+      } else if (h.init_labels.empty()) {
         // the reference's region growing, then local ids -> global
         // the indicator of the weights: the set one, else the computed curvature
         const std::vector<double>& ind = !h.indicator.empty() ? h.indicator : h.curvature;
@@ -1065,7 +1075,7 @@ int acvd_set_params(acvd_ctx* ctx, int32_t mesh_id, const acvd_params* p) {
   if (p->metric != ACVD_METRIC_ISOTROPIC && p->metric != ACVD_METRIC_QEM) return fail(ACVD_ERR_ARG, "unknown metric");
   if (p->quadrics_optimization_level < 0 || p->quadrics_optimization_level > 3)
     return fail(ACVD_ERR_ARG, "quadrics optimization level must be 0..3");
-  if (p->initial_sampling != 0 && p->initial_sampling != 1) return fail(ACVD_ERR_ARG, "initial sampling must be 0 or 1");
+  if (p->initial_sampling < 0 || p->initial_sampling > 3) return fail(ACVD_ERR_ARG, "initial sampling must be 0..3");
   if (p->max_number_of_loops < 0) return fail(ACVD_ERR_ARG, "negative loop limit");
   if (p->first_phase_rounds < 0 || p->first_phase_rounds > kMatchRounds)
     return fail(ACVD_ERR_ARG, "first phase rounds must be 0..12");
@@ -1167,7 +1177,7 @@ int acvd_process_clustering(acvd_ctx* ctx) {
     if (h.params.number_of_clusters <= 0) return fail(ACVD_ERR_ARG, "must set NumberOfClusters to more than zero");
     if (int64_t(h.fixed.size()) > h.params.number_of_clusters)
       return fail(ACVD_ERR_ARG, "more fixed clusters than clusters");
-    if (!h.fixed.empty() && h.params.initial_sampling == 1 && h.init_labels.empty())
+    if (!h.fixed.empty() && h.params.initial_sampling != 0 && h.init_labels.empty())
       return fail(ACVD_ERR_ARG, "fixed clusters need the reference's initial sampling (0) or initial labels");
     if (!h.frozen.empty() && int64_t(h.frozen.size()) != h.params.number_of_clusters)
       return fail(ACVD_ERR_ARG, "frozen flags do not match the number of clusters");
@@ -1476,6 +1486,13 @@ int acvd_force_manifold(acvd_ctx* ctx, int32_t mesh_id, int32_t max_restarts, in
   if (restarts) *restarts = n;
   if (remaining_issues) *remaining_issues = issues;
   return rc;
+}
+
+int acvd_reference_seed_sampling(int64_t nv, int32_t n_clusters, int32_t* labels) {
+  if (nv <= 0 || !labels) return fail(ACVD_ERR_ARG, "empty mesh or null buffer");
+  if (n_clusters <= 0 || int64_t(n_clusters) > nv) return fail(ACVD_ERR_ARG, "clusters must be 1..items");
+  reference_seed_sampling(nv, n_clusters, labels);
+  return ACVD_OK;
 }
 
 int acvd_mesh_boundary_fix(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, const int32_t* labels,

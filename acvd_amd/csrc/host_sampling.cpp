@@ -38,7 +38,57 @@ double face_area(const double* xyz, const int32_t* t) {
   return 0.25 * std::sqrt(std::fabs(4.0 * a * c - (a - b + c) * (a - b + c)));
 }
 
+// vtkMath::Random as VTK computes it: vtkMinimalStandardRandomSequence, the
+// Park-Miller "minimal standard" generator (multiplier 16807, modulus
+// 2^31 - 1, Schrage's decomposition q = 127773, r = 2836).  SetSeed maps the
+// seed into [1, 2^31 - 2] and advances three times; Random() advances, then
+// returns seed / (2^31 - 1); Random(a, b) = a + Random() * (b - a).
+struct MinimalStandard {
+  int32_t seed = 1;
+  void set_seed(int32_t v) {
+    seed = v;
+    if (seed < 1) seed += 2147483646;
+    else if (seed == 2147483647) seed = 1;
+    next();
+    next();
+    next();
+  }
+  void next() {
+    const int32_t hi = seed / 127773, lo = seed % 127773;
+    seed = 16807 * lo - 2836 * hi;
+    if (seed <= 0) seed += 2147483647;
+  }
+  double random() {
+    next();
+    return double(seed) / 2147483647.0;
+  }
+  double random(double a, double b) {
+    next();
+    return a == b ? a : a + (double(seed) / 2147483647.0) * (b - a);
+  }
+};
+
 }  // namespace
+
+bool reference_seed_sampling(int64_t nv, int32_t K, int32_t* labels) {
+  for (int64_t i = 0; i < nv; ++i) labels[i] = K;
+  if (K < 0 || int64_t(K) > nv) return false;
+  MinimalStandard rng;
+  rng.set_seed(5000);
+  (void)rng.random();
+  const float size = float(nv);
+  for (int32_t c = 0; c < K; ++c)
+    for (;;) {
+      const float test = float(rng.random(0.0, double(size)));
+      const int64_t number = int64_t(std::floor(double(test) + 0.5));
+      if (number >= nv) continue;  // the reference reads one past its labels here
+      if (labels[number] == K) {
+        labels[number] = c;
+        break;
+      }
+    }
+  return true;
+}
 
 void vertex_topology(int64_t nv, int64_t nf, const int32_t* tri, VertexTopology* out) {
   VertexTopology& T = *out;

@@ -27,4 +27,12 @@ bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const
                                 const double* indicator, double gradation, int32_t K, int32_t* labels,
                                 int64_t n_fixed = 0, const int32_t* fixed = nullptr);
 
+// InitSamples with sampling type 0 (vtkUniformClustering.h:1123-1159): one
+// item per cluster drawn with vtkMath::Random seeded 5000 (an item already
+// taken is drawn again), every other item null.  labels receives 0..K-1,
+// K = null.  The reference's draw can land one past the last item (it
+// rounds a draw in [0, nv]); such a draw is drawn again.  Returns false when
+// K exceeds the items.
+bool reference_seed_sampling(int64_t nv, int32_t K, int32_t* labels);
+
 }  // namespace acvd

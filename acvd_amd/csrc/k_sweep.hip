@@ -118,8 +118,7 @@ __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm,
     const int32_t an = a.anchor ? a.anchor[c] : -1;
     if (an >= 0) {  // a fixed cluster: its point is its anchor item's position
       const float4 p = a.xw[an];
-      const double ap[3] = {double(p.x), double(p.y), double(p.z)};
-      T = anchored_T(g, ap);
+      T = anchored_T(g, AnchorPos{double(p.x), double(p.y), double(p.z), true});
     } else if (constrained) {
       int64_t q10[10];
       for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);
@@ -655,15 +654,13 @@ __device__ bool qem_pool_choose(const SweepArgs& a, const SweepView& v, QemPool*
         const int32_t c = tc < 0 ? ~tc : tc;
         const DevMesh& dm = a.meshes[tm];
         const Scales sc = dm.scales;
-        double ap[3];
-        const bool an = anchor_pos(v, c, ap);
+        const AnchorPos an = anchor_pos(v, c);
         double Tn = 0.0;  // a centroid point (an unconstrained mesh's unanchored cluster)
-        if (an || a.state[tm].constrained) {
+        if (an.valid || a.state[tm].constrained) {
           int64_t iw, is[3], iq[10], ws[4], q10[10];
           item_fixed(v, ti, sc, &iw, is, iq);
           load_acc(v, c, ws, q10);
-          Tn = qem_T_trial(ws, q10, iw, is, iq, tc < 0 ? -1 : 1, sc, dm.level, v.evec + 9 * int64_t(c),
-                           an ? ap : nullptr);
+          Tn = qem_T_trial(ws, q10, iw, is, iq, tc < 0 ? -1 : 1, sc, dm.level, v.evec + 9 * int64_t(c), an);
         }
         pool->Tn[lane] = Tn;
         if (tc >= 0) {
@@ -1413,8 +1410,7 @@ __global__ void k_log_cluster_energy(SweepArgs a, double* __restrict__ e) {
     const int32_t an = a.anchor ? a.anchor[c] : -1;
     if (an >= 0) {
       const float4 p = a.xw[an];
-      const double ap[3] = {double(p.x), double(p.y), double(p.z)};
-      T = anchored_T(g, ap);
+      T = anchored_T(g, AnchorPos{double(p.x), double(p.y), double(p.z), true});
     } else if (a.state[m].constrained) {
       int64_t q10[10];
       for (int k = 0; k < 10; ++k) q10[k] = ld_i64(&a.acc_q[10 * c + k]);

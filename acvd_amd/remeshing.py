@@ -307,6 +307,14 @@ def force_manifold_step(n_vertices, triangles, labels, n_clusters, out_triangles
     return n.value, lab, k.value, frozen[:k.value]
 
 
+def reference_seed_sampling(n_items, n_clusters):
+    """InitSamples type 0 (acvd_reference_seed_sampling): labels 0..K-1,
+    K = null, one random item per cluster."""
+    out = np.empty(int(n_items), np.int32)
+    check(N.lib().acvd_reference_seed_sampling(int(n_items), int(n_clusters), _p(out)))
+    return out
+
+
 def mesh_boundary_fix(points, triangles, labels, n_clusters):
     """FixMeshBoundaries on given labels (acvd_mesh_boundary_fix): the output
     vertices it adds (ids n_clusters, ...) and the triangles it appends."""
@@ -421,6 +429,16 @@ class _DiscreteRemeshing:
     def SetMaxNumberOfConvergences(self, v):
         self._params["max_number_of_convergences"] = int(v)
 
+    def SetInitialSamplingType(self, t):
+        """vtkUniformClustering::SetInitialSamplingType (vtkUniformClustering.h:108,
+        1123-1175): 1 the region growing (default), 0 one random item per
+        cluster (vtkMath::Random seeded 5000), -1 item k for cluster k, 2 the
+        labels of SetInitialClustering."""
+        t = int(t)
+        if t not in (-1, 0, 1, 2):
+            raise ValueError("initial sampling type must be -1, 0, 1 or 2")
+        self._params["initial_sampling"] = {1: 0, 0: 2, -1: 3, 2: 0}[t]
+
     def SetInitialClustering(self, labels):
         self._initial = None if labels is None else np.ascontiguousarray(labels, np.int32)
 
@@ -465,6 +483,13 @@ class _DiscreteRemeshing:
 
     def SetDisplay(self, d):
         pass  # no rendering on this path
+
+    def SetNumberOfThreads(self, n):
+        """vtkThreadedClustering::SetNumberOfThreads (vtkThreadedClustering.h:58-74,
+        ACVDP / ACVDQP): accepted for drop-in use; the device runs the sweep."""
+
+    def SetPoolingRatio(self, r):
+        """vtkThreadedClustering::SetPoolingRatio: accepted; see SetNumberOfThreads."""
 
     def SetSeed(self, s):
         self._params["seed"] = int(s) & 0xFFFFFFFF

@@ -151,6 +151,10 @@ def run_gpu_leg(args):
     N = len(xyz)
     log(f"workload: torus 4096x2048, {N} vertices, {len(tri)} triangles, K={K}")
     ctx = make_ctx(xyz, tri, H, K)
+    from acvd_amd import autotune
+    cached = autotune.apply_cached(ctx)
+    tuning = ctx.get_tuning()
+    log(f"launch shape {tuning} ({'autotuner cache' if cached else 'library defaults'})")
     t = time.time()
     ctx.build()
     log(f"device build (CSR + items + scales): {time.time() - t:.2f} s wall")
@@ -210,6 +214,7 @@ def run_gpu_leg(args):
     return {
         "N": N, "K": K, "init": init, "labels": labels, "dev_ms": dev_ms, "wall_ms": wall_ms, "stats": st,
         "energy_gpu": energy, "xyz": xyz, "tri": tri, "H": H, "init_ms": init_ms, "serial": serial,
+        "tuning": {**tuning, "source": "acvd_amd/tuning.json" if cached else "library defaults"},
         "roofline": {"bound": "hbm", "achieved": gd["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gd["frac"], "traffic": measured_traffic(names[dom]),
                      "traffic_source": "profiles/c3_pmc_traffic.json", "kernel": names[dom],
@@ -481,7 +486,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "torus 4096x2048 (8,388,608 vertices, 16.8M triangles) -> 100,000 clusters, QEM "
                                "(ACVDQ, unconstrained init), gradation 1.5 on |H|, FP32 positions",
-                   "items": g["N"], "clusters": g["K"], "parallelism": "single GPU"},
+                   "items": g["N"], "clusters": g["K"], "parallelism": "single GPU", "launch_shape": g["tuning"]},
         "passes": g["stats"]["loops"],
         "moves": g["stats"]["total_moves"],
         "item_moves_per_s": round(g["stats"]["total_moves"] / (statistics.median(g["dev_ms"]) * 1e-3)),

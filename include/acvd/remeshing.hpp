@@ -98,7 +98,17 @@ class DiscreteRemeshing {
   // then "Final Energy :<energy>" (:695-699, 1383-1395)
   void SetWriteToGlobalEnergyLog(int v) { params_.write_energy_log = v; }
   void SetOutputDirectory(const char* d) { output_directory_ = d ? d : ""; }
+  // SetInitialSamplingType (vtkUniformClustering.h:108, 1123-1175): 1 the
+  // region growing (default), 0 one random item per cluster, -1 item k for
+  // cluster k, 2 the SetInitialClustering labels
+  void SetInitialSamplingType(int t) { params_.initial_sampling = t == 0 ? 2 : (t == -1 ? 3 : 0); }
   void SetSeed(uint32_t s) { params_.seed = s; }
+  // accepted for drop-in use (vtkThreadedClustering.h:58-74, ACVDP / ACVDQP;
+  // SetDisplay, vtkUniformClustering.h:87): the device runs the sweep and
+  // nothing is rendered
+  void SetNumberOfThreads(int) {}
+  void SetPoolingRatio(int) {}
+  void SetDisplay(int) {}
   void SetDevice(int d) { device_ = d; }
   MetricHandle* GetMetric() { return &metric_; }
   int64_t GetNumberOfItems() const { return input_ ? input_->GetNumberOfPoints() : 0; }

@@ -52,7 +52,10 @@ typedef struct acvd_ctx acvd_ctx;
  *   seed                        <- (no counterpart) seed of the accept priorities and of the device sampling
  *   initial_sampling            <- InitSamples type (vtkUniformClustering.h:1123-1175): 0 = the reference's
  *                                  default region growing (type 1, :1177-1316, sequential, on the host),
- *                                  1 = weighted random seeds grown on the device (no counterpart; faster)
+ *                                  1 = weighted random seeds grown on the device (no counterpart; faster),
+ *                                  2 = the reference's type 0 (:1126-1159): one item per cluster drawn with
+ *                                  vtkMath::Random seeded 5000, the rest grown by the fill; 3 = its type -1
+ *                                  (:1148): item k starts cluster k
  *   curvature_indicator         <- vtkDiscreteRemeshing::SamplingPreProcessing (vtkDiscreteRemeshing.h:583-653):
  *                                  1 = when gradation > 0 and no indicator is set, the build computes the
  *                                  reference's curvature indicator (vtkCurvatureMeasure, polynomial fit on
@@ -327,6 +330,10 @@ int acvd_force_manifold(acvd_ctx* ctx, int32_t mesh_id, int32_t max_restarts, in
  * edges rather than the vertex's).  The output grows in place
  * (acvd_get_output_sizes / acvd_get_output); *changed = 1 when it did. */
 int acvd_fix_mesh_boundaries(acvd_ctx* ctx, int32_t mesh_id, int32_t* changed);
+/* InitSamples type 0 on the host (vtkUniformClustering.h:1126-1159): labels
+ * (0..K-1, K = null) with one item per cluster drawn with vtkMath::Random
+ * seeded 5000 (VTK's minimal standard generator). */
+int acvd_reference_seed_sampling(int64_t nv, int32_t n_clusters, int32_t* labels);
 /* Host-only form of the same rule on given labels (0..K-1, others: none) of
  * an input mesh: *nv_added vertices (output ids K, K + 1, ...) and *nf_added
  * triangles that FixMeshBoundaries appends; call with null buffers for the

@@ -327,6 +327,19 @@ int orc_initial_sampling(void* mh, int metric, int K, double gradation, const do
   return 0;
 }
 
+// InitSamples type 0 (vtkUniformClustering.h:1126-1159) on a mesh: labels
+// 0..K-1, K = null.
+int orc_seed_sampling(void* mh, int K, int32_t* out) {
+  const RefMesh& m = *static_cast<RefMesh*>(mh);
+  Clustering<IsoMetric> cl(m);
+  cl.set_number_of_clusters(K);
+  cl.allocate();
+  cl.initial_sampling_type = 0;
+  cl.init_samples();
+  for (int64_t i = 0; i < m.nv; ++i) out[i] = cl.labels[i];
+  return 0;
+}
+
 // Reference FillHolesInClustering (connexity off) on a labelling (in place).
 int orc_fill_holes(void* mh, int K, int32_t* labels) {
   const RefMesh& m = *static_cast<RefMesh*>(mh);
@@ -443,6 +456,15 @@ void orc_hdr_fix_item(const float* xw4, const float* q10, const double* scales3,
 
 // The C++ standard fixes the 10000th output of a default-seeded std::mt19937
 // (4123659995); the reference's initial sampling draws from this engine.
+// Park and Miller's published check of the minimal standard generator: from
+// x = 1, the 10000th value is 1043618065 (the oracle's VtkMinStd step).
+int64_t orc_minstd_10000(void) {
+  VtkMinStd g;
+  g.x = 1;
+  for (int i = 0; i < 10000; ++i) g.step();
+  return g.x;
+}
+
 uint32_t orc_mt19937_10000(void) {
   std::mt19937 g;
   uint32_t v = 0;

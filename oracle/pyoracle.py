@@ -92,6 +92,8 @@ def lib():
         L.orc_hdr_qem_T.argtypes = [P, P, P, ctypes.c_int, P]
         L.orc_hdr_fix_item.argtypes = [P, P, P, P]
         L.orc_mt19937_10000.restype = ctypes.c_uint32
+        L.orc_minstd_10000.restype = ctypes.c_int64
+        L.orc_seed_sampling.argtypes = [P, ctypes.c_int, P]
         L.orc_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -202,6 +204,12 @@ class RefMesh:
         out = np.empty(self.nv, np.int32)
         ind = None if indicator is None else np.ascontiguousarray(indicator, np.float64)
         lib().orc_initial_sampling(self.h, metric, K, gradation, _p(ind), _p(out))
+        return out
+
+    def seed_sampling(self, K):
+        """InitSamples type 0 (vtkMath::Random seeded 5000)."""
+        out = np.empty(self.nv, np.int32)
+        lib().orc_seed_sampling(self.h, K, _p(out))
         return out
 
     def fill_holes(self, K, labels):
@@ -381,6 +389,10 @@ def hdr_fix_item(xw4, q10, scales):
     lib().orc_hdr_fix_item(_p(np.ascontiguousarray(xw4, np.float32)), _p(qq),
                            _p(np.ascontiguousarray(scales, np.float64)), _p(out))
     return out
+
+
+def minstd_10000():
+    return int(lib().orc_minstd_10000())
 
 
 def mt19937_10000():

@@ -67,8 +67,8 @@ struct Replay {
       double Tc = 0.0;
       if (metric == 1) {
         if (!anchor.empty() && anchor[c] >= 0) {
-          const double a[3] = {double(xw[4 * int64_t(anchor[c])]), double(xw[4 * int64_t(anchor[c]) + 1]),
-                               double(xw[4 * int64_t(anchor[c]) + 2])};
+          const acvd::AnchorPos a{double(xw[4 * int64_t(anchor[c])]), double(xw[4 * int64_t(anchor[c]) + 1]),
+                                  double(xw[4 * int64_t(anchor[c]) + 2]), true};
           Tc = acvd::anchored_T(g, a);
         } else if (st.constrained) {
           Tc = acvd::qem_T_from_fixed(reinterpret_cast<const int64_t*>(&acc_q[10 * int64_t(c)]), g, sc, level,
@@ -133,8 +133,8 @@ struct Replay {
       double V0[9];
       for (int k = 0; k < 9; ++k) V0[k] = evec[9 * int64_t(c) + k];
       if (!anchor.empty() && anchor[c] >= 0) {
-        const double a[3] = {double(xw[4 * int64_t(anchor[c])]), double(xw[4 * int64_t(anchor[c]) + 1]),
-                             double(xw[4 * int64_t(anchor[c]) + 2])};
+        const acvd::AnchorPos a{double(xw[4 * int64_t(anchor[c])]), double(xw[4 * int64_t(anchor[c]) + 1]),
+                                double(xw[4 * int64_t(anchor[c]) + 2]), true};
         T[c] = acvd::anchored_T(geom[c], a);
       } else {
         T[c] = st.constrained ? acvd::qem_T_from_fixed(q10, geom[c], sc, level, nullptr, warm ? V0 : nullptr,

@@ -50,6 +50,31 @@ struct LoopTrace {
   int32_t loop, mods, connexity, constrained, convergence, disconnected;
 };
 
+// vtkMath's uniform sequence (vtkMinimalStandardRandomSequence): Park and
+// Miller's minimal standard x' = 16807 x mod (2^31 - 1), computed with
+// Schrage's method; SetSeed clamps the seed into [1, 2^31 - 2] and steps
+// three times; each draw steps, then reads x / (2^31 - 1).
+struct VtkMinStd {
+  int64_t x = 1;
+  void step() { x = (16807 * x) % 2147483647; }
+  void set_seed(int32_t s) {
+    x = s;
+    if (x < 1) x += 2147483646;
+    else if (x == 2147483647) x = 1;
+    step();
+    step();
+    step();
+  }
+  double uniform() {
+    step();
+    return double(x) / 2147483647.0;
+  }
+  double range(double a, double b) {
+    step();
+    return a == b ? a : a + (double(x) / 2147483647.0) * (b - a);
+  }
+};
+
 struct RunStats {
   int64_t loops = 0;
   int64_t convergences = 0;
@@ -427,12 +452,18 @@ struct Clustering {
     } else if (initial_sampling_type == 1) {
       initial_random_sampling(labels, K);
     } else {
-      // type 0: one random item per cluster.  The reference draws with
-      // vtkMath::Random seeded 5000; this uses std::mt19937_64 instead.
-      std::mt19937_64 rng(type0_seed);
+      // type 0 (:1126-1159): vtkMath::RandomSeed(5000), one Random(), then
+      // per cluster Random(0, n) as float, rounded, redrawn while taken (a
+      // draw rounding to n, past the last item, is redrawn as well)
+      VtkMinStd rng;
+      rng.set_seed(int32_t(type0_seed));
+      (void)rng.uniform();
+      const float size = float(n_items());
       for (int c = 0; c < K; ++c) {
         for (;;) {
-          int64_t number = int64_t(rng() % uint64_t(n_items()));
+          const float test = float(rng.range(0.0, double(size)));
+          const int64_t number = int64_t(std::floor(double(test) + 0.5));
+          if (number >= n_items()) continue;
           if (labels[number] == K) {
             labels[number] = c;
             break;
