@@ -108,6 +108,8 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(SweepArgs a, const int32_
 // the solve leaves the cluster's eigenvectors in evec, which warm-start the
 // trial solves on the cluster and, with warm = true, this solve itself (the
 // derive after a move, whose quadric differs from the last one by one item).
+// kExtras = false: no mesh has fixed clusters (the anchor test is compiled out).
+template <bool kExtras = true>
 __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm, int constrained, bool warm) {
   const int64_t W = ld_i64(&a.acc_ws[4 * c]);
   const ClusterGeom g = geom_from_fixed(W, ld_i64(&a.acc_ws[4 * c + 1]), ld_i64(&a.acc_ws[4 * c + 2]),
@@ -115,7 +117,7 @@ __device__ void derive_cluster(const SweepArgs& a, int64_t c, const DevMesh& dm,
   a.geom[c] = g;
   if (dm.metric == 1) {
     double T = 0.0;
-    const int32_t an = a.anchor ? a.anchor[c] : -1;
+    const int32_t an = (kExtras && a.anchor) ? a.anchor[c] : -1;
     if (an >= 0) {  // a fixed cluster: its point is its anchor item's position
       const float4 p = a.xw[an];
       T = anchored_T(g, AnchorPos{double(p.x), double(p.y), double(p.z), true});
@@ -142,6 +144,10 @@ __global__ void k_derive(SweepArgs a, const int32_t* __restrict__ mask, int set_
 }
 
 // ------------------------------------------------------------------- propose
+// kExtras = false: no fixed or frozen cluster exists in the context, and the
+// view says so at compile time (the kernels' anchor and freeze tests fold
+// away; the host picks the instantiation).
+template <bool kExtras>
 __device__ inline SweepView make_view(const SweepArgs& a) {
   SweepView v;
   v.row_ptr = a.row_ptr;
@@ -158,8 +164,8 @@ __device__ inline SweepView make_view(const SweepArgs& a) {
   v.stamp = a.stamp;
   v.cand_pass = a.cand_pass;
   v.ring_adj = a.ring_adj;
-  v.anchor = a.anchor;
-  v.frozen = a.frozen;
+  v.anchor = kExtras ? a.anchor : nullptr;
+  v.frozen = kExtras ? a.frozen : nullptr;
   return v;
 }
 
@@ -612,6 +618,7 @@ __device__ inline void pool_post_long(const SweepView& v, QemPool* pool, int64_t
 // solves first (a wave posts at most kPoolTasks of them), so that the lanes
 // keep only the choice state across the solve steps; long-row owners post
 // after them what fits and the rest in later fills.
+template <bool kExtras>
 __device__ bool qem_pool_choose(const SweepArgs& a, const SweepView& v, QemPool* pool, int kind, int ntask,
                                 int64_t i, int32_t lab, int32_t m, const int32_t* nb, const QemItem& s,
                                 uint32_t open, int32_t* target_out) {
@@ -654,7 +661,7 @@ __device__ bool qem_pool_choose(const SweepArgs& a, const SweepView& v, QemPool*
         const int32_t c = tc < 0 ? ~tc : tc;
         const DevMesh& dm = a.meshes[tm];
         const Scales sc = dm.scales;
-        const AnchorPos an = anchor_pos(v, c);
+        const AnchorPos an = kExtras ? anchor_pos(v, c) : AnchorPos{0.0, 0.0, 0.0, false};
         double Tn = 0.0;  // a centroid point (an unconstrained mesh's unanchored cluster)
         if (an.valid || a.state[tm].constrained) {
           int64_t iw, is[3], iq[10], ws[4], q10[10];
@@ -711,7 +718,7 @@ __device__ bool qem_pool_choose(const SweepArgs& a, const SweepView& v, QemPool*
 // segment.  Block b owns segment b (capacity cand_cap, count seg_cand[b]);
 // the second launch of a mixed batch (append = 1) continues the segments of
 // the first.
-template <int METRIC>
+template <int METRIC, bool kExtras>
 __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
   __shared__ WorkIndex wi;
   __shared__ KeyAgg agg;
@@ -721,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
   keyagg_clear(&agg);
   load_work_index(a.work_count, &wi);
   const int32_t n = wi.start[kWorkSlices];
-  const SweepView v = make_view(a);
+  const SweepView v = make_view<kExtras>(a);
   const int64_t seg = int64_t(blockIdx.x) * a.cand_cap;
   for (int32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
     const int32_t t = base + int32_t(threadIdx.x);
@@ -748,7 +755,8 @@ __global__ __launch_bounds__(kBlock) void k_eval(SweepArgs a, int append) {
     }
     if constexpr (METRIC == 1) {
       int32_t pt = kNull;
-      const bool r = qem_pool_choose(a, v, &pools[threadIdx.x >> 6], kind, ntask, i, lab, m, nb, s, open, &pt);
+      const bool r =
+          qem_pool_choose<kExtras>(a, v, &pools[threadIdx.x >> 6], kind, ntask, i, lab, m, nb, s, open, &pt);
       if (kind != kPoolNone) {
         want = r;
         target = pt;
@@ -792,7 +800,7 @@ __device__ inline int block_sum(int v) {
   return tot;  // valid in thread 0
 }
 
-template <int METRIC, class Reg>
+template <int METRIC, bool kExtras, bool kCount, class Reg>
 __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemPool* pool, bool valid, int64_t e,
                                     int round, Reg&& reg, int* n_live, int* n_stale) {
   // the candidate record in one round trip
@@ -833,7 +841,7 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
   }
   if constexpr (METRIC == 1) {
     int32_t pt = kNull;
-    const bool r = qem_pool_choose(a, v, pool, kind, ntask, i, lab, m, nb, s, open, &pt);
+    const bool r = qem_pool_choose<kExtras>(a, v, pool, kind, ntask, i, lab, m, nb, s, open, &pt);
     if (use_pool) {
       live = r;
       target = pt;
@@ -844,8 +852,10 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
     if (lab != kNull) reg(lab, key);
     reg(target, key);
   }
-  *n_live += live;
-  *n_stale += stale;
+  if constexpr (kCount) {
+    *n_live += live;
+    *n_stale += stale;
+  }
 }
 
 // Re-evaluate (round r > 0): a live candidate (cand_tgt >= 0) whose own
@@ -853,7 +863,8 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
 // against the state left by round r-1: it registers its round-r key with its
 // (possibly new) target or is dropped (-3).  The others keep their move, whose
 // dE depends only on the two unchanged clusters, and register it again.
-template <int METRIC>
+// kCount: count the registered and re-evaluated candidates (timed runs).
+template <int METRIC, bool kExtras, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   __shared__ KeyAgg agg;
   __shared__ SegIndex si;
@@ -862,18 +873,18 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   const bool past = past_last_round(a, round);
   keyagg_clear(&agg);  // made visible by the barrier in round_was_empty
   if (round_was_empty(a, round - 1) || past) return;
-  const SweepView v = make_view(a);
+  const SweepView v = make_view<kExtras>(a);
   QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
   int n_live = 0, n_stale = 0;
   walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
-    reeval_entry<METRIC>(
+    reeval_entry<METRIC, kExtras, kCount>(
         a, v, pool, valid, e, round, [&](int32_t c, unsigned long long key) { keyagg_min(&agg, c, key); },
         &n_live, &n_stale);
     keyagg_flush(&agg, a.prio);
   });
   // candidates registered and re-evaluated in this round (statistics of the
   // timed runs: the block sums cost a few barriers per round)
-  if (a.count_rounds) {
+  if constexpr (kCount) {
     const int tl = block_sum(n_live);
     const int ts = block_sum(n_stale);
     if (threadIdx.x == 0) {
@@ -925,7 +936,7 @@ struct DeriveList {
 
 // Apply of candidate entry e (when valid) in round `round`; every lane of the
 // wave calls it.  Returns true when the candidate moved (item *moved_item).
-template <bool kSolve>
+template <bool kSolve, bool kExtras>
 __device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool valid, int64_t e, int round,
                                    int32_t* moved_item) {
   bool win = false;
@@ -984,8 +995,8 @@ __device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool vali
       a.cl_round[b] = rtag;
       if (a.item_mesh) atomicAdd(&st->pass_mods, 1);
       if (!kSolve) {
-        if (lab != kNull) derive_cluster(a, lab, dm, 0, false);
-        derive_cluster(a, b, dm, 0, false);
+        if (lab != kNull) derive_cluster<kExtras>(a, lab, dm, 0, false);
+        derive_cluster<kExtras>(a, b, dm, 0, false);
       }
     }
   }
@@ -1005,7 +1016,7 @@ __device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool vali
     for (int k = __lane_id(); k < total; k += 64) {
       const int64_t c = dl->cl[k];
       const int m = mesh_of_cluster(a, c);
-      derive_cluster(a, c, a.meshes[m], a.state[m].constrained, true);
+      derive_cluster<kExtras>(a, c, a.meshes[m], a.state[m].constrained, true);
     }
     wave_lds_sync();
   }
@@ -1013,7 +1024,7 @@ __device__ inline bool apply_entry(const SweepArgs& a, DeriveList* dl, bool vali
   return win;
 }
 
-template <bool kSolve>
+template <bool kSolve, bool kExtras>
 __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   __shared__ int32_t s_moved;
   __shared__ SegIndex si;
@@ -1029,7 +1040,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(SweepArgs a, int round) {
   int mods = 0;
   walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
     int32_t i;
-    const bool win = apply_entry<kSolve>(a, dl, valid, e, round, &i);
+    const bool win = apply_entry<kSolve, kExtras>(a, dl, valid, e, round, &i);
     mods += win;
     const int64_t em = seg_append(mseg, &s_moved, win);
     if (em >= 0) a.moved[em] = i;
@@ -1500,22 +1511,32 @@ hipError_t launch_scan(hipStream_t s, const SweepArgs& a, bool tiles) {
 // One evaluate block per candidate segment: block b writes segment b's count.
 // qem_unconstrained: no QEM mesh is constrained (the QEM meshes take the
 // isotropic rule, METRIC 2).
-hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained) {
-  if (any_iso) hipLaunchKernelGGL(k_eval<0>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, 0);
+// The kernels come in two builds: with the anchor and freeze tests (some mesh
+// has fixed or frozen clusters) and without them.
+template <bool kX>
+void launch_eval_x(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained) {
+  if (any_iso) hipLaunchKernelGGL((k_eval<0, kX>), dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, 0);
   if (any_qem && qem_unconstrained)
-    hipLaunchKernelGGL(k_eval<2>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
+    hipLaunchKernelGGL((k_eval<2, kX>), dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
   else if (any_qem)
-    hipLaunchKernelGGL(k_eval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
+    hipLaunchKernelGGL((k_eval<1, kX>), dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, any_iso ? 1 : 0);
+}
+
+hipError_t launch_eval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained) {
+  if (a.anchor || a.frozen) launch_eval_x<true>(s, a, any_iso, any_qem, qem_unconstrained);
+  else launch_eval_x<false>(s, a, any_iso, any_qem, qem_unconstrained);
   return hipGetLastError();
 }
 
 // One apply block per candidate segment (its moved segment has the same
 // index).  constrained: some mesh is constrained (derives may need a solve).
 hipError_t launch_apply(hipStream_t s, const SweepArgs& a, int round, bool constrained) {
-  if (constrained)
-    hipLaunchKernelGGL(k_apply<true>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-  else
-    hipLaunchKernelGGL(k_apply<false>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  const dim3 g(unsigned(a.n_seg)), b(kBlock);
+  const bool x = a.anchor || a.frozen;
+  if (constrained && x) hipLaunchKernelGGL((k_apply<true, true>), g, b, 0, s, a, round);
+  else if (constrained) hipLaunchKernelGGL((k_apply<true, false>), g, b, 0, s, a, round);
+  else if (x) hipLaunchKernelGGL((k_apply<false, true>), g, b, 0, s, a, round);
+  else hipLaunchKernelGGL((k_apply<false, false>), g, b, 0, s, a, round);
   return hipGetLastError();
 }
 
@@ -1524,13 +1545,22 @@ hipError_t launch_tile_update(hipStream_t s, const SweepArgs& a) {
   return hipGetLastError();
 }
 
+template <bool kX, bool kC>
+void launch_reeval_x(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained,
+                     int round) {
+  const dim3 g(unsigned(a.n_seg)), b(kBlock);
+  if (any_iso) hipLaunchKernelGGL((k_reeval<0, kX, kC>), g, b, 0, s, a, round);
+  if (any_qem && qem_unconstrained) hipLaunchKernelGGL((k_reeval<2, kX, kC>), g, b, 0, s, a, round);
+  else if (any_qem) hipLaunchKernelGGL((k_reeval<1, kX, kC>), g, b, 0, s, a, round);
+}
+
 hipError_t launch_reeval(hipStream_t s, const SweepArgs& a, bool any_iso, bool any_qem, bool qem_unconstrained,
                          int round) {
-  if (any_iso) hipLaunchKernelGGL(k_reeval<0>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-  if (any_qem && qem_unconstrained)
-    hipLaunchKernelGGL(k_reeval<2>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
-  else if (any_qem)
-    hipLaunchKernelGGL(k_reeval<1>, dim3(unsigned(a.n_seg)), dim3(kBlock), 0, s, a, round);
+  const bool x = a.anchor || a.frozen, c = a.count_rounds != 0;
+  if (x && c) launch_reeval_x<true, true>(s, a, any_iso, any_qem, qem_unconstrained, round);
+  else if (x) launch_reeval_x<true, false>(s, a, any_iso, any_qem, qem_unconstrained, round);
+  else if (c) launch_reeval_x<false, true>(s, a, any_iso, any_qem, qem_unconstrained, round);
+  else launch_reeval_x<false, false>(s, a, any_iso, any_qem, qem_unconstrained, round);
   return hipGetLastError();
 }
 
