@@ -153,6 +153,7 @@ struct acvd_ctx {
   double build_ms = 0.0, converge_ms = 0.0;
   Ctrl last_ctrl{};
   bool timing = false;
+  bool count_rounds = false;  // timing mode 1: the accept-round counters as well
   bool labels_seeded = false;  // device labels hold the initial sampling
   bool keep_frozen = false;    // the ForceManifold restarts keep the frozen flags
   double kernel_ms[4] = {0, 0, 0, 0};
@@ -187,7 +188,7 @@ struct acvd_ctx {
     a.geom = d_geom.p;
     a.anchor = any_fixed ? d_anchor.p : nullptr;
     a.frozen = any_frozen ? d_frozen.p : nullptr;
-    a.count_rounds = timing ? 1 : 0;
+    a.count_rounds = count_rounds ? 1 : 0;
     a.T = any_qem ? d_T.p : nullptr;
     a.evec = any_qem ? d_evec.p : nullptr;
     a.sizes = d_sizes.p;
@@ -1259,7 +1260,9 @@ int acvd_initial_sampling(acvd_ctx* ctx) {
 
 int acvd_set_timing(acvd_ctx* ctx, int enable) {
   if (int rc = check_ctx(ctx)) return rc;
+  if (enable < 0 || enable > 2) return fail(ACVD_ERR_ARG, "timing mode must be 0, 1 or 2");
   ctx->timing = enable != 0;
+  ctx->count_rounds = enable == 1;
   return ACVD_OK;
 }
 

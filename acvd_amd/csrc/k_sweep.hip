@@ -821,8 +821,9 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
     m = mesh_of_item(a, i);
     const DevMesh& dm = a.meshes[m];
     const MeshState& st = a.state[m];
-    // a mesh past its last round of the pass registers nothing (no move)
-    if (dm.metric == mesh_metric(METRIC) && round < pass_rounds(st)) {
+    // a mesh past its last round of the pass registers nothing (no move); a
+    // context of one mesh has left the kernel already (past_last_round)
+    if (dm.metric == mesh_metric(METRIC) && (!a.item_mesh || round < pass_rounds(st))) {
       const int32_t ra = lab != kNull ? a.cl_round[lab] : -1, rb = a.cl_round[b];
       stale = ra >= since || rb >= since;
       key = accept_key(dm.seed, st.loops, round, uint32_t(i - dm.item_off));

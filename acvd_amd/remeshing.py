@@ -141,8 +141,10 @@ class Context:
         """Device initial clustering of every mesh without user labels."""
         check(N.lib().acvd_initial_sampling(self._h))
 
-    def set_timing(self, enable):
-        check(N.lib().acvd_set_timing(self._h, 1 if enable else 0))
+    def set_timing(self, enable, counters=True):
+        """Per-kernel HIP-event timing of the next runs; counters: also count
+        the accept-round work (the counting kernels run slightly slower)."""
+        check(N.lib().acvd_set_timing(self._h, (1 if counters else 2) if enable else 0))
 
     def process(self):
         check(N.lib().acvd_process_clustering(self._h))

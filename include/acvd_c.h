@@ -207,7 +207,9 @@ int acvd_initial_sampling(acvd_ctx* ctx);
 /* Per-kernel timing (no counterpart in the reference): when enabled, passes
  * are launched one by one with HIP events around each sweep kernel (instead
  * of replaying the captured chunk graph) and the times are reported in
- * acvd_stats.kernel_ms / kernel_launches. */
+ * acvd_stats.kernel_ms / kernel_launches.  enable: 0 off; 1 events and the
+ * accept-round counters (acvd_stats.ctx_reeval_live / ctx_reevaluated; the
+ * counting kernels are a few per cent slower); 2 events only. */
 int acvd_set_timing(acvd_ctx* ctx, int enable);
 
 /* ---- results (copies into caller buffers) --------------------------------- */
