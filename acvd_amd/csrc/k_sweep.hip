@@ -200,16 +200,49 @@ __device__ inline bool needs_eval_fast(const SweepArgs& a, int64_t i, int32_t la
   return boundary && dirty;
 }
 
+// Scan while the tile sets are off (nearly every tile holds a changed
+// cluster): every item in id order, so that the item words and CSR rows are
+// read in order.  Block b takes items [b * kBlock, (b + 1) * kBlock) and
+// appends to work slice b % kWorkSlices (at most one tile's worth per block,
+// within the slice capacity).  The evaluate's results do not depend on the
+// order of the work list.  C3 torus: 139 -> 127 us per full scan.
+__global__ __launch_bounds__(kBlock) void k_scan_linear(SweepArgs a) {
+  __shared__ int32_t s_cnt[kBlock / 64], s_base;
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  bool want = false;
+  if (i < a.n_items) {
+    const MeshState& st = a.state[mesh_of_item(a, i)];
+    if (st.phase == kPhaseSweep) want = needs_eval_fast(a, i, a.labels[i], st.loops);
+  }
+  const unsigned long long bal = __ballot(want);
+  if (lane == 0) s_cnt[w] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      const int c = s_cnt[k];
+      s_cnt[k] = tot;
+      tot += c;
+    }
+    s_base = tot ? atomicAdd(&a.work_count[blockIdx.x % kWorkSlices], tot) : 0;
+  }
+  __syncthreads();
+  if (want) {
+    const int32_t pos = s_base + s_cnt[w] + __popcll(bal & ((1ull << lane) - 1ull));
+    a.work[int64_t(blockIdx.x % kWorkSlices) * a.work_cap + pos] = int32_t(i);
+  }
+}
+
 // Scan: items of sweeping meshes that must be evaluated this pass are
 // compacted into the work list (slice t % kWorkSlices of tile t).  One
 // wavefront per tile: the lanes first test the tile's cluster set against the
 // cluster stamps (and the tile's last proposal loop); only an active tile
 // reads its items and CSR rows.  The test is conservative: every item
 // needs_eval would select lies in an active tile.
-// kTiles: the tile sets are kept (else every tile is marked always scanned
-// and the kernel does not read the tile words).
+// It runs while the tile sets are kept; before that k_scan_linear scans every
+// item.
 static_assert(kTileSet == 64, "k_scan: one set entry per lane");
-template <bool kTiles>
 __global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
   // one wavefront per tile, kTilesPerScanBlock tiles per block (the waves
   // are independent: no block-level synchronisation)
@@ -219,9 +252,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(SweepArgs a) {
   const int m = a.item_mesh ? a.tile_mesh[t] : 0;
   // the tile's count, last proposal and set entry of this lane (the set holds
   // kTileSet = 64 entries; entries past cnt are ignored) read with the state
-  const int32_t cnt = kTiles ? a.tile_cnt[t] : kTileSet + 1;
-  const int32_t prop = kTiles ? a.tile_prop[t] : 0;
-  const int32_t c_lane = kTiles ? a.tile_set[int64_t(t) * kTileSet + lane] : 0;
+  const int32_t cnt = a.tile_cnt[t];
+  const int32_t prop = a.tile_prop[t];
+  const int32_t c_lane = a.tile_set[int64_t(t) * kTileSet + lane];
   const MeshState& st = a.state[m];
   if (st.phase != kPhaseSweep) return;
   const int32_t L = st.loops;
@@ -1504,8 +1537,11 @@ hipError_t launch_tile_sets(hipStream_t s, const SweepArgs& a, const int32_t* me
 hipError_t launch_scan(hipStream_t s, const SweepArgs& a, bool tiles) {
   if (a.n_items == 0) return hipSuccess;
   const dim3 grid(unsigned((a.n_tiles + kTilesPerScanBlock - 1) / kTilesPerScanBlock));
-  if (tiles) hipLaunchKernelGGL(k_scan<true>, grid, dim3(kBlock), 0, s, a);
-  else hipLaunchKernelGGL(k_scan<false>, grid, dim3(kBlock), 0, s, a);
+  // blocks of kBlock items never outnumber the scan tiles (each mesh's tiles
+  // cover its items), so the linear scan stays within the slice capacity
+  static_assert(kBlock == kTileItems, "one scan block per tile's worth of items");
+  if (tiles) hipLaunchKernelGGL(k_scan, grid, dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL(k_scan_linear, dim3(unsigned((a.n_items + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
