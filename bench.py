@@ -232,6 +232,73 @@ def run_gpu_leg(args):
     }
 
 
+def start_small_gaps():
+    """C1 (icosphere level 6, 40,962 vertices, isotropic, K = 2,000) and C2
+    (cube 400 x 400 per face, 960,002 vertices, QEM, K = 20,000): the serial
+    restatement (oracle A) of each starts in a host thread from a fully
+    labelled initial clustering; run_small_gaps runs the GPU on the same
+    labels and reports the final-energy gaps (the north star's <= 0.1 %)."""
+    import threading
+    from acvd_amd import meshgen
+    from oracle import pyoracle as po
+    legs = {}
+    for name in ("C1", "C2"):
+        if name == "C1":
+            xyz, tri = meshgen.icosphere(6, 1, 0.02)
+            metric, K, unc = po.ISO, 2000, 0
+            m = po.RefMesh(xyz, tri)
+            init = m.fill_holes(K, meshgen.seed_labels(len(xyz), K, 1))
+        else:
+            xyz, tri = meshgen.cube(400, 2, 1e-4)
+            metric, K, unc = po.QEM, 20000, 1
+            m = po.RefMesh(xyz, tri)
+            init = m.fill_holes(K, m.initial_sampling(metric, K))
+        box = {"xyz": xyz, "tri": tri, "metric": metric, "K": K, "unc": unc, "m": m, "init": init}
+
+        def work(box=box):
+            try:
+                t = time.time()
+                box["labels"], _ = box["m"].serial_run(box["metric"], box["K"], unconstrained=box["unc"],
+                                                       init=box["init"])
+                box["serial_s"] = time.time() - t
+            except Exception as e:  # reported by the caller
+                box["error"] = e
+
+        th = threading.Thread(target=work, daemon=True)
+        th.start()
+        box["thread"] = th
+        legs[name] = box
+    return legs
+
+
+def run_small_gaps(legs):
+    import acvd_amd
+    out = {}
+    for name, b in legs.items():
+        ctx = acvd_amd.Context(0)
+        ctx.set_input(b["xyz"], b["tri"])
+        ctx.set_params(0, number_of_clusters=b["K"], metric=b["metric"], unconstrained_initialization=b["unc"],
+                       seed=0)
+        ctx.set_initial_labels(0, b["init"])
+        ctx.process()
+        ctx.process()
+        gpu_ms = ctx.stats(0)["converge_ms"]
+        lab = ctx.labels(0)
+        ctx.close()
+        b["thread"].join()
+        if "error" in b:
+            raise b["error"]
+        m, K, metric = b["m"], b["K"], b["metric"]
+        eg = m.energy(metric, K, lab)[0]
+        es = m.energy(metric, K, b["labels"])[0]
+        gap = (eg - es) / es * 100
+        out[name] = {"items": len(b["xyz"]), "clusters": K, "metric": "QEM" if metric else "isotropic",
+                     "gpu_ms": round(gpu_ms, 2), "serial_s": round(b["serial_s"], 1),
+                     "energy_gap_vs_serial_pct": round(gap, 4), "within_0.1pct": bool(gap <= 0.1)}
+        log(f"{name}: GPU {gpu_ms:.2f} ms, serial restatement {b['serial_s']:.1f} s, energy gap {gap:+.4f}%")
+    return out
+
+
 def run_cpu_baseline(args, g):
     """ACVDQP restatement (oracle B, vtkThreadedClustering) on all host cores,
     from the GPU's initial labels, on the full workload, after the serial
@@ -478,6 +545,7 @@ def main():
     if args.copy_calibration:
         print(json.dumps(run_copy_calibration(args)), flush=True)
         return
+    small = None if args.skip_cpu else start_small_gaps()
     g = run_gpu_leg(args)
     out = {
         "metric": "10M->100k ACVDQ converge ms; batched vs serial meshes/s at <=0.1% energy gap",
@@ -507,6 +575,8 @@ def main():
         out["batched"] = run_batch_leg(args)
     if not args.no_energy:
         out["energy"] = run_energy_leg(args)
+    if small is not None:
+        out["energy_gap_small"] = run_small_gaps(small)
     if not args.skip_cpu:
         out["cpu_baseline"] = run_cpu_baseline(args, g)
     print(json.dumps(out), flush=True)
