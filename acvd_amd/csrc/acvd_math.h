@@ -933,11 +933,21 @@ ACVD_HD int qem_mode(const MeshState& s) { return s.constrained ? 1 : (s.anchore
 //   rounds (173 passes), +0.02 / -0.03 % with 1 (~700 passes);
 //   on the GPU the 8,388,608-vertex torus (K = 100,000) takes 384 passes and
 //   239 ms with 12 rounds, 1,556 passes and 403 ms with 1.
-// So the default is one round for meshes of at most kFirstRoundsAutoItems
-// items, where the distinct local minima differ most and a pass costs little,
-// and twelve above, where the gap is within noise and the passes dominate.
-constexpr int32_t kFirstRoundsAutoItems = int32_t(1) << 20;
-ACVD_HD int32_t auto_first_rounds(int64_t n_items) { return n_items <= kFirstRoundsAutoItems ? 1 : kMatchRounds; }
+//   star-shaped icospheres, K = V/15, isotropic, two meshes x four accept
+//   seeds per size (gap median, max; mean passes) with 1 / 2 / 12 rounds:
+//   10,242 vertices  -0.30 % +0.13 54 / -0.08 % +0.39 33 / +0.01 % +0.40 24;
+//   40,962           -0.04 % +0.04 60 / -0.02 % +0.07 39 / +0.10 % +0.18 29;
+//   163,842          -0.04 % -0.01 62 / +0.02 % +0.06 43 / +0.03 % +0.09 34;
+//   655,362          -0.07 % -0.05 68 / -0.03 % +0.00 49 / +0.02 % +0.02 39.
+// So the default is one round for meshes of at most 2^17 items, where the
+// distinct local minima differ most and a pass costs little, two up to 2^19
+// (within 0.1 % at 30 % fewer passes) and twelve above, where the gap is
+// within noise and the passes dominate.
+constexpr int32_t kFirstRoundsOneItems = int32_t(1) << 17;
+constexpr int32_t kFirstRoundsTwoItems = int32_t(1) << 19;
+ACVD_HD int32_t auto_first_rounds(int64_t n_items) {
+  return n_items <= kFirstRoundsOneItems ? 1 : (n_items <= kFirstRoundsTwoItems ? 2 : kMatchRounds);
+}
 
 ACVD_HD int32_t pass_rounds(const MeshState& s) { return s.convergences == 0 ? s.first_rounds : kMatchRounds; }
 

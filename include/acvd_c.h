@@ -63,8 +63,9 @@ typedef struct acvd_ctx acvd_ctx;
  *                                  on the device and weights the items with it; Remesh() of the class API
  *                                  sets it, as the reference's Remesh runs SamplingPreProcessing
  *   first_phase_rounds          <- (no counterpart) accept rounds per pass until the first convergence,
- *                                  1..12; 0 = the library's rule: 1 for meshes of at most 2^20 vertices
- *                                  (lower final energy), 12 above (fewer passes); later phases run 12
+ *                                  1..12; 0 = the library's rule: 1 for meshes of at most 2^17 vertices
+ *                                  (lower final energy), 2 up to 2^19, 12 above (fewer passes); later
+ *                                  phases run 12
  *   write_energy_log            <- SetWriteToGlobalEnergyLog (vtkUniformClustering.h:123): nonzero = record the
  *                                  energy after every loop and after the run (acvd_get_energy_log)
  */
