@@ -88,7 +88,8 @@ EXPORTS = [
     "acvd_get_scales", "acvd_build", "acvd_initial_sampling", "acvd_set_timing", "acvd_build_output",
     "acvd_get_output_sizes", "acvd_get_output", "acvd_energy_create", "acvd_energy_destroy", "acvd_energy_add_mesh",
     "acvd_energy_num_items", "acvd_energy_forward", "acvd_energy_backward", "acvd_reference_initial_sampling",
-    "acvd_set_fixed_clusters", "acvd_subdivide", "acvd_get_energy_log", "acvd_force_manifold",
+    "acvd_set_fixed_clusters", "acvd_subdivide", "acvd_subdivide_input", "acvd_get_input_sizes", "acvd_get_input",
+    "acvd_get_energy_log", "acvd_force_manifold",
     "acvd_set_frozen_clusters", "acvd_vertex_manifold", "acvd_force_manifold_step", "acvd_fix_mesh_boundaries",
     "acvd_mesh_boundary_fix", "acvd_reference_seed_sampling", "acvd_set_tuning", "acvd_get_tuning", "acvd_autotune",
     "acvd_energy_forward_cvt_noalloc",
@@ -156,6 +157,9 @@ def lib():
     L.acvd_energy_backward.argtypes = [P, P, P, P, P, P, P, P]
     L.acvd_set_fixed_clusters.argtypes = [P, i32, i64, P]
     L.acvd_subdivide.argtypes = [i64, P, i64, P, ctypes.POINTER(i64), ctypes.POINTER(i64), P, P, P]
+    L.acvd_subdivide_input.argtypes = [P, i32, i64, ctypes.POINTER(i32)]
+    L.acvd_get_input_sizes.argtypes = [P, i32, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.acvd_get_input.argtypes = [P, i32, P, P]
     L.acvd_get_energy_log.argtypes = [P, i32, i64, ctypes.POINTER(i64), P, P, P]
     L.acvd_force_manifold.argtypes = [P, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i64)]
     L.acvd_set_frozen_clusters.argtypes = [P, i32, i64, P]

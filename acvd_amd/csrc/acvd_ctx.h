@@ -180,6 +180,18 @@ hipError_t launch_seg_reduce(hipStream_t s, int n_blocks, const int2* block_tab,
                              const double* raw_w, int what, double* partial);
 hipError_t launch_clamp_and_scales(hipStream_t s, const SweepArgs& a, int n_blocks, const int2* block_tab,
                                    const double* partial, int stage, const double* raw_w, DevMesh* meshes_rw);
+// subdivision (one level: keys, sort, run heads, edge numbers and midpoints, faces)
+hipError_t launch_sub_keys(hipStream_t s, int64_t nf, int64_t nv, const int32_t* tri, unsigned long long* keys,
+                           int32_t* vals, int32_t* face_edge, int32_t* active);
+hipError_t sort_sub_keys(hipStream_t s, int64_t n3, int64_t nv, unsigned long long* k0, unsigned long long* k1,
+                         int32_t* v0, int32_t* v1, void** tmp, size_t* tmp_bytes);
+hipError_t launch_sub_heads(hipStream_t s, int64_t n3, int64_t nv, const unsigned long long* keys,
+                            const int32_t* vals, int32_t* flag);
+hipError_t launch_sub_edges(hipStream_t s, int64_t n3, int64_t nv, const unsigned long long* keys,
+                            const int32_t* vals, const int32_t* num, const int32_t* tri, const double* xyz,
+                            double* xyz_out, const double* ind, double* ind_out, int32_t* face_edge);
+hipError_t launch_sub_faces(hipStream_t s, int64_t nf, int64_t nv, const int32_t* tri, const int32_t* face_edge,
+                            const int32_t* fpos, int32_t* tri_out);
 // sweep
 hipError_t launch_zero_clusters(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask);
 hipError_t launch_accumulate(hipStream_t s, const SweepArgs& a, const int32_t* mesh_mask,

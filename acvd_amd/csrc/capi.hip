@@ -1142,6 +1142,115 @@ int acvd_subdivide(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri
   return ACVD_OK;
 }
 
+int acvd_subdivide_input(acvd_ctx* ctx, int32_t mesh_id, int64_t min_vertices, int32_t* levels) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  HostMesh& h = ctx->meshes[mesh_id];
+  if (levels) *levels = 0;
+  if (h.nv >= min_vertices) return ACVD_OK;
+  if (!h.init_labels.empty())
+    return fail(ACVD_ERR_STATE, "initial labels are set for the input's vertices; the input needs subdivision");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const bool with_ind = !h.indicator.empty();
+  int64_t nv = h.nv, nf = h.nf;
+  DBuf<double> xyz, ind;
+  DBuf<int32_t> tri;
+  HIP_TRY(xyz.alloc(size_t(3 * nv)));
+  HIP_TRY(tri.alloc(size_t(3 * nf)));
+  HIP_TRY(hipMemcpyAsync(xyz.p, h.xyz.data(), sizeof(double) * 3 * nv, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(tri.p, h.tri.data(), sizeof(int32_t) * 3 * nf, hipMemcpyHostToDevice, s));
+  if (with_ind) {
+    HIP_TRY(ind.alloc(size_t(nv)));
+    HIP_TRY(hipMemcpyAsync(ind.p, h.indicator.data(), sizeof(double) * nv, hipMemcpyHostToDevice, s));
+  }
+  int32_t done = 0;
+  while (nv < min_vertices) {
+    const int64_t n3 = 3 * nf;
+    if (n3 + 1 >= int64_t(INT32_MAX)) return fail(ACVD_ERR_LIMIT, "subdivision: more than 2^31 half-edges");
+    DBuf<unsigned long long> k0, k1;
+    DBuf<int32_t> v0, v1, flag, num, face_edge, active, fpos;
+    HIP_TRY(k0.alloc(size_t(n3)));
+    HIP_TRY(k1.alloc(size_t(n3)));
+    HIP_TRY(v0.alloc(size_t(n3)));
+    HIP_TRY(v1.alloc(size_t(n3)));
+    HIP_TRY(flag.alloc(size_t(n3 + 1)));
+    HIP_TRY(num.alloc(size_t(n3 + 1)));
+    HIP_TRY(face_edge.alloc(size_t(n3)));
+    HIP_TRY(active.alloc(size_t(nf + 1)));
+    HIP_TRY(fpos.alloc(size_t(nf + 1)));
+    HIP_TRY(hipMemsetAsync(flag.p + n3, 0, sizeof(int32_t), s));
+    HIP_TRY(hipMemsetAsync(active.p + nf, 0, sizeof(int32_t), s));
+    HIP_TRY(launch_sub_keys(s, nf, nv, tri.p, k0.p, v0.p, face_edge.p, active.p));
+    HIP_TRY(sort_sub_keys(s, n3, nv, k0.p, k1.p, v0.p, v1.p, &ctx->cub_tmp, &ctx->cub_bytes));
+    HIP_TRY(launch_sub_heads(s, n3, nv, k1.p, v1.p, flag.p));
+    HIP_TRY(exclusive_scan_i32(s, flag.p, num.p, n3 + 1, &ctx->cub_tmp, &ctx->cub_bytes));
+    HIP_TRY(exclusive_scan_i32(s, active.p, fpos.p, nf + 1, &ctx->cub_tmp, &ctx->cub_bytes));
+    int32_t ne = 0, nact = 0;
+    HIP_TRY(hipMemcpyAsync(&ne, num.p + n3, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&nact, fpos.p + nf, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nv + ne >= kMaxItemsPerMesh) return fail(ACVD_ERR_LIMIT, "subdivided mesh has more than 2^27 vertices");
+    if (nact == 0) return fail(ACVD_ERR_ARG, "subdivision: no active face");
+    const int64_t nv2 = nv + ne, nf2 = 4 * int64_t(nact);
+    DBuf<double> xyz2, ind2;
+    DBuf<int32_t> tri2;
+    HIP_TRY(xyz2.alloc(size_t(3 * nv2)));
+    HIP_TRY(tri2.alloc(size_t(3 * nf2)));
+    HIP_TRY(hipMemcpyAsync(xyz2.p, xyz.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToDevice, s));
+    if (with_ind) {
+      HIP_TRY(ind2.alloc(size_t(nv2)));
+      HIP_TRY(hipMemcpyAsync(ind2.p, ind.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(launch_sub_edges(s, n3, nv, k1.p, v1.p, num.p, tri.p, xyz.p, xyz2.p, with_ind ? ind.p : nullptr,
+                             with_ind ? ind2.p : nullptr, face_edge.p));
+    HIP_TRY(launch_sub_faces(s, nf, nv, tri.p, face_edge.p, fpos.p, tri2.p));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::swap(xyz.p, xyz2.p);
+    std::swap(xyz.n, xyz2.n);
+    std::swap(tri.p, tri2.p);
+    std::swap(tri.n, tri2.n);
+    std::swap(ind.p, ind2.p);
+    std::swap(ind.n, ind2.n);
+    nv = nv2;
+    nf = nf2;
+    ++done;
+  }
+  h.xyz.resize(size_t(3 * nv));
+  h.tri.resize(size_t(3 * nf));
+  HIP_TRY(hipMemcpyAsync(h.xyz.data(), xyz.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h.tri.data(), tri.p, sizeof(int32_t) * 3 * nf, hipMemcpyDeviceToHost, s));
+  if (with_ind) {
+    h.indicator.resize(size_t(nv));
+    HIP_TRY(hipMemcpyAsync(h.indicator.data(), ind.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  h.nv = nv;
+  h.nf = nf;
+  h.curvature.clear();
+  h.out_xyz.clear();
+  h.out_tri.clear();
+  h.clustered = false;
+  h.logged = false;
+  ctx->invalidate();
+  if (levels) *levels = done;
+  return ACVD_OK;
+}
+
+int acvd_get_input_sizes(const acvd_ctx* ctx, int32_t mesh_id, int64_t* nv, int64_t* nf) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  if (nv) *nv = ctx->meshes[mesh_id].nv;
+  if (nf) *nf = ctx->meshes[mesh_id].nf;
+  return ACVD_OK;
+}
+
+int acvd_get_input(const acvd_ctx* ctx, int32_t mesh_id, double* xyz, int32_t* tri) {
+  if (int rc = check_mesh(ctx, mesh_id)) return rc;
+  const HostMesh& h = ctx->meshes[mesh_id];
+  if (xyz) std::copy(h.xyz.begin(), h.xyz.end(), xyz);
+  if (tri) std::copy(h.tri.begin(), h.tri.end(), tri);
+  return ACVD_OK;
+}
+
 int acvd_set_fixed_clusters(acvd_ctx* ctx, int32_t mesh_id, int64_t n, const int32_t* items) {
   if (int rc = check_mesh(ctx, mesh_id)) return rc;
   if (n < 0 || (n > 0 && !items)) return fail(ACVD_ERR_ARG, "null fixed-item list");

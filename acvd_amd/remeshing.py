@@ -134,6 +134,28 @@ class Context:
         check(N.lib().acvd_get_indicator(self._h, mesh_id, _p(out)))
         return out
 
+    def subdivide_input(self, mesh_id, min_vertices):
+        """Midpoint subdivision of the input on the device until it has at
+        least min_vertices vertices (CheckSubsamplingRatio,
+        vtkDiscreteRemeshing.h:840-875); a set indicator follows, each
+        midpoint taking the mean of its ends.  Returns the number of levels."""
+        self._n(mesh_id)
+        lv = ctypes.c_int32()
+        check(N.lib().acvd_subdivide_input(self._h, mesh_id, int(min_vertices), ctypes.byref(lv)))
+        nv = ctypes.c_int64()
+        check(N.lib().acvd_get_input_sizes(self._h, mesh_id, ctypes.byref(nv), None))
+        self._nv[mesh_id] = nv.value
+        return lv.value
+
+    def input(self, mesh_id=0):
+        """The input mesh as the context holds it (points, triangles)."""
+        nv, nf = ctypes.c_int64(), ctypes.c_int64()
+        check(N.lib().acvd_get_input_sizes(self._h, mesh_id, ctypes.byref(nv), ctypes.byref(nf)))
+        xyz = np.empty((nv.value, 3), np.float64)
+        tri = np.empty((nf.value, 3), np.int32)
+        check(N.lib().acvd_get_input(self._h, mesh_id, _p(xyz), _p(tri)))
+        return xyz, tri
+
     def build(self):
         check(N.lib().acvd_build(self._h))
 
@@ -564,33 +586,33 @@ class _DiscreteRemeshing:
         if K <= 0:
             return
         original = self._input
-        mesh = original
-        chain = []
-        while mesh.GetNumberOfPoints() < self._subsampling_threshold * K:
-            if self._params["console_output"]:
-                print("Subdividing mesh")
-            x, t, par = subdivide(mesh.points, mesh.triangles)
-            mesh = Mesh(x, t)
-            chain.append(par)
         curvature = 1
-        if chain:
+        min_vertices = int(np.ceil(self._subsampling_threshold * K))
+        if original.GetNumberOfPoints() < min_vertices:
             if self._initial is not None:
                 raise AcvdError("SetInitialClustering labels the input's vertices; the input needs subdivision")
+            # subdivision on the device; the indicator (set, or the curvature
+            # indicator of the original input) follows to the midpoints
+            c = Context()
+            c.set_input(original.points, original.triangles)
             ind = self._indicator
             if ind is None and self._params["gradation"] > 0:
-                c = Context()
-                c.set_input(original.points, original.triangles)
                 c.set_params(0, number_of_clusters=1, metric=self.METRIC, gradation=self._params["gradation"],
                              curvature_indicator=1)
                 c.build()
                 ind = c.indicator(0)
-                c.close()
             if ind is not None:
-                ind = np.asarray(ind, np.float64)
-                for par in chain:
-                    ind = np.concatenate([ind, 0.5 * (ind[par[:, 0]] + ind[par[:, 1]])])
+                c.set_indicator(0, ind)
+            levels = c.subdivide_input(0, min_vertices)
+            if self._params["console_output"]:
+                for _ in range(levels):
+                    print("Subdividing mesh")
+            x, t = c.input(0)
+            if ind is not None:
+                ind = c.indicator(0)
+            c.close()
             self._original_input = original
-            self._input = mesh
+            self._input = Mesh(x, t)
             self._indicator = ind
             curvature = 0
         if self.ProcessClustering(curvature_indicator=curvature) is None:

@@ -524,11 +524,44 @@ def run_copy_calibration(args):
     return {"bytes_per_copy": 4 * n, "copies": 3}
 
 
+def run_subdivision(args):
+    """Midpoint subdivision of Remesh() on a coarse input: the device pass
+    (Context.subdivide_input: upload, kernels, download into the context's
+    host copy) against the host pass (acvd_subdivide) on the 1024x1024 torus,
+    1,048,576 -> 4,194,304 vertices; both must give the same mesh."""
+    import acvd_amd
+    from acvd_amd import meshgen
+    from acvd_amd.remeshing import subdivide
+    x, t = meshgen.torus(1024, 1024, seed=1)
+    x = np.asarray(x, np.float64)
+    ind = np.random.default_rng(1).random(len(x))
+    dev = []
+    for _ in range(3):
+        ctx = acvd_amd.Context()
+        ctx.set_input(x, t)
+        ctx.set_indicator(0, ind)
+        t0 = time.perf_counter()
+        levels = ctx.subdivide_input(0, 4000000)
+        dev.append((time.perf_counter() - t0) * 1e3)
+        ox, ot = ctx.input(0)
+        ctx.close()
+        log(f"device subdivision {len(x)} -> {len(ox)} vertices, {len(ot)} faces, {levels} level: {dev[-1]:.1f} ms")
+    t0 = time.perf_counter()
+    hx, ht, _ = subdivide(x, t)
+    host = (time.perf_counter() - t0) * 1e3
+    log(f"host subdivision (sizes, then data): {host:.1f} ms")
+    return {"workload": "torus 1024x1024 -> one midpoint subdivision level", "vertices_in": len(x),
+            "vertices_out": len(ox), "faces_out": len(ot), "device_ms": [round(v, 1) for v in dev],
+            "host_ms": round(host, 1), "identical": bool(np.array_equal(hx, ox) and np.array_equal(ht, ot))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM byte counters)")
+    ap.add_argument("--subdivision", action="store_true",
+                    help="only time the device subdivision of a 1 M-vertex torus against the host pass")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU legs (serial gap and ACVDQP baseline)")
@@ -544,6 +577,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         print(json.dumps(run_copy_calibration(args)), flush=True)
+        return
+    if args.subdivision:
+        print(json.dumps(run_subdivision(args)), flush=True)
         return
     small = None if args.skip_cpu else start_small_gaps()
     g = run_gpu_leg(args)

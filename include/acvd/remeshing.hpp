@@ -149,50 +149,25 @@ class DiscreteRemeshing {
   // failure.
   // CheckSubsamplingRatio (vtkDiscreteRemeshing.h:840-875): while the input
   // has fewer than SubsamplingThreshold * K vertices it is replaced by its
-  // midpoint subdivision (acvd_subdivide), which GetInput() then returns; the
-  // curvature indicator is computed on the original input and interpolated
-  // to the midpoints as the mean of their parents (:700-745).
+  // midpoint subdivision (acvd_subdivide_input, on the device), which
+  // GetInput() then returns; the curvature indicator is computed on the
+  // original input and interpolated to the midpoints as the mean of their
+  // parents (:700-745).
   int Remesh() {
     if (!input_) return fail("no input"), 0;
-    const Mesh* mesh = input_;
-    std::vector<std::vector<int32_t>> chain;
-    while (mesh->GetNumberOfPoints() < int64_t(subsampling_threshold_) * params_.number_of_clusters) {
-      if (params_.console_output) std::printf("Subdividing mesh\n");
-      Mesh next;
-      std::vector<int32_t> parents;
-      if (!subdivide(*mesh, &next, &parents)) return 0;
-      subdivided_ = std::move(next);
-      mesh = &subdivided_;
-      chain.push_back(std::move(parents));
-    }
-    if (chain.empty()) {
+    const int64_t min_vertices = int64_t(subsampling_threshold_) * params_.number_of_clusters;
+    if (input_->GetNumberOfPoints() >= min_vertices) {
       if (!ProcessClustering(1)) return 0;
       return Finish();
     }
     if (initial_) return fail("the initial clustering labels the input's vertices; the input needs subdivision"), 0;
-    std::vector<double> ind;
-    if (indicator_) {
-      ind = *indicator_;
-    } else if (params_.gradation > 0) {  // curvature of the original input
-      acvd_ctx* c = nullptr;
-      if (!ok(acvd_create(&c, device_))) return 0;
-      acvd_params p = params_;
-      p.metric = METRIC;
-      p.number_of_clusters = 1;
-      p.curvature_indicator = 1;
-      ind.resize(size_t(input_->GetNumberOfPoints()));
-      const bool good = ok(acvd_set_input_f64(c, input_->GetNumberOfPoints(), input_->points.data(),
-                                              input_->GetNumberOfCells(), input_->triangles.data())) &&
-                        ok(acvd_set_params(c, 0, &p)) && ok(acvd_build(c)) && ok(acvd_get_indicator(c, 0, ind.data()));
-      acvd_destroy(c);
-      if (!good) return 0;
-    }
-    if (!ind.empty())
-      for (const auto& par : chain)
-        for (size_t k = 0; k < par.size() / 2; ++k) ind.push_back(0.5 * (ind[size_t(par[2 * k])] + ind[size_t(par[2 * k + 1])]));
+    acvd_ctx* c = nullptr;
+    if (!ok(acvd_create(&c, device_))) return 0;
+    const bool good = subdivide(c, min_vertices);
+    acvd_destroy(c);
+    if (!good) return 0;
     original_ = input_;
     input_ = &subdivided_;
-    interpolated_ = std::move(ind);
     if (!interpolated_.empty()) indicator_ = &interpolated_;
     if (!ProcessClustering(0)) return 0;
     return Finish();
@@ -241,16 +216,40 @@ class DiscreteRemeshing {
   DiscreteRemeshing& operator=(const DiscreteRemeshing&) = delete;
 
  private:
-  bool subdivide(const Mesh& in, Mesh* out, std::vector<int32_t>* parents) {
-    int64_t nv = 0, nf = 0;
-    if (!ok(acvd_subdivide(in.GetNumberOfPoints(), in.points.data(), in.GetNumberOfCells(), in.triangles.data(), &nv,
-                           &nf, nullptr, nullptr, nullptr)))
+  // the input (and the indicator it is weighted with) subdivided in context c
+  bool subdivide(acvd_ctx* c, int64_t min_vertices) {
+    const Mesh& in = *input_;
+    if (!ok(acvd_set_input_f64(c, in.GetNumberOfPoints(), in.points.data(), in.GetNumberOfCells(),
+                               in.triangles.data())))
       return false;
-    out->points.resize(size_t(3 * nv));
-    out->triangles.resize(size_t(3 * nf));
-    parents->resize(size_t(2 * (nv - in.GetNumberOfPoints())));
-    return ok(acvd_subdivide(in.GetNumberOfPoints(), in.points.data(), in.GetNumberOfCells(), in.triangles.data(), &nv,
-                             &nf, out->points.data(), out->triangles.data(), parents->data()));
+    std::vector<double> ind;
+    if (indicator_) {
+      ind = *indicator_;
+    } else if (params_.gradation > 0) {  // curvature of the original input
+      acvd_params p = params_;
+      p.metric = METRIC;
+      p.number_of_clusters = 1;
+      p.curvature_indicator = 1;
+      ind.resize(size_t(in.GetNumberOfPoints()));
+      if (!ok(acvd_set_params(c, 0, &p)) || !ok(acvd_build(c)) || !ok(acvd_get_indicator(c, 0, ind.data())))
+        return false;
+    }
+    if (!ind.empty() && !ok(acvd_set_indicator(c, 0, ind.data()))) return false;
+    int32_t levels = 0;
+    if (!ok(acvd_subdivide_input(c, 0, min_vertices, &levels))) return false;
+    if (params_.console_output)
+      for (int32_t k = 0; k < levels; ++k) std::printf("Subdividing mesh\n");
+    int64_t nv = 0, nf = 0;
+    if (!ok(acvd_get_input_sizes(c, 0, &nv, &nf))) return false;
+    subdivided_.points.resize(size_t(3 * nv));
+    subdivided_.triangles.resize(size_t(3 * nf));
+    if (!ok(acvd_get_input(c, 0, subdivided_.points.data(), subdivided_.triangles.data()))) return false;
+    interpolated_.clear();
+    if (!ind.empty()) {
+      interpolated_.resize(size_t(nv));
+      if (!ok(acvd_get_indicator(c, 0, interpolated_.data()))) return false;
+    }
+    return true;
   }
   // The file of WriteToGlobalEnergyLog: loop, seconds (6 significant
   // digits), energy (15); a file that cannot be opened is skipped, as the

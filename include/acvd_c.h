@@ -265,6 +265,24 @@ int acvd_reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, c
 int acvd_subdivide(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int64_t* out_nv, int64_t* out_nf,
                    double* out_xyz, int32_t* out_tri, int32_t* parents);
 
+/* CheckSubsamplingRatio (vtkDiscreteRemeshing.h:840-875) on the device: while
+ * mesh mesh_id has fewer than min_vertices vertices it is replaced by its
+ * midpoint subdivision (the vertices, faces and order of acvd_subdivide, one
+ * level at a time on the context's GPU), and an indicator set with
+ * acvd_set_indicator is carried along, each midpoint taking the mean of its
+ * two ends (vtkDiscreteRemeshing.h:700-745).  Fixed items keep their ids.
+ * *levels receives the number of levels applied (0 when the mesh already has
+ * enough vertices).  ACVD_ERR_STATE when initial labels are set (they label
+ * the input's vertices), ACVD_ERR_LIMIT past 2^27 vertices.  The subdivided
+ * mesh is what acvd_get_input() returns and what the next build clusters. */
+int acvd_subdivide_input(acvd_ctx* ctx, int32_t mesh_id, int64_t min_vertices, int32_t* levels);
+
+/* The input mesh mesh_id as the context holds it (vtkDiscreteRemeshing's
+ * GetInput(), the subdivided mesh after acvd_subdivide_input): sizes, then
+ * xyz[3*nv] and tri[3*nf] (either may be NULL). */
+int acvd_get_input_sizes(const acvd_ctx* ctx, int32_t mesh_id, int64_t* nv, int64_t* nf);
+int acvd_get_input(const acvd_ctx* ctx, int32_t mesh_id, double* xyz, int32_t* tri);
+
 /* ---- output mesh ---------------------------------------------------------- */
 /* BuildDelaunayTriangulation (vtkDiscreteRemeshing.h:1002-1152, vertex
  * clustering branch), i.e. what GetOutput() (:61) returns after Remesh():

@@ -107,3 +107,63 @@ def test_remesh_subdivides_a_coarse_input(gpu_ready):
     np.testing.assert_array_equal(r.GetClustering(), ctx.labels(0))
     assert r.GetOutput().GetNumberOfPoints() == K
     r.Delete()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(meshes()))
+def test_device_subdivision_equals_restatement(gpu_ready, name):
+    """acvd_subdivide_input, one level: the restatement's vertices, faces and
+    order, bit for bit; the indicator's midpoints are the means of the ends."""
+    x, t = meshes()[name]
+    ind = np.random.default_rng(5).random(len(x)) + 0.5
+    ctx = acvd_amd.Context()
+    ctx.set_input(x, t)
+    ctx.set_indicator(0, ind)
+    assert ctx.subdivide_input(0, len(x) + 1) == 1
+    ox, ot = ctx.input(0)
+    rx, rt, rpar = subdivide_restated(x, t)
+    np.testing.assert_array_equal(ox, rx)
+    np.testing.assert_array_equal(ot, rt)
+    np.testing.assert_array_equal(ctx.indicator(0), np.concatenate([ind, 0.5 * (ind[rpar[:, 0]] + ind[rpar[:, 1]])]))
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mesh,min_vertices,levels", [("ico", 600, 2), ("torus", 150000, 2), ("cube", 10, 0)])
+def test_device_subdivision_levels_equal_host_chain(gpu_ready, mesh, min_vertices, levels):
+    """Several levels on the device against the host subdivision applied
+    level by level (and the indicator carried along the parents)."""
+    if mesh == "ico":
+        x, t = meshgen.icosphere(1, 8, 0.05)
+    elif mesh == "torus":
+        x, t = meshgen.torus(128, 96, seed=2)
+    else:
+        x, t = meshgen.cube(3, 2, 1e-4)
+    x = np.asarray(x, np.float64)
+    ind = np.random.default_rng(6).random(len(x))
+    ctx = acvd_amd.Context()
+    ctx.set_input(x, t)
+    ctx.set_indicator(0, ind)
+    assert ctx.subdivide_input(0, min_vertices) == levels
+    hx, ht, hind = x, np.asarray(t, np.int32), ind
+    for _ in range(levels):
+        hx, ht, par = subdivide(hx, ht)
+        hind = np.concatenate([hind, 0.5 * (hind[par[:, 0]] + hind[par[:, 1]])])
+    ox, ot = ctx.input(0)
+    print(f"{mesh}: {len(x)} -> {len(ox)} vertices in {levels} levels", flush=True)
+    assert len(ox) >= min_vertices or levels == 0
+    np.testing.assert_array_equal(ox, hx)
+    np.testing.assert_array_equal(ot, ht)
+    np.testing.assert_array_equal(ctx.indicator(0), hind)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_device_subdivision_rejects_initial_labels(gpu_ready):
+    x, t = meshgen.icosphere(1, 8, 0.05)
+    ctx = acvd_amd.Context()
+    ctx.set_input(x, t)
+    ctx.set_initial_labels(0, np.zeros(len(x), np.int32))
+    with pytest.raises(acvd_amd.AcvdError):
+        ctx.subdivide_input(0, 1000)
+    ctx.close()
