@@ -173,6 +173,9 @@ hipError_t launch_ring_adj(hipStream_t s, int64_t n, const int32_t* row_ptr, con
 hipError_t launch_curvature(hipStream_t s, int64_t off, int64_t n, const double* pos, const int32_t* tri,
                             const int32_t* row_ptr, const int32_t* col, const int32_t* vfptr, const int32_t* vfcol,
                             double* out);
+hipError_t launch_sample_topology(hipStream_t s, int64_t off, int64_t n, const double* pos, const int32_t* tri,
+                                  const int32_t* vfptr, const int32_t* vfcol, const int32_t* row_ptr, int32_t* ring,
+                                  int32_t* ring_len, int32_t* key, double* area);
 hipError_t launch_item_build(hipStream_t s, const SweepArgs& a, const double* pos, const int32_t* tri_global,
                              const int32_t* vfptr, const int32_t* vfcol, const double* indicator,
                              double* raw_w, float4* xw, float* q);

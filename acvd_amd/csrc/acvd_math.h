@@ -58,6 +58,20 @@ ACVD_HD double dsqrt(double x) {
 #endif
 }
 
+// Triangle area in FP64 from the squared edge lengths a, b, c:
+// sqrt(|4ac - (a - b + c)^2|) / 4 (Common/vtkSurface.cxx:1280-1283); the
+// same operations on the host and the device.
+ACVD_HD double tri_area_f64(const double* p1, const double* p2, const double* p3) {
+  const double a = (p1[0] - p2[0]) * (p1[0] - p2[0]) + (p1[1] - p2[1]) * (p1[1] - p2[1]) +
+                   (p1[2] - p2[2]) * (p1[2] - p2[2]);
+  const double b = (p2[0] - p3[0]) * (p2[0] - p3[0]) + (p2[1] - p3[1]) * (p2[1] - p3[1]) +
+                   (p2[2] - p3[2]) * (p2[2] - p3[2]);
+  const double c = (p3[0] - p1[0]) * (p3[0] - p1[0]) + (p3[1] - p1[1]) * (p3[1] - p1[1]) +
+                   (p3[2] - p1[2]) * (p3[2] - p1[2]);
+  const double d = 4.0 * a * c - (a - b + c) * (a - b + c);
+  return 0.25 * dsqrt(__builtin_fabs(d));
+}
+
 ACVD_HD int64_t to_fixed(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __double2ll_rn(v);

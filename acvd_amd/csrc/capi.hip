@@ -22,6 +22,7 @@
 #include "acvd_ctx.h"
 #include "host_mesh.h"
 #include "host_sampling.h"
+#include "host_parallel.h"
 
 using namespace acvd;
 
@@ -109,6 +110,10 @@ struct acvd_ctx {
   DBuf<Ctrl> d_ctrl;
   DBuf<int32_t> d_item_mesh, d_clus_mesh;
   DBuf<int32_t> d_row_ptr, d_col, d_tri;
+  // kept from the build for the reference's initial sampling: FP64 positions,
+  // vertex -> face rows (faces in increasing id)
+  DBuf<double> d_pos;
+  DBuf<int32_t> d_vfptr, d_vfcol;
   DBuf<float4> d_xw;
   DBuf<float> d_q;
   DBuf<int32_t> d_labels, d_sizes, d_stamp, d_cand_pass, d_pending;
@@ -405,7 +410,8 @@ int build(acvd_ctx* c) {
   HIP_TRY(hipMemcpyAsync(c->d_mask_all.p, ones.data(), M * sizeof(int32_t), hipMemcpyHostToDevice, s));
   HIP_TRY(c->d_tri.alloc(3 * F));
   HIP_TRY(hipMemcpyAsync(c->d_tri.p, tri.data(), 3 * F * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  DBuf<double> d_pos, d_ind, d_raw;
+  DBuf<double> d_ind, d_raw;
+  DBuf<double>& d_pos = c->d_pos;
   HIP_TRY(d_pos.alloc(3 * N));
   HIP_TRY(hipMemcpyAsync(d_pos.p, pos.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
   if (any_ind || any_curv) {
@@ -414,7 +420,9 @@ int build(acvd_ctx* c) {
   }
   // ---- CSR
   {
-    DBuf<int32_t> deg2, vfdeg, tptr, vfptr, tfill, vffill, tcol, vfcol, deg;
+    DBuf<int32_t> deg2, vfdeg, tptr, tfill, vffill, tcol, deg;
+    DBuf<int32_t>& vfptr = c->d_vfptr;
+    DBuf<int32_t>& vfcol = c->d_vfcol;
     HIP_TRY(deg2.alloc(N + 1));
     HIP_TRY(vfdeg.alloc(N + 1));
     HIP_TRY(tptr.alloc(N + 1));
@@ -942,13 +950,61 @@ int minimize_energy(acvd_ctx* c) {
   return finish_energy_log(c);
 }
 
+// The reference's initial sampling (type 1) of mesh h on a built context:
+// the rings in edge-creation order and the vertex areas come from the device
+// (k_sample_topology over the build's adjacency and vertex -> face rows), the
+// weights (indicator^gradation, clamp) and the region growing, sequential by
+// nature, run on the host.  Same labels as reference_initial_sampling.
+int device_reference_sampling(acvd_ctx* c, const HostMesh& h, const double* ind, int32_t* lab) {
+  hipStream_t s = c->stream;
+  const int64_t nv = h.nv;
+  // the shuffled order depends on nv alone: drawn on another thread while
+  // the device builds the rings and areas
+  SamplingOrder order;
+  std::thread shuffler([&order, nv] { sampling_order(nv, &order); });
+  struct Join {
+    std::thread& t;
+    ~Join() {
+      if (t.joinable()) t.join();
+    }
+  } join{shuffler};
+  DBuf<int32_t> ring, len, key;
+  DBuf<double> area;
+  HIP_TRY(ring.alloc(size_t(c->E2)));
+  HIP_TRY(len.alloc(size_t(nv)));
+  HIP_TRY(key.alloc(c->d_vfcol.n));
+  HIP_TRY(area.alloc(size_t(nv)));
+  HIP_TRY(launch_sample_topology(s, h.item_off, nv, c->d_pos.p, c->d_tri.p, c->d_vfptr.p, c->d_vfcol.p,
+                                 c->d_row_ptr.p, ring.p, len.p, key.p, area.p));
+  huge_vector<int32_t> h_ring(static_cast<size_t>(h.n_adj)), h_len(static_cast<size_t>(nv)),
+      rp(static_cast<size_t>(nv + 1));
+  huge_vector<double> w(static_cast<size_t>(nv));
+  huge_vector<int64_t> rbeg(static_cast<size_t>(nv));
+  HIP_TRY(hipMemcpyAsync(h_ring.data(), ring.p + h.adj_off, size_t(h.n_adj) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                         s));
+  HIP_TRY(hipMemcpyAsync(h_len.data(), len.p, size_t(nv) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(w.data(), area.p, size_t(nv) * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(rp.data(), c->d_row_ptr.p + h.item_off, size_t(nv + 1) * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  parallel_for(nv, [&](int64_t v0, int64_t v1) {
+    for (int64_t v = v0; v < v1; ++v) rbeg[size_t(v)] = int64_t(rp[size_t(v)]) - h.adj_off;
+  });
+  sampling_weights(nv, h.params.metric, ind, h.params.gradation, w.data());
+  shuffler.join();
+  if (!region_growing(nv, rbeg.data(), h_len.data(), h_ring.data(), w.data(), h.params.number_of_clusters, lab,
+                      int64_t(h.fixed.size()), h.fixed.data(), &order))
+    return fail(ACVD_ERR_STATE, "initial sampling failed");
+  return ACVD_OK;
+}
+
 int initial_labels(acvd_ctx* c) {
   hipStream_t s = c->stream;
   SweepArgs a = c->args();
   for (auto& h : c->meshes) {
     const int32_t K = h.params.number_of_clusters;
     if (!h.init_labels.empty() || h.params.initial_sampling != 1) {
-      std::vector<int32_t> lab(h.nv);
+      huge_vector<int32_t> lab(h.nv);
       if (h.init_labels.empty() && h.params.initial_sampling >= 2) {
         // the reference's sampling types 0 (one random item per cluster) and
         // -1 (item k for cluster k)
@@ -963,10 +1019,7 @@ int initial_labels(acvd_ctx* c) {
         // the reference's region growing, then local ids -> global
         // the indicator of the weights: the set one, else the computed curvature
         const std::vector<double>& ind = !h.indicator.empty() ? h.indicator : h.curvature;
-        if (!reference_initial_sampling(h.nv, h.xyz.data(), h.nf, h.tri.data(), h.params.metric,
-                                        ind.empty() ? nullptr : ind.data(), h.params.gradation, K,
-                                        lab.data(), int64_t(h.fixed.size()), h.fixed.data()))
-          return fail(ACVD_ERR_STATE, "initial sampling failed");
+        if (int rc = device_reference_sampling(c, h, ind.empty() ? nullptr : ind.data(), lab.data())) return rc;
         for (int64_t i = 0; i < h.nv; ++i) lab[i] = lab[i] == K ? kNull : int32_t(lab[i] + h.clus_off);
       } else {
         for (int64_t i = 0; i < h.nv; ++i) {

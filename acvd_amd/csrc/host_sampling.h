@@ -2,7 +2,10 @@
 #pragma once
 
 #include <cstdint>
+#include <random>
 #include <vector>
+
+#include "host_parallel.h"
 
 namespace acvd {
 
@@ -26,6 +29,29 @@ void vertex_topology(int64_t nv, int64_t nf, const int32_t* tri, VertexTopology*
 bool reference_initial_sampling(int64_t nv, const double* xyz, int64_t nf, const int32_t* tri, int metric,
                                 const double* indicator, double gradation, int32_t K, int32_t* labels,
                                 int64_t n_fixed = 0, const int32_t* fixed = nullptr);
+
+// The reference's sampling weights from the vertex areas in w (FP64, summed
+// in GetVertexNeighbourFaces order): times indicator^gradation (isotropic:
+// whenever an indicator is set; QEM: when gradation > 0), then clamped to
+// [avg/R, avg*R], R = 1e5 (isotropic) or 1e4 (QEM).  In place.
+void sampling_weights(int64_t nv, int metric, const double* indicator, double gradation, double* w);
+
+// The shuffled item order of ComputeInitialRandomSampling
+// (vtkUniformClustering.h:1202-1207: std::mt19937 seeded 0, then
+// swap(Items[i], Items[rng() % n]) for i = n-1 .. 1) and the generator after
+// it (the second half of the sampling draws on).  It depends on n alone.
+struct SamplingOrder {
+  huge_vector<int32_t> items;
+  std::mt19937 rng;
+};
+void sampling_order(int64_t n, SamplingOrder* out);
+
+// ComputeInitialRandomSampling (vtkUniformClustering.h:1177-1316) given the
+// rings, ring[rbeg[v] .. rbeg[v] + rlen[v]) in edge-creation order, and the
+// weights.  labels receives 0..K-1, K = null.  order: the shuffled order
+// for nv items (sampling_order), or null to compute it here; it is used up.
+bool region_growing(int64_t nv, const int64_t* rbeg, const int32_t* rlen, const int32_t* ring, const double* w,
+                    int32_t K, int32_t* labels, int64_t n_fixed, const int32_t* fixed, SamplingOrder* order = nullptr);
 
 // InitSamples with sampling type 0 (vtkUniformClustering.h:1123-1159): one
 // item per cluster drawn with vtkMath::Random seeded 5000 (an item already

@@ -512,6 +512,91 @@ __global__ void k_curvature(int64_t off, int64_t n, const double* __restrict__ p
   out[v] = ind;
 }
 
+namespace {
+
+// Topology and weights of the reference's initial sampling for the vertices
+// [off, off + n) of one mesh, one thread per vertex, from the vertex -> face
+// rows (faces in increasing id; a face listed twice for a vertex, or whose
+// first two vertices coincide, counts once / not at all):
+//  - ring: the neighbours in edge-creation order (edges (v0,v1), (v1,v2),
+//    (v2,v0) of each face in turn, a neighbour at its first sighting,
+//    Common/vtkSurfaceBase.cxx:1057-1068, 1166-1221), written at the row's
+//    place in the adjacency (row_ptr: it holds every distinct neighbour, so
+//    the ring fits), vertex ids local to the mesh; ring_len its length;
+//  - area: the sum of a third of each face's area in the order
+//    GetVertexNeighbourFaces lists the faces (the faces of each ring
+//    neighbour in turn, a face at its first appearance,
+//    vtkSurfaceBase.cxx:964-984).  A face's place is the ring index of its
+//    first vertex in the ring (key, stored beside the face in the row), then
+//    its id.  FP64 in the host's order of operations.
+__global__ void k_sample_topology(int64_t off, int64_t n, const double* __restrict__ pos,
+                                  const int32_t* __restrict__ tri, const int32_t* __restrict__ vfptr,
+                                  const int32_t* __restrict__ vfcol, const int32_t* __restrict__ row_ptr,
+                                  int32_t* __restrict__ ring, int32_t* __restrict__ ring_len, int32_t* __restrict__ key,
+                                  double* __restrict__ area) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t v = int32_t(off + i);
+  int32_t* rg = ring + row_ptr[v];
+  const int32_t k0 = vfptr[v], k1 = vfptr[v + 1];
+  int32_t cnt = 0, prev = -1;
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t f = vfcol[k];
+    if (f == prev) continue;
+    prev = f;
+    const int32_t* t = tri + 3 * int64_t(f);
+    if (t[0] == t[1]) continue;
+    for (int j = 0; j < 3; ++j) {
+      const int32_t a = t[j], b = t[j == 2 ? 0 : j + 1];
+      if (a == b) continue;
+      const int32_t u = a == v ? b : (b == v ? a : -1);
+      if (u < 0) continue;
+      bool seen = false;
+      for (int32_t q = 0; q < cnt; ++q)
+        if (rg[q] == u) {
+          seen = true;
+          break;
+        }
+      if (!seen) rg[cnt++] = u;
+    }
+  }
+  ring_len[i] = cnt;
+  prev = -1;
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t f = vfcol[k];
+    const int32_t* t = tri + 3 * int64_t(f);
+    int32_t kk = -1;
+    if (f != prev && t[0] != t[1])
+      for (int32_t r = 0; r < cnt; ++r)
+        if (t[0] == rg[r] || t[1] == rg[r] || t[2] == rg[r]) {
+          kk = r;
+          break;
+        }
+    prev = f;
+    key[k] = kk;
+  }
+  double sum = 0.0;
+  for (int32_t r = 0; r < cnt; ++r)
+    for (int32_t k = k0; k < k1; ++k) {
+      if (key[k] != r) continue;
+      const int32_t* t = tri + 3 * int64_t(vfcol[k]);
+      sum += tri_area_f64(pos + 3 * int64_t(t[0]), pos + 3 * int64_t(t[1]), pos + 3 * int64_t(t[2])) / 3.0;
+    }
+  area[i] = sum;
+  for (int32_t r = 0; r < cnt; ++r) rg[r] -= int32_t(off);
+}
+
+}  // namespace
+
+hipError_t launch_sample_topology(hipStream_t s, int64_t off, int64_t n, const double* pos, const int32_t* tri,
+                                  const int32_t* vfptr, const int32_t* vfcol, const int32_t* row_ptr, int32_t* ring,
+                                  int32_t* ring_len, int32_t* key, double* area) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample_topology, dim3(grid_for(n)), dim3(kBlock), 0, s, off, n, pos, tri, vfptr, vfcol,
+                     row_ptr, ring, ring_len, key, area);
+  return hipGetLastError();
+}
+
 hipError_t launch_curvature(hipStream_t s, int64_t off, int64_t n, const double* pos, const int32_t* tri,
                             const int32_t* row_ptr, const int32_t* col, const int32_t* vfptr, const int32_t* vfcol,
                             double* out) {

@@ -76,11 +76,14 @@ def test_cpp_dropin_compiles_links_and_fails_loudly_without_device():
     assert r.stdout.startswith("fail") and "no HIP device" in r.stdout
 
 
-@pytest.mark.parametrize("case", ["ico-iso", "cube-qem", "torus-iso-graded", "torus-qem-graded", "small-K-N"])
+@pytest.mark.parametrize("case", ["ico-iso", "cube-qem", "torus-iso-graded", "torus-qem-graded", "small-K-N",
+                                  "large-K3000", "large-K300-graded"])
 def test_reference_initial_sampling_equals_oracle(case):
     """The product's host restatement of the reference's default initial
     sampling (no device needed) gives the oracle's labels exactly: same
-    topology order, same FP64 weights, same std::mt19937 shuffle."""
+    topology order, same FP64 weights, same std::mt19937 shuffle.  The
+    meshes of 2^18 items and more ("large-*", 294,912 items) take the
+    speculative parallel growth, whose regions must be the serial loop's."""
     from acvd_amd.remeshing import reference_initial_sampling
     from oracle import pyoracle as po
     ind, grad = None, 0.0
@@ -90,6 +93,12 @@ def test_reference_initial_sampling_equals_oracle(case):
         (xyz, tri), metric, K = acvd_amd.meshgen.cube(20, 2, 1e-4), acvd_amd.METRIC_QEM, 120
     elif case == "small-K-N":
         (xyz, tri), metric, K = acvd_amd.meshgen.icosphere(2, 1, 0.0), acvd_amd.METRIC_ISOTROPIC, 100
+    elif case.startswith("large"):
+        xyz, tri = acvd_amd.meshgen.torus(768, 384, 5)
+        metric = acvd_amd.METRIC_QEM
+        K = 3000 if case == "large-K3000" else 300
+        if case.endswith("graded"):
+            ind, grad = acvd_amd.meshgen.mean_curvature(xyz, tri), 1.5
     else:
         xyz, tri = acvd_amd.meshgen.torus(96, 48, 3)
         ind, grad = acvd_amd.meshgen.mean_curvature(xyz, tri), 1.5
