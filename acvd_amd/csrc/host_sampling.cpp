@@ -351,8 +351,8 @@ void grow_regions_parallel(int64_t n, const int32_t* items, const int64_t* rbeg,
     const int32_t c = std::min(first_conflict.load(std::memory_order_relaxed), nc);
     for (int32_t i = t; i < nc; i += T)
       for (int32_t x : spec[size_t(i)].lab) {
-        if (i < c) labels[x] = top_id - i;
-        owner[size_t(x)] = INT32_MAX;
+        if (i < c) labels[x] = top_id - i;  // the prefix's regions are disjoint
+        __atomic_store_n(&owner[size_t(x)], INT32_MAX, __ATOMIC_RELAXED);
       }
   };
   auto team = [&](int t) {

@@ -78,3 +78,27 @@ def test_degenerate_faces_equal_the_host_pass(gpu_ready):
     reached = want < K
     assert reached.sum() > n // 2
     np.testing.assert_array_equal(got[reached], want[reached])
+
+
+def test_large_mesh_with_fixed_clusters(gpu_ready):
+    """2^18 items and more take the speculative parallel growth; fixed
+    clusters start at their items and the grown regions take the ids after
+    them."""
+    xyz, tri = meshgen.torus(640, 480, seed=6)  # 307,200 items
+    K = 4000
+    fixed = np.random.default_rng(3).choice(len(xyz), 40, replace=False).astype(np.int32)
+    ctx = acvd_amd.Context()
+    ctx.set_input(xyz, tri)
+    ctx.set_params(0, number_of_clusters=K, metric=QEM, initial_sampling=0)
+    ctx.set_fixed_clusters(0, fixed)
+    ctx.initial_sampling()
+    got = ctx.labels(0)
+    po.set_fixed(fixed)
+    try:
+        want = po.RefMesh(xyz, tri).initial_sampling(QEM, K)
+    finally:
+        po.set_fixed(None)
+    reached = want < K
+    np.testing.assert_array_equal(got[reached], want[reached])
+    np.testing.assert_array_equal(got[fixed], np.arange(len(fixed)))
+    ctx.close()

@@ -77,7 +77,7 @@ def test_cpp_dropin_compiles_links_and_fails_loudly_without_device():
 
 
 @pytest.mark.parametrize("case", ["ico-iso", "cube-qem", "torus-iso-graded", "torus-qem-graded", "small-K-N",
-                                  "large-K3000", "large-K300-graded"])
+                                  "large-K3000", "large-K300-graded", "large-K-half"])
 def test_reference_initial_sampling_equals_oracle(case):
     """The product's host restatement of the reference's default initial
     sampling (no device needed) gives the oracle's labels exactly: same
@@ -96,7 +96,7 @@ def test_reference_initial_sampling_equals_oracle(case):
     elif case.startswith("large"):
         xyz, tri = acvd_amd.meshgen.torus(768, 384, 5)
         metric = acvd_amd.METRIC_QEM
-        K = 3000 if case == "large-K3000" else 300
+        K = {"large-K3000": 3000, "large-K-half": len(xyz) // 2}.get(case, 300)
         if case.endswith("graded"):
             ind, grad = acvd_amd.meshgen.mean_curvature(xyz, tri), 1.5
     else:
