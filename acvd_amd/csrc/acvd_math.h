@@ -897,6 +897,26 @@ ACVD_HD bool propose_item(const SweepView& v, int64_t i, int32_t lab, int connex
   return target != kNull && iso_accept(ta, best);
 }
 
+// Re-check, in a later accept round of the pass, of a stale proposal that the
+// isotropic rule decides (the isotropic metric, or a QEM mesh while no T term
+// is active), item i of cluster lab proposing target b: the move is kept
+// while it is still one the evaluate could propose (b still a neighbouring
+// cluster, the same guards as propose_item) and still improving; no better
+// target is searched (the next pass's evaluate does that).  Null items are
+// proposed again in full.
+ACVD_HD bool iso_recheck(const SweepView& v, int64_t i, int32_t lab, int32_t b, int connexity) {
+  if (v.sizes[lab] <= 1 || is_frozen(v, lab) || is_frozen(v, b)) return false;
+  const int32_t r0 = v.row_ptr[i], r1 = v.row_ptr[i + 1];
+  bool adjacent = false;
+  for (int32_t k = r0; k < r1 && !adjacent; ++k) adjacent = v.labels[v.col[k]] == b;
+  if (!adjacent) return false;
+  if (connexity && ring_check(v, i, lab)) return false;
+  const float px = v.xw[4 * i], py = v.xw[4 * i + 1], pz = v.xw[4 * i + 2], pw = v.xw[4 * i + 3];
+  const ClusterGeom ga = v.geom[lab];
+  if (!can_leave(pw, ga)) return false;
+  return iso_accept(leave_term(px, py, pz, pw, ga), join_term(px, py, pz, pw, v.geom[b]));
+}
+
 // Per-mesh convergence state, advanced once per pass by the controller with
 // the reference's phase logic (vtkUniformClustering.h:724-830).
 enum MeshPhase : int32_t {

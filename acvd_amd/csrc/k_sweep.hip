@@ -867,7 +867,10 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
         use_pool = METRIC == 1 && lab != kNull && qem_mode(st);
         if (use_pool)
           kind = pool_screen(v, i, lab, st.connexity, nb, &s, &open, &ntask);
-        else  // no FP64 solve: the null-item rule or the isotropic one
+        else if (lab != kNull) {  // the isotropic rule: the move itself is checked again
+          target = b;
+          live = iso_recheck(v, i, lab, b, st.connexity);
+        } else  // the null-item rule
           live = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st.connexity, st.constrained, dm.level,
                                                         dm.scales, &target);
       }

@@ -238,6 +238,11 @@ struct Replay {
           const bool stale = (lab != acvd::kNull && cl_round[lab] >= since[t]) || cl_round[target[t]] >= since[t];
           if (!stale) continue;
           since[t] = L * acvd::kMatchRounds + round;
+          if (lab != acvd::kNull && (metric == 0 || acvd::qem_mode(st) == 0)) {
+            // the isotropic rule: the move itself is checked again
+            target[t] = acvd::iso_recheck(view(), cand[t], lab, target[t], st.connexity) ? target[t] : -3;
+            continue;
+          }
           int32_t nt = acvd::kNull;
           target[t] = propose(cand[t], &nt) ? nt : -3;
         }
