@@ -69,6 +69,7 @@ class Stats(ctypes.Structure):
         ("ctx_reeval_live", ctypes.c_int64),
         ("ctx_reevaluated", ctypes.c_int64),
         ("ctx_accepted_constrained", ctypes.c_int64),
+        ("ctx_rechecked", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -87,6 +87,7 @@ struct Ctrl {
   int32_t mods[kMatchRounds][kSlices];  // applied moves of this pass, per accept round
   int32_t live[kMatchRounds][kSlices];  // candidates registered by the re-evaluate of round r
   int32_t stale[kMatchRounds][kSlices]; // candidates re-evaluated in round r
+  int32_t recheck[kMatchRounds][kSlices]; // stale candidates whose own move was re-checked in round r
   unsigned long long evaluated;  // boundary items evaluated (all passes)
   unsigned long long proposed;   // candidates proposed (all passes)
   unsigned long long accepted;   // moves applied (all passes)
@@ -96,6 +97,7 @@ struct Ctrl {
   unsigned long long walked;      // candidate entries read by the round kernels, dead ones included
   unsigned long long reeval_live; // live candidates registered by re-evaluate rounds
   unsigned long long reevaluated; // candidates re-evaluated (their clusters moved)
+  unsigned long long rechecked;   // stale candidates whose own move was re-checked (the isotropic rule)
   // Per-pass counts of the last kPassLogCap passes (ring indexed by pass
   // number): the per-loop console output reads them between pass chunks.
   int32_t pass_log[kPassLogCap][kPassLogFields];

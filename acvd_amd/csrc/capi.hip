@@ -1392,6 +1392,7 @@ int acvd_process_clustering(acvd_ctx* ctx) {
     S.ctx_walked = int64_t(ctx->last_ctrl.walked);
     S.ctx_reeval_live = int64_t(ctx->last_ctrl.reeval_live);
     S.ctx_reevaluated = int64_t(ctx->last_ctrl.reevaluated);
+    S.ctx_rechecked = int64_t(ctx->last_ctrl.rechecked);
     S.ctx_accepted_constrained = int64_t(ctx->last_ctrl.accepted_constrained);
     S.converge_ms = ms;
     S.build_ms = ctx->build_ms;

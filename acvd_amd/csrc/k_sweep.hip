@@ -835,13 +835,13 @@ __device__ inline int block_sum(int v) {
 
 template <int METRIC, bool kExtras, bool kCount, class Reg>
 __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemPool* pool, bool valid, int64_t e,
-                                    int round, Reg&& reg, int* n_live, int* n_stale) {
+                                    int round, Reg&& reg, int* n_live, int* n_stale, int* n_recheck) {
   // the candidate record in one round trip
   const int32_t b = valid ? a.cand_tgt[e] : -1;
   const int32_t i_e = valid ? a.cand[e] : 0;
   const int32_t lab_e = valid ? a.cand_lab[e] : kNull;
   const int32_t since = valid ? a.cand_since[e] : 0;
-  bool live = false, stale = false, use_pool = false;
+  bool live = false, stale = false, use_pool = false, recheck = false;
   int32_t i = 0, lab = kNull, target = b, m = 0;
   unsigned long long key = 0;
   int32_t nb[kRingCache];
@@ -869,6 +869,7 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
           kind = pool_screen(v, i, lab, st.connexity, nb, &s, &open, &ntask);
         else if (lab != kNull) {  // the isotropic rule: the move itself is checked again
           target = b;
+          recheck = true;
           live = iso_recheck(v, i, lab, b, st.connexity);
         } else  // the null-item rule
           live = propose_item<METRIC == 1 ? 2 : METRIC>(v, i, lab, st.connexity, st.constrained, dm.level,
@@ -891,7 +892,8 @@ __device__ inline void reeval_entry(const SweepArgs& a, const SweepView& v, QemP
   }
   if constexpr (kCount) {
     *n_live += live;
-    *n_stale += stale;
+    *n_stale += stale && !recheck;
+    *n_recheck += recheck;
   }
 }
 
@@ -912,11 +914,11 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   if (round_was_empty(a, round - 1) || past) return;
   const SweepView v = make_view<kExtras>(a);
   QemPool* pool = &pools[METRIC == 1 ? threadIdx.x >> 6 : 0];
-  int n_live = 0, n_stale = 0;
+  int n_live = 0, n_stale = 0, n_recheck = 0;
   walk_candidates(a, w, &si, [&](bool valid, int64_t e) {
     reeval_entry<METRIC, kExtras, kCount>(
         a, v, pool, valid, e, round, [&](int32_t c, unsigned long long key) { keyagg_min(&agg, c, key); },
-        &n_live, &n_stale);
+        &n_live, &n_stale, &n_recheck);
     keyagg_flush(&agg, a.prio);
   });
   // candidates registered and re-evaluated in this round (statistics of the
@@ -924,9 +926,11 @@ __global__ __launch_bounds__(kBlock) void k_reeval(SweepArgs a, int round) {
   if constexpr (kCount) {
     const int tl = block_sum(n_live);
     const int ts = block_sum(n_stale);
+    const int tr = block_sum(n_recheck);
     if (threadIdx.x == 0) {
       if (tl) atomicAdd(&a.ctrl->live[round][blockIdx.x % kSlices], tl);
       if (ts) atomicAdd(&a.ctrl->stale[round][blockIdx.x % kSlices], ts);
+      if (tr) atomicAdd(&a.ctrl->recheck[round][blockIdx.x % kSlices], tr);
     }
   }
 }
@@ -1147,19 +1151,21 @@ __global__ void k_controller(SweepArgs a) {
       s_part[1][threadIdx.x >> 6] = p;
     }
   }
-  __shared__ int s_live, s_stale;
+  __shared__ int s_live, s_stale, s_recheck;
   if (threadIdx.x < 64) {  // lane k reads slice k of the move counters, wave sums
     int dr[kMatchRounds];
-    int lv = 0, st = 0;
+    int lv = 0, st = 0, rc = 0;
     for (int r = 0; r < kMatchRounds; ++r) {
       dr[r] = a.ctrl->mods[r][threadIdx.x];
       lv += a.ctrl->live[r][threadIdx.x];
       st += a.ctrl->stale[r][threadIdx.x];
+      rc += a.ctrl->recheck[r][threadIdx.x];
     }
     for (int off = 32; off > 0; off >>= 1) {
       for (int r = 0; r < kMatchRounds; ++r) dr[r] += __shfl_xor(dr[r], off);
       lv += __shfl_xor(lv, off);
       st += __shfl_xor(st, off);
+      rc += __shfl_xor(rc, off);
     }
     if (threadIdx.x == 0) {
       int d = 0;
@@ -1170,6 +1176,7 @@ __global__ void k_controller(SweepArgs a) {
       s_mods = d;
       s_live = lv;
       s_stale = st;
+      s_recheck = rc;
     }
   }
   __syncthreads();
@@ -1211,6 +1218,7 @@ __global__ void k_controller(SweepArgs a) {
       a.ctrl->mods[r][threadIdx.x] = 0;
       a.ctrl->live[r][threadIdx.x] = 0;
       a.ctrl->stale[r][threadIdx.x] = 0;
+      a.ctrl->recheck[r][threadIdx.x] = 0;
     }
   if (threadIdx.x == 0 && (any_sweep & 2)) {
     const unsigned long long pass = a.ctrl->passes;
@@ -1237,6 +1245,7 @@ __global__ void k_controller(SweepArgs a) {
       a.ctrl->walked += (unsigned long long)s_props * (unsigned long long)(2 * walked - 1);
       a.ctrl->reeval_live += (unsigned long long)s_live;
       a.ctrl->reevaluated += (unsigned long long)s_stale;
+      a.ctrl->rechecked += (unsigned long long)s_recheck;
     }
     a.ctrl->any_sweep = any_sweep & 1;
     a.ctrl->seg_walk = s_props <= kSegWalkMax ? 1 : 0;

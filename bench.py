@@ -110,6 +110,10 @@ def accept_bytes(st, N, E2):
       per re-evaluated candidate an evaluation: 72 B of item record and row
         bounds, 8 B per CSR entry (neighbour id and label), three cluster
         records of 156 B;
+      per re-checked candidate (the isotropic rule checks its own move
+        again: ctx_rechecked) 88 B (item position and weight, row bounds,
+        the size of its cluster, centroid and W of both clusters) and 8 B
+        per CSR entry (the target among the neighbour labels);
       per move 252 B (label, target and moved-list words; per cluster the
         fixed-point W and S read and written, size, stamp and round tag read
         and written, the derived centroid and W written) and, while the QEM
@@ -118,6 +122,7 @@ def accept_bytes(st, N, E2):
     deg = E2 / N
     return (4.0 * st["ctx_walked"] + 24.0 * (st["ctx_proposed"] + st["ctx_reeval_live"])
             + 36.0 * st["ctx_reeval_live"] + (72.0 + 8.0 * deg + 3 * 156.0) * st["ctx_reevaluated"]
+            + (88.0 + 8.0 * deg) * st["ctx_rechecked"]
             + 252.0 * st["ctx_accepted"] + 480.0 * st["ctx_accepted_constrained"])
 
 
@@ -190,7 +195,8 @@ def run_gpu_leg(args):
     ctx.set_timing(True, counters=False)
     ctx.process()
     tst = dict(ctx.stats(0))
-    for k in ("ctx_rounds", "ctx_walked", "ctx_reeval_live", "ctx_reevaluated", "ctx_accepted_constrained"):
+    for k in ("ctx_rounds", "ctx_walked", "ctx_reeval_live", "ctx_reevaluated", "ctx_rechecked",
+              "ctx_accepted_constrained"):
         tst[k] = cst[k]
     ctx.set_timing(False)
     names = ["k_scan", "k_eval<QEM>", "accept rounds (k_apply, k_reeval, k_tile_update)", "k_controller"]
@@ -215,7 +221,7 @@ def run_gpu_leg(args):
         log(f"{names[k]}: {launches[k]} launches, total {ms:.1f} ms, avg {ms / launches[k] * 1e3:.1f} us" + extra)
     log(f"work items/pass {work_per:.0f}, proposals/pass {prop_per:.0f}, accept rounds {tst['ctx_rounds']}, "
         f"candidate entries walked {tst['ctx_walked']}, re-registered {tst['ctx_reeval_live']}, re-evaluated "
-        f"{tst['ctx_reevaluated']}, moves {tst['ctx_accepted']} ({tst['ctx_accepted_constrained']} constrained); "
+        f"{tst['ctx_reevaluated']}, re-checked {tst['ctx_rechecked']}, moves {tst['ctx_accepted']} ({tst['ctx_accepted_constrained']} constrained); "
         f"timed-step total {sum(tst['kernel_ms']):.1f} ms")
     dom = max((0, 1, 2), key=lambda k: tst["kernel_ms"][k])
     gd = groups[names[dom]]

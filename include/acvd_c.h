@@ -122,13 +122,15 @@ typedef struct acvd_stats {
    * [3] controller; launches count one per pass for [2] */
   double kernel_ms[4];
   int64_t kernel_launches[4];
-  /* accept-round work, context-wide, all passes (the last two only with
-   * timing on, acvd_set_timing; 0 otherwise) */
+  /* accept-round work, context-wide, all passes (the re-registered,
+   * re-evaluated and re-checked counts only with timing on,
+   * acvd_set_timing; 0 otherwise) */
   int64_t ctx_rounds;       /* accept rounds that walked the candidate list */
   int64_t ctx_walked;       /* candidate entries read by the round kernels (dead ones included) */
   int64_t ctx_reeval_live;  /* live candidates registered again by re-evaluate rounds */
-  int64_t ctx_reevaluated;  /* candidates re-evaluated because one of their clusters moved */
+  int64_t ctx_reevaluated;  /* candidates evaluated again in full because one of their clusters moved */
   int64_t ctx_accepted_constrained; /* moves applied while mesh 0 was constrained (QEM derives solve) */
+  int64_t ctx_rechecked;    /* stale candidates whose own move was checked again (the isotropic rule) */
 } acvd_stats;
 
 /* ---- library -------------------------------------------------------------- */
