@@ -488,6 +488,10 @@ int64_t orc_replay_energy_log(int64_t cap, double* energy, double* final_energy)
 
 // path: bit 0 generic QEM rule, bit 1 every round (no end after an empty round),
 // bit 16 record the energy log
+// moves of the last replay to a cluster none of the item's neighbours held
+static int64_t g_replay_nonadj = 0;
+int64_t orc_replay_nonadjacent() { return g_replay_nonadj; }
+
 int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* col, const float* xw,
                         const float* q, const double* scales3, int K, int metric, int level, uint32_t seed,
                         int unconstrained, int max_loops, int max_conv, const int32_t* init, int32_t* out,
@@ -524,6 +528,7 @@ int orc_replay_run_path(int path, int64_t n, const int32_t* rp, const int32_t* c
   r.run(init);
   g_replay_log = r.energy_log;
   g_replay_final = r.final_energy;
+  g_replay_nonadj = r.stats.non_adjacent_moves;
   for (int64_t i = 0; i < n; ++i) out[i] = r.labels[i] == acvd::kNull ? K : r.labels[i];
   if (stats6) {
     stats6[0] = r.stats.loops;

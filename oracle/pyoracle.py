@@ -76,6 +76,8 @@ def lib():
                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P]
         L.orc_replay_run_path.argtypes = [ctypes.c_int] + L.orc_replay_run.argtypes
         L.orc_replay_energy_log.restype = ctypes.c_int64
+        L.orc_replay_nonadjacent.restype = ctypes.c_int64
+        L.orc_replay_nonadjacent.argtypes = []
         L.orc_replay_energy_log.argtypes = [ctypes.c_int64, P, P]
         L.orc_set_energy_log.argtypes = [ctypes.c_int]
         L.orc_set_frozen.argtypes = [P, ctypes.c_int64]
@@ -339,6 +341,7 @@ def replay_run(row_ptr, col, xw, q, scales, K, metric, init, level=3, seed=0, un
                               unconstrained, max_loops, max_conv, _p(ini), _p(out), _p(st))
     keys = ["loops", "convergences", "total_moves", "evaluated", "proposed", "last_disconnected"]
     stats = dict(zip(keys, (int(v) for v in st)))
+    stats["non_adjacent_moves"] = int(lib().orc_replay_nonadjacent())
     if energy_log:
         m = lib().orc_replay_energy_log(0, None, None)
         e = np.empty(m, np.float64)

@@ -23,6 +23,9 @@ namespace oracle {
 struct ReplayStats {
   int64_t loops = 0, convergences = 0, total_moves = 0, evaluated = 0, proposed = 0;
   int32_t last_disconnected = 0;
+  // moves to a cluster no neighbour of the item belongs to (the reference
+  // moves items across cut edges only: must stay 0)
+  int64_t non_adjacent_moves = 0;
 };
 
 struct Replay {
@@ -177,6 +180,9 @@ struct Replay {
   }
 
   void apply_move(int64_t i, int32_t lab, int32_t b, int32_t L, int32_t rtag) {
+    bool adjacent = false;
+    for (int32_t k = rp[i]; k < rp[i + 1] && !adjacent; ++k) adjacent = labels[col[k]] == b;
+    if (!adjacent) stats.non_adjacent_moves++;
     labels[i] = b;
     const float w = xw[4 * i + 3];
     const int64_t iw = acvd::fix_w(w, sc);

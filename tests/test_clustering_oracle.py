@@ -203,3 +203,26 @@ def test_a_round_without_moves_ends_the_pass(case):
                           all_rounds=True)
     np.testing.assert_array_equal(a, b)
     assert sa == sb
+
+
+@pytest.mark.parametrize("metric,unconstrained,first_rounds,K", [(po.ISO, 0, 0, 400), (po.QEM, 1, 12, 400),
+                                                                  (po.ISO, 0, 12, 1500), (po.QEM, 1, 12, 1500),
+                                                                  (po.ISO, 0, 12, 3000)])
+def test_every_move_goes_to_a_neighbouring_cluster(metric, unconstrained, first_rounds, K):
+    """The reference moves items across cut edges only
+    (vtkUniformClustering.h:860-995): every move of the replayed schedule,
+    the evaluate's proposals and the later rounds' re-checked and
+    re-evaluated ones alike, takes the item to a cluster one of its
+    neighbours holds (the device applies the same moves: its labels equal
+    the replay's bit for bit in the GPU tests).  Clusters of 2-4 items
+    (K = 1500, 3000 on 6,144 items) churn enough that a re-check without
+    its neighbour test moves items to clusters no longer adjacent."""
+    xyz, tri = meshgen.torus(96, 64, seed=8)
+    H = meshgen.mean_curvature(xyz, tri)
+    m = po.RefMesh(xyz, tri)
+    rp, col, xw, q, sc = po.device_inputs(xyz, tri, metric, H, 1.5)
+    init = m.initial_sampling(metric, K, 1.5, H)
+    lab, st = po.replay_run(rp, col, xw, q, sc, K, metric, init, seed=5, unconstrained=unconstrained,
+                            first_rounds=first_rounds)
+    assert st["total_moves"] > 1000
+    assert st["non_adjacent_moves"] == 0
