@@ -243,7 +243,7 @@ def start_small_gaps():
     (cube 400 x 400 per face, 960,002 vertices, QEM, K = 20,000): the serial
     restatement (oracle A) of each starts in a host thread from a fully
     labelled initial clustering; run_small_gaps runs the GPU on the same
-    labels and reports the final-energy gaps (the north star's <= 0.1 %)."""
+    labels and reports the final-energy gaps (the bar: <= 0.1 %)."""
     import threading
     from acvd_amd import meshgen
     from oracle import pyoracle as po
